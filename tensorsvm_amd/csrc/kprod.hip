@@ -1,0 +1,540 @@
+// Fused labelled-RBF kernel product on gfx950 MFMA.
+//
+//   W[i, c] = y1_i * sum_j exp(-gamma |x1_i - x2_j|^2) * (y2_j A[j, c])
+//
+// Replaces the reference's per-8192^2-tile pipeline (tensorsvm-train.cu:2476-2567):
+// vecnorm (2569-2581) + XIJ GEMM (2506-2518) + rbf_kergen (2598-2612, which writes the
+// tile to HBM) + the K*A GEMM (2537-2555).  Here one kernel keeps the exponentiated
+// tile in registers, flash-attention style:
+//
+//   S^T (32 j x 32 i) = Qop_j . Pop_i^T      MFMA (the distance, in log2 units)
+//   E = exp2(S^T)                             VALU v_exp_f32, packed to the MFMA A operand
+//   O (i x c) += E . A'_j                     MFMA, O stays in registers over all j
+//
+// The distance is folded into one GEMM with augmented operands (norms ride along as
+// extra K columns):  Pop_i . Qop_j = 2 g' x_i.x_j - g'|x_i|^2 - g'|x_j|^2 with
+// g' = gamma*log2(e), so exp2 gives the kernel directly.  In the BF16 form each FP32
+// value is split into bf16 hi + lo parts and the GEMM carries hi*hi + hi*lo + lo*hi
+// (about 16 mantissa bits for the distance) while the big product E.A' is plain BF16
+// with FP32 accumulation.  The FP32 form uses v_mfma_f32_32x32x2f32 throughout.
+//
+// Layout produced by the prep kernels (padding is zero-filled):
+//   BF16: Pop  [m_pad][KD] bf16, Qop [n_pad][KD] bf16, KD = roundup(3d+4, 16)
+//         AT   [n_pad/32][r_pad][32] bf16 = y2_j A[j,c], j permuted inside each chunk
+//              so one 16-byte LDS read is a whole MFMA B operand.
+//   FP32: Pop  [m_pad][KF] f32, Qop [n_pad][KF] f32, KF = roundup(d+2, 2)
+//         Af   [n_pad][r_pad] f32 = y2_j A[j,c]
+#include "common.h"
+#include "internal.h"
+
+namespace tsvm {
+
+// position of j (0..31) inside a permuted 32-j chunk row (see kprod_bf16 B operand)
+__device__ __host__ inline int perm32(int j) {
+  int kap = j >> 4, jj = j & 15;
+  int half = (jj & 7) >> 2;
+  int t = ((jj >> 3) << 2) | (jj & 3);
+  return ((2 * kap + half) << 3) | t;
+}
+
+__device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
+
+// rows side: [xh, xh, xl, 1, 1, bh, bl, 0...]
+__global__ void prep_rows_bf16(const float* __restrict__ X, int64_t m, int64_t m_pad, int d,
+                               int KD, double gl2, __bf16* __restrict__ P) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= m_pad) return;
+  __bf16* p = P + i * KD;
+  const __bf16 z = (__bf16)0.0f;
+  if (i >= m) {
+    for (int t = 0; t < KD; t++) p[t] = z;
+    return;
+  }
+  const float* x = X + i * d;
+  double nrm = 0.0;
+  for (int t = 0; t < d; t++) {
+    float v = x[t];
+    nrm += (double)v * v;
+    __bf16 hi, lo;
+    split_bf16(v, hi, lo);
+    p[t] = hi;
+    p[d + t] = hi;
+    p[2 * d + t] = lo;
+  }
+  float b = (float)(-gl2 * nrm);
+  __bf16 bh, bl;
+  split_bf16(b, bh, bl);
+  p[3 * d] = (__bf16)1.0f;
+  p[3 * d + 1] = (__bf16)1.0f;
+  p[3 * d + 2] = bh;
+  p[3 * d + 3] = bl;
+  for (int t = 3 * d + 4; t < KD; t++) p[t] = z;
+}
+
+// columns side: [uh, ul, uh, bh, bl, 1, 1, 0...] with u = 2 g' x
+__global__ void prep_cols_bf16(const float* __restrict__ X, int64_t n, int64_t n_pad, int d,
+                               int KD, double gl2, __bf16* __restrict__ Q) {
+  int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n_pad) return;
+  __bf16* q = Q + j * KD;
+  const __bf16 z = (__bf16)0.0f;
+  if (j >= n) {
+    for (int t = 0; t < KD; t++) q[t] = z;
+    return;
+  }
+  const float* x = X + j * d;
+  double nrm = 0.0;
+  for (int t = 0; t < d; t++) {
+    float v = x[t];
+    nrm += (double)v * v;
+    float u = (float)(2.0 * gl2 * (double)v);
+    __bf16 hi, lo;
+    split_bf16(u, hi, lo);
+    q[t] = hi;
+    q[d + t] = lo;
+    q[2 * d + t] = hi;
+  }
+  float b = (float)(-gl2 * nrm);
+  __bf16 bh, bl;
+  split_bf16(b, bh, bl);
+  q[3 * d] = bh;
+  q[3 * d + 1] = bl;
+  q[3 * d + 2] = (__bf16)1.0f;
+  q[3 * d + 3] = (__bf16)1.0f;
+  for (int t = 3 * d + 4; t < KD; t++) q[t] = z;
+}
+
+// AT[chunk][c][perm32(jj)] = bf16(y_j A[j, c]); a 32(j) x 32(c) tile per block (256 thr)
+__global__ void prep_A_bf16(const float* __restrict__ A, int64_t lda, const float* __restrict__ y,
+                            int64_t n, int64_t r, int64_t r_pad, __bf16* __restrict__ AT) {
+  __shared__ float tile[32][33];
+  const int64_t chunk = blockIdx.x;      // j chunk
+  const int64_t cb = blockIdx.y * 32;    // column block
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int jj = ty; jj < 32; jj += 8) {
+    int64_t j = chunk * 32 + jj, c = cb + tx;
+    float v = 0.0f;
+    if (j < n && c < r) v = y[j] * A[j * lda + c];
+    tile[jj][tx] = v;
+  }
+  __syncthreads();
+  for (int cc = ty; cc < 32; cc += 8) {
+    int64_t c = cb + cc;
+    if (c >= r_pad) continue;
+    AT[(chunk * r_pad + c) * 32 + perm32(tx)] = (__bf16)tile[tx][cc];
+  }
+}
+
+__global__ void prep_rows_f32(const float* __restrict__ X, int64_t m, int64_t m_pad, int d, int KF,
+                              double gl2, float* __restrict__ P) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= m_pad) return;
+  float* p = P + i * KF;
+  if (i >= m) {
+    for (int t = 0; t < KF; t++) p[t] = 0.0f;
+    return;
+  }
+  const float* x = X + i * d;
+  double nrm = 0.0;
+  for (int t = 0; t < d; t++) {
+    nrm += (double)x[t] * x[t];
+    p[t] = x[t];
+  }
+  p[d] = 1.0f;
+  p[d + 1] = (float)(-gl2 * nrm);
+  for (int t = d + 2; t < KF; t++) p[t] = 0.0f;
+}
+
+__global__ void prep_cols_f32(const float* __restrict__ X, int64_t n, int64_t n_pad, int d, int KF,
+                              double gl2, float* __restrict__ Q) {
+  int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n_pad) return;
+  float* q = Q + j * KF;
+  if (j >= n) {
+    for (int t = 0; t < KF; t++) q[t] = 0.0f;
+    return;
+  }
+  const float* x = X + j * d;
+  double nrm = 0.0;
+  for (int t = 0; t < d; t++) {
+    nrm += (double)x[t] * x[t];
+    q[t] = (float)(2.0 * gl2 * (double)x[t]);
+  }
+  q[d] = (float)(-gl2 * nrm);
+  q[d + 1] = 1.0f;
+  for (int t = d + 2; t < KF; t++) q[t] = 0.0f;
+}
+
+__global__ void prep_A_f32(const float* __restrict__ A, int64_t lda, const float* __restrict__ y,
+                           int64_t n, int64_t n_pad, int64_t r, int64_t r_pad, float* __restrict__ Af) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= n_pad * r_pad) return;
+  int64_t j = idx / r_pad, c = idx - j * r_pad;
+  float v = 0.0f;
+  if (j < n && c < r) v = y[j] * A[j * lda + c];
+  Af[idx] = v;
+}
+
+// -------------------------------------------------------------------------------------
+// BF16 main kernel.  256 threads = 4 waves; wave w owns rows [64w, 64w+64) of the
+// block's 256 rows (two 32-row MFMA blocks) and all BN = 32*NT columns of the block.
+// Per 32-j chunk: S^T via KD/16 MFMA 32x32x16 per row block, exp2 on 16 values per
+// lane per row block, then 2*NT MFMA 32x32x16 per row block for O += E.A'.
+// LDS (double-buffered): Qop chunk 32 x KD, AT chunk BN x 32 (row stride padded).
+// -------------------------------------------------------------------------------------
+template <int NT, int KD>
+__global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
+                                                  const __bf16* __restrict__ Q,
+                                                  const __bf16* __restrict__ AT,
+                                                  const float* __restrict__ y1,
+                                                  float* __restrict__ W, int64_t ldw, int64_t m,
+                                                  int64_t r, int64_t n_chunks, int64_t r_pad) {
+  constexpr int BN = NT * 32;
+  constexpr int KS = KD / 16;
+  constexpr int QS = KD + 8;   // LDS row stride (bf16) of the Qop chunk
+  constexpr int AS = 32 + 8;   // LDS row stride (bf16) of the AT chunk
+  constexpr int QV = 32 * KD / 8;   // 16-byte vectors per Qop chunk
+  constexpr int AV = BN * 32 / 8;   // 16-byte vectors per AT chunk
+  constexpr int NV = (QV + AV + 255) / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 sQ[2][32 * QS];
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][BN * AS];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, h = lane >> 5;
+  const int64_t rowbase = (int64_t)blockIdx.x * 256 + wave * 64;
+  const int64_t c0 = (int64_t)blockIdx.y * BN;
+
+  // S^T B operand: this wave's P rows, resident in registers for the whole j loop.
+  bf16x8_t pb[2][KS];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++)
+      pb[rb][ks] = *(const bf16x8_t*)(P + (rowbase + 32 * rb + l32) * KD + ks * 16 + 8 * h);
+
+  f32x16_t acc[2][NT];
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[rb][ct][q] = 0.0f;
+
+  uint4 stage[NV];
+  auto gload = [&](int64_t t) {
+    const uint4* gq = (const uint4*)(Q + t * 32 * KD);
+    const uint4* ga = (const uint4*)(AT + (t * r_pad + c0) * 32);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      int idx = tid + v * 256;
+      if (idx < QV) stage[v] = gq[idx];
+      else if (idx < QV + AV) stage[v] = ga[idx - QV];
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      int idx = tid + v * 256;
+      if (idx < QV) {
+        int row = idx / (KD / 8), col = (idx % (KD / 8)) * 8;
+        *(uint4*)&sQ[buf][row * QS + col] = stage[v];
+      } else if (idx < QV + AV) {
+        int a = idx - QV;
+        int row = a / 4, col = (a % 4) * 8;
+        *(uint4*)&sA[buf][row * AS + col] = stage[v];
+      }
+    }
+  };
+
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int64_t t = 0; t < n_chunks; t++) {
+    const int buf = (int)(t & 1);
+    if (t + 1 < n_chunks) gload(t + 1);
+    // distance tile S^T (32 j x 32 i) for both row blocks
+    f32x16_t s[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) s[rb][q] = 0.0f;
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      bf16x8_t qa = *(const bf16x8_t*)&sQ[buf][l32 * QS + ks * 16 + 8 * h];
+      s[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa, pb[0][ks], s[0], 0, 0, 0);
+      s[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa, pb[1][ks], s[1], 0, 0, 0);
+    }
+    // E = exp2(S^T): lane holds i = l32 and j = 8(q/4) + 4h + q%4; registers 8k..8k+7
+    // form the A operand of K-half k (j in 16k + {4h..4h+3, 8+4h..8+4h+3}).
+    bf16x8_t e[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+      for (int kap = 0; kap < 2; kap++)
+#pragma unroll
+        for (int tt = 0; tt < 8; tt++)
+          e[rb][kap][tt] = (__bf16)__builtin_amdgcn_exp2f(s[rb][8 * kap + tt]);
+    // O += E . A'_j
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++) {
+#pragma unroll
+      for (int kap = 0; kap < 2; kap++) {
+        bf16x8_t bv = *(const bf16x8_t*)&sA[buf][(ct * 32 + l32) * AS + (2 * kap + h) * 8];
+        acc[0][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[0][kap], bv, acc[0][ct], 0, 0, 0);
+        acc[1][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[1][kap], bv, acc[1][ct], 0, 0, 0);
+      }
+    }
+    if (t + 1 < n_chunks) sstore(buf ^ 1);
+    __syncthreads();
+  }
+  // epilogue: W[i, c] = y1_i * O[i, c]; acc register q holds row 8(q/4) + 4h + q%4
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      int64_t i = rowbase + 32 * rb + 8 * (q / 4) + 4 * h + (q % 4);
+      if (i >= m) continue;
+      float yi = y1[i];
+#pragma unroll
+      for (int ct = 0; ct < NT; ct++) {
+        int64_t c = c0 + ct * 32 + l32;
+        if (c < r) W[i * ldw + c] = yi * acc[rb][ct][q];
+      }
+    }
+  }
+}
+
+// -------------------------------------------------------------------------------------
+// FP32 main kernel.  256 threads = 4 waves; wave w owns rows [32w, 32w+32) of the block's
+// 128 rows and all BN = 32*NT columns.  MFMA 32x32x2 f32 throughout.  LDS (single
+// buffer, register prefetch): Qop chunk 32 x KF, A' chunk 32 x BN (row stride padded).
+// -------------------------------------------------------------------------------------
+template <int NT, int KF>
+__global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
+                                                 const float* __restrict__ Q,
+                                                 const float* __restrict__ Af,
+                                                 const float* __restrict__ y1,
+                                                 float* __restrict__ W, int64_t ldw, int64_t m,
+                                                 int64_t r, int64_t n_chunks, int64_t r_pad) {
+  constexpr int BN = NT * 32;
+  constexpr int KSF = KF / 2;
+  constexpr int AS = BN + 8;          // LDS row stride (floats) of the A' chunk
+  constexpr int QV = 32 * KF / 4;     // float4 vectors per Qop chunk
+  constexpr int AV = 32 * BN / 4;     // float4 vectors per A' chunk
+  constexpr int NV = (QV + AV + 255) / 256;
+  __shared__ __attribute__((aligned(16))) float sQ[32 * KF];
+  __shared__ __attribute__((aligned(16))) float sA[32 * AS];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, h = lane >> 5;
+  const int64_t rowbase = (int64_t)blockIdx.x * 128 + wave * 32;
+  const int64_t c0 = (int64_t)blockIdx.y * BN;
+
+  float pb[KSF];
+#pragma unroll
+  for (int ks = 0; ks < KSF; ks++) pb[ks] = P[(rowbase + l32) * KF + 2 * ks + h];
+
+  f32x16_t acc[NT];
+#pragma unroll
+  for (int ct = 0; ct < NT; ct++)
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[ct][q] = 0.0f;
+
+  float4 stage[NV];
+  auto gload = [&](int64_t t) {
+    const float4* gq = (const float4*)(Q + t * 32 * KF);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      int idx = tid + v * 256;
+      if (idx < QV) {
+        stage[v] = gq[idx];
+      } else if (idx < QV + AV) {
+        int a = idx - QV;
+        int row = a / (BN / 4), col = (a % (BN / 4)) * 4;
+        stage[v] = *(const float4*)(Af + (t * 32 + row) * r_pad + c0 + col);
+      }
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      int idx = tid + v * 256;
+      if (idx < QV) {
+        *(float4*)&sQ[idx * 4] = stage[v];
+      } else if (idx < QV + AV) {
+        int a = idx - QV;
+        int row = a / (BN / 4), col = (a % (BN / 4)) * 4;
+        *(float4*)&sA[row * AS + col] = stage[v];
+      }
+    }
+  };
+
+  gload(0);
+  for (int64_t t = 0; t < n_chunks; t++) {
+    sstore();
+    __syncthreads();
+    if (t + 1 < n_chunks) gload(t + 1);
+    f32x16_t s;
+#pragma unroll
+    for (int q = 0; q < 16; q++) s[q] = 0.0f;
+#pragma unroll
+    for (int ks = 0; ks < KSF; ks++)
+      s = __builtin_amdgcn_mfma_f32_32x32x2f32(sQ[l32 * KF + 2 * ks + h], pb[ks], s, 0, 0, 0);
+    float e[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) e[q] = __builtin_amdgcn_exp2f(s[q]);
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        int j = 8 * (q / 4) + 4 * h + (q % 4);
+        acc[ct] = __builtin_amdgcn_mfma_f32_32x32x2f32(e[q], sA[j * AS + ct * 32 + l32], acc[ct],
+                                                        0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    int64_t i = rowbase + 8 * (q / 4) + 4 * h + (q % 4);
+    if (i >= m) continue;
+    float yi = y1[i];
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++) {
+      int64_t c = c0 + ct * 32 + l32;
+      if (c < r) W[i * ldw + c] = yi * acc[ct][q];
+    }
+  }
+}
+
+// -------------------------------------------------------------------------------------
+// host launchers
+// -------------------------------------------------------------------------------------
+template <int NT, int KD>
+static void launch_bf16(const __bf16* P, const __bf16* Q, const __bf16* AT, const float* y1,
+                        float* W, int64_t ldw, int64_t m, int64_t r, int64_t n_chunks,
+                        int64_t r_pad, int64_t m_pad, hipStream_t s) {
+  dim3 grid((unsigned)(m_pad / 256), (unsigned)(r_pad / (NT * 32)));
+  hipLaunchKernelGGL((kprod_bf16<NT, KD>), grid, dim3(256), 0, s, P, Q, AT, y1, W, ldw, m, r,
+                     n_chunks, r_pad);
+}
+template <int NT, int KF>
+static void launch_f32(const float* P, const float* Q, const float* Af, const float* y1,
+                       float* W, int64_t ldw, int64_t m, int64_t r, int64_t n_chunks,
+                       int64_t r_pad, int64_t m_pad, hipStream_t s) {
+  dim3 grid((unsigned)(m_pad / 128), (unsigned)(r_pad / (NT * 32)));
+  hipLaunchKernelGGL((kprod_f32<NT, KF>), grid, dim3(256), 0, s, P, Q, Af, y1, W, ldw, m, r,
+                     n_chunks, r_pad);
+}
+
+tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t m,
+                               const float* X2, const float* y2, int64_t n, int64_t d,
+                               const float* A, int64_t lda, int64_t r, double gamma,
+                               int precision, float* W, int64_t ldw) {
+  if (m <= 0 || r <= 0) return TSVM_OK;
+  if (n < 0 || d <= 0 || lda < r || ldw < r || !(gamma >= 0.0))
+    return fail(c, TSVM_ERR_ARG, "kernel_product: bad shape m=%ld n=%ld d=%ld r=%ld", (long)m,
+                (long)n, (long)d, (long)r);
+  hipStream_t s = c->stream;
+  const double gl2 = gamma * TSVM_LOG2E;
+  const int64_t n_pad = round_up(n > 0 ? n : 1, 32), n_chunks = n_pad / 32;
+  if (precision == TSVM_PREC_BF16) {
+    // operand width: the smallest compiled size holding [hi, hi, lo, norm terms]
+    static const int kds[] = {16, 32, 48, 64, 80, 96, 112, 128, 176, 256};
+    int KD = 0;
+    for (int v : kds)
+      if (v >= 3 * d + 4) { KD = v; break; }
+    if (KD == 0)
+      return fail(c, TSVM_ERR_ARG, "kernel_product(bf16): d=%ld exceeds 84 features", (long)d);
+    const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : 4);
+    const int64_t BN = NT * 32, m_pad = round_up(m, 256), r_pad = round_up(r, BN);
+    void *P, *Q, *AT;
+    TSVM_HIP(c, c->ws.get("kp_P", m_pad * KD * 2, &P));
+    TSVM_HIP(c, c->ws.get("kp_Q", n_pad * KD * 2, &Q));
+    TSVM_HIP(c, c->ws.get("kp_A", n_pad * r_pad * 2, &AT));
+    hipLaunchKernelGGL(prep_rows_bf16, dim3((unsigned)ceil_div(m_pad, 256)), dim3(256), 0, s, X1, m,
+                       m_pad, (int)d, KD, gl2, (__bf16*)P);
+    hipLaunchKernelGGL(prep_cols_bf16, dim3((unsigned)ceil_div(n_pad, 256)), dim3(256), 0, s, X2, n,
+                       n_pad, (int)d, KD, gl2, (__bf16*)Q);
+    hipLaunchKernelGGL(prep_A_bf16, dim3((unsigned)n_chunks, (unsigned)(r_pad / 32)), dim3(256), 0,
+                       s, A, lda, y2, n, r, r_pad, (__bf16*)AT);
+    TSVM_HIP(c, hipGetLastError());
+    const __bf16 *Pb = (const __bf16*)P, *Qb = (const __bf16*)Q, *Ab = (const __bf16*)AT;
+#define KP_BF16_CASE(kd)                                                                   \
+  case kd:                                                                                 \
+    if (NT == 1) launch_bf16<1, kd>(Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s); \
+    else if (NT == 2) launch_bf16<2, kd>(Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s); \
+    else launch_bf16<4, kd>(Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s);       \
+    break;
+    switch (KD) {
+      KP_BF16_CASE(16)
+      KP_BF16_CASE(32)
+      KP_BF16_CASE(48)
+      KP_BF16_CASE(64)
+      KP_BF16_CASE(80)
+      KP_BF16_CASE(96)
+      KP_BF16_CASE(112)
+      KP_BF16_CASE(128)
+      KP_BF16_CASE(176)
+      KP_BF16_CASE(256)
+      default:
+        return fail(c, TSVM_ERR_ARG, "kernel_product(bf16): KD=%d not compiled", KD);
+    }
+#undef KP_BF16_CASE
+  } else if (precision == TSVM_PREC_FP32) {
+    static const int kfs[] = {4, 8, 12, 16, 20, 24, 34, 56, 64, 96, 130};
+    int KF = 0;
+    for (int v : kfs)
+      if (v >= d + 2) { KF = v; break; }
+    if (KF == 0)
+      return fail(c, TSVM_ERR_ARG, "kernel_product(fp32): d=%ld exceeds 128 features", (long)d);
+    const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : (r <= 128 ? 4 : 8));
+    const int64_t BN = NT * 32, m_pad = round_up(m, 128), r_pad = round_up(r, BN);
+    void *P, *Q, *Af;
+    TSVM_HIP(c, c->ws.get("kp_P", m_pad * KF * 4, &P));
+    TSVM_HIP(c, c->ws.get("kp_Q", n_pad * KF * 4, &Q));
+    TSVM_HIP(c, c->ws.get("kp_A", n_pad * r_pad * 4, &Af));
+    hipLaunchKernelGGL(prep_rows_f32, dim3((unsigned)ceil_div(m_pad, 256)), dim3(256), 0, s, X1, m,
+                       m_pad, (int)d, KF, gl2, (float*)P);
+    hipLaunchKernelGGL(prep_cols_f32, dim3((unsigned)ceil_div(n_pad, 256)), dim3(256), 0, s, X2, n,
+                       n_pad, (int)d, KF, gl2, (float*)Q);
+    hipLaunchKernelGGL(prep_A_f32, dim3((unsigned)ceil_div(n_pad * r_pad, 256)), dim3(256), 0, s,
+                       A, lda, y2, n, n_pad, r, r_pad, (float*)Af);
+    TSVM_HIP(c, hipGetLastError());
+    const float *Pf = (const float*)P, *Qf = (const float*)Q, *Aff = (const float*)Af;
+#define KP_F32_CASE(kf)                                                                        \
+  case kf:                                                                                     \
+    if (NT == 1) launch_f32<1, kf>(Pf, Qf, Aff, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s);  \
+    else if (NT == 2) launch_f32<2, kf>(Pf, Qf, Aff, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s); \
+    else if (NT == 4) launch_f32<4, kf>(Pf, Qf, Aff, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s); \
+    else launch_f32<8, kf>(Pf, Qf, Aff, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s);          \
+    break;
+    switch (KF) {
+      KP_F32_CASE(4)
+      KP_F32_CASE(8)
+      KP_F32_CASE(12)
+      KP_F32_CASE(16)
+      KP_F32_CASE(20)
+      KP_F32_CASE(24)
+      KP_F32_CASE(34)
+      KP_F32_CASE(56)
+      KP_F32_CASE(64)
+      KP_F32_CASE(96)
+      KP_F32_CASE(130)
+      default:
+        return fail(c, TSVM_ERR_ARG, "kernel_product(fp32): KF=%d not compiled", KF);
+    }
+#undef KP_F32_CASE
+  } else {
+    return fail(c, TSVM_ERR_ARG, "kernel_product: unknown precision %d", precision);
+  }
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+}  // namespace tsvm

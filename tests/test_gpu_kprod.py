@@ -1,0 +1,63 @@
+"""Fused RBF kernel product (HIP, through the C-ABI) against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from datagen import gaussian_mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(m, n, d, r, gamma, seed):
+    rng = np.random.default_rng(seed)
+    X1 = rng.standard_normal((m, d)).astype(np.float32) * 0.7
+    X2 = rng.standard_normal((n, d)).astype(np.float32) * 0.7
+    y1 = np.where(rng.random(m) < 0.5, 1.0, -1.0).astype(np.float32)
+    y2 = np.where(rng.random(n) < 0.5, 1.0, -1.0).astype(np.float32)
+    A = rng.standard_normal((n, r)).astype(np.float32)
+    return X1, y1, X2, y2, A
+
+
+@pytest.mark.parametrize("m,n,d,r", [(300, 257, 18, 40), (1000, 1000, 18, 32), (513, 700, 54, 100),
+                                     (129, 64, 32, 256), (77, 33, 5, 7), (256, 2048, 18, 512)])
+def test_kprod_fp32(ctx, m, n, d, r):
+    import tensorsvm_amd as T
+    X1, y1, X2, y2, A = _case(m, n, d, r, 0.05, m + n)
+    W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, T.PREC_FP32)
+    ref = O.kernel_product(X1, y1, X2, y2, A, 0.05)
+    scale = np.abs(ref).max()
+    # FP32 products accumulated over n terms: relative to the output scale
+    assert np.abs(W - ref).max() <= 2e-5 * scale * max(1.0, np.sqrt(n / 256))
+
+
+@pytest.mark.parametrize("m,n,d,r", [(300, 257, 18, 40), (1000, 1000, 18, 32), (513, 700, 54, 100),
+                                     (129, 64, 32, 256), (77, 33, 5, 7), (256, 2048, 18, 512)])
+def test_kprod_bf16(ctx, m, n, d, r):
+    import tensorsvm_amd as T
+    X1, y1, X2, y2, A = _case(m, n, d, r, 0.05, m + n + 1)
+    W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, T.PREC_BF16)
+    # oracle on the bf16-rounded right-hand side: the remaining error is the bf16
+    # rounding of the kernel values (relative 2^-9, unbiased) and the split distance
+    import torch
+    Ab = torch.from_numpy(A).to(torch.bfloat16).to(torch.float32).numpy()
+    ref = O.kernel_product(X1, y1, X2, y2, Ab, 0.05)
+    # E = bf16(K) = K (1 + delta), |delta| <= 2^-9 (round to nearest), so the error of an
+    # entry is bounded by 2^-9 sum_j |K_ij A_jc| (plus FP32 accumulation and the ~2^-16
+    # relative error of the split distance), and on average far below that bound.
+    Kabs = O.kernel_product(X1, np.ones_like(y1), X2, np.ones_like(y2), np.abs(Ab), 0.05)
+    err = np.abs(W - ref)
+    bound = 2.0 ** -9 * Kabs
+    assert (err <= 1.05 * bound + 1e-5 * np.abs(ref).max()).all()
+    assert np.median(err / (bound + 1e-30)) < 0.3
+
+
+def test_kprod_gaussian_mixture_gamma(ctx):
+    """Config-shaped data (d=18, gamma=0.05) and a larger gamma; FP32 form vs FP64."""
+    import tensorsvm_amd as T
+    X, y = gaussian_mixture(3000, 18, 1000)
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((3000, 64)).astype(np.float32)
+    for g in (0.05, 1.0):
+        W = ctx.kernel_product(X, y, X, y, A, g, T.PREC_FP32)
+        ref = O.kernel_product(X, y, X, y, A, g)
+        assert np.abs(W - ref).max() <= 3e-5 * np.abs(ref).max()
