@@ -64,6 +64,142 @@ tsvm_status tsvm_kernel_product_dev(tsvm_ctx *ctx, const float *X1, const float 
                                     int64_t d, const float *A, int64_t r, double gamma,
                                     int precision, float *W);
 
+/* Timings and diagnostics of the low-rank factor construction. */
+typedef struct {
+  double t_product1;   /* W = K_Y Omega (s, HIP events on the context's stream) */
+  double t_orth;       /* orthonormalisation of W (s) */
+  double t_product2;   /* Q = K_Y W (s, HIP events) */
+  double t_core;       /* W^T W, C = W^T Q, host eigen-decomposition (s) */
+  double t_factor;     /* U = W X (s) */
+  double t_total;      /* whole call (s) */
+  int64_t realk;       /* eigenvalues > 0 kept (the reference's "real rank", 2198-2211) */
+  double eig_max, eig_min;
+  int orth_passes;     /* Cholesky-QR passes taken */
+} tsvm_lra_info;
+
+/* Rank-k factor U (n x k) of the labelled RBF kernel, U U^T ~ diag(y) K diag(y):
+ *   W = K_Y Omega; W = orth(W); Q = K_Y W; C = W^T Q; C = V diag(w) V^T;
+ *   U = W V diag(sqrt(max(w, 0))).
+ * Replaces LRA (tensorsvm-train.cu:1941-2249; default C_EIG path with one range-finder
+ * pass).  The orthonormaliser is a shifted Cholesky-QR on FP64 MFMA instead of the
+ * reference's TSQR (617-856): U U^T depends only on range(W).  omega: n x k row-major
+ * test matrix, or NULL to draw the reference's gaussrand() stream (tsvm_gaussrand_fill).
+ * U: n x k row-major FP32 output (host). */
+tsvm_status tsvm_lra(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
+                     int64_t k, double gamma, int precision, const float *omega, float *U,
+                     tsvm_lra_info *info);
+/* Device-pointer form: X, y, omega (n x k, consumed: used as workspace) and U are
+ * device pointers; synchronised on return. */
+tsvm_status tsvm_lra_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
+                         int64_t k, double gamma, int precision, float *omega, float *U,
+                         tsvm_lra_info *info);
+
+/* Interior-point solver parameters (the reference's constants are the defaults). */
+#define TSVM_STOP_REF 0 /* mu, |r_x|, |r_y| < 1e-7 (tensorsvm-train.cu:1647) */
+#define TSVM_STOP_GAP 1 /* (pobj-dobj)/(1+|pobj|) <= tol, |r_x|/(1+sqrt n) <= tol,
+                           |r_y| <= tol (1 + |x|_1 / C) */
+#define TSVM_CONVERGED 0
+#define TSVM_ILL_COND 1 /* max D / min D > 1e18 (1653) */
+#define TSVM_MAXIT 2
+#define TSVM_CHOL_FAIL 3 /* capacitance Cholesky failed: current iterate returned (1711) */
+typedef struct {
+  double C;          /* box bound */
+  int kernel_type;   /* 2: RBF initial point (1535-1552); 0: linear one (1553-1579) */
+  int maxit;         /* MAX_MPC_ITER = 100 (62) */
+  int stop_mode;     /* TSVM_STOP_REF or TSVM_STOP_GAP */
+  double tol;        /* for TSVM_STOP_GAP */
+  int precision;     /* TSVM_PREC_FP32: FP64 capacitance; TSVM_PREC_BF16: BF16 MFMA
+                        capacitance + iterative refinement against the FP32 factor */
+  int refine;        /* refinement sweeps per Newton solve (BF16 form), default 1 */
+  int verbose;       /* print the reference's iteration line every 5 iterations */
+} tsvm_ipm_params;
+typedef struct {
+  int iters, reason;
+  double mu, normdx, normdy, pobj, dobj, gap_rel, condD;
+  double y;          /* equality multiplier (= rho = -b at a free SV) */
+  double t_total;    /* s */
+  double t_syrk;     /* s spent building Z^T D^-1 Z (all iterations) */
+  double syrk_tflops;/* algorithmic n k (k+1) flops / t_syrk */
+  double refine_res; /* last relative refinement residual (BF16 form) */
+} tsvm_ipm_info;
+void tsvm_ipm_default_params(tsvm_ipm_params *p);
+/* Mehrotra predictor-corrector for the SVM dual with Q = Z Z^T, every O(n) and O(nk)
+ * step on the GPU.  Replaces mpc (tensorsvm-train.cu:1507-1785), NewtonStep (1794-1822)
+ * and SMWSolve (1824-1840): the Newton system (D + Z Z^T) is solved by
+ * Sherman-Morrison-Woodbury with the capacitance I + Z^T D^-1 Z built on MFMA and
+ * factored by an FP64 Cholesky on the GPU.  Z: n x k labelled factor (FP32, row-major),
+ * a: labels (+-1).  alpha (= X), xi, s: n FP64 outputs (s may be NULL). */
+tsvm_status tsvm_ipm(tsvm_ctx *ctx, const float *Z, const float *a, int64_t n, int64_t k,
+                     const tsvm_ipm_params *prm, double *alpha, double *xi, double *s,
+                     tsvm_ipm_info *info);
+/* Device-pointer form (Z, a, alpha, xi, s on the device; s may be NULL). */
+tsvm_status tsvm_ipm_dev(tsvm_ctx *ctx, const float *Z, const float *a, int64_t n, int64_t k,
+                         const tsvm_ipm_params *prm, double *alpha, double *xi, double *s,
+                         tsvm_ipm_info *info);
+
+/* Whole training run: factor (T=2) or labelled data (T=0), IPM, bias. */
+typedef struct {
+  int64_t k;          /* rank (-k, default 50) */
+  double C;           /* -c (default 1) */
+  double gamma;       /* -g (default 0) */
+  int kernel_type;    /* -t: 2 RBF (LRA + IPM), 0 linear (IPM on diag(y) X) */
+  int precision;      /* TSVM_PREC_FP32 / TSVM_PREC_BF16 (kernel products; -tensorcore) */
+  int stop_mode;      /* TSVM_STOP_REF (default) or TSVM_STOP_GAP */
+  double tol;         /* gap tolerance for TSVM_STOP_GAP */
+  int maxit;          /* IPM iterations (100) */
+  int refine;         /* refinement sweeps (BF16 capacitance) */
+  int verbose;        /* reference-style progress lines on stdout */
+} tsvm_train_params;
+typedef struct {
+  tsvm_lra_info lra;
+  tsvm_ipm_info ipm;
+  double b;           /* bias; rho = -b */
+  double b_std;       /* spread of the per-free-SV bias samples (reference "std b") */
+  int b_samples;      /* free SVs averaged (<= 50 RBF, <= 5000 linear) */
+  int64_t nsv, nfree; /* alpha > 1e-6; and alpha < C - 1e-6 */
+  double t_bias, t_total;  /* s */
+} tsvm_train_info;
+void tsvm_train_default_params(tsvm_train_params *p);
+/* Train on X (n x d row-major FP32) with labels y in {+1,-1}: the RBF branch of main
+ * (tensorsvm-train.cu:957-987: LRA then mpc) or the linear branch (907-918), then the bias
+ * of writemodel (301-331 linear, 353-376 RBF).  omega: n x k test matrix or NULL (the
+ * reference's gaussrand() stream).  alpha: n FP64 output.  w_out (linear only, may be
+ * NULL): d weights. */
+tsvm_status tsvm_train(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
+                       const tsvm_train_params *prm, const float *omega, double *alpha,
+                       double *w_out, tsvm_train_info *info);
+/* Device-pointer form of the RBF branch: X, y, omega (n x k, consumed) and alpha on the
+ * device; the factor stays on the device (no host copies of size n k). */
+tsvm_status tsvm_train_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
+                           const tsvm_train_params *prm, float *omega, double *alpha,
+                           tsvm_train_info *info);
+
+/* Decision values f(t) = sum_i alpha_i y_i exp(-gamma |x_i - t|^2) + b for m test rows
+ * (host pointers; only alpha_i > 1e-6 contribute).  The predictor the reference delegates
+ * to ThunderSVM (README.md:27-37) -- its own predict() (166-270) is dead code -- as a
+ * kernel product with one right-hand side. */
+tsvm_status tsvm_decision_function(tsvm_ctx *ctx, const float *X, const float *y,
+                                   const double *alpha, int64_t n, int64_t d, double gamma,
+                                   double b, const float *T, int64_t m, double *f);
+
+/* LIBSVM text input (libsvmread, tensorsvm-train.cu:1162-1278): dense row-major FP32
+ * features (*inst, n x d) and labels (*labels), malloc'ed; free with tsvm_free.
+ * scale256: divide values by 256 (-scale256).  TSVM_ERR_IO / TSVM_ERR_FORMAT on failure
+ * (the reference prints and returns with partial arrays). */
+tsvm_status tsvm_read_libsvm(const char *path, int scale256, float **labels, float **inst,
+                             int64_t *n, int64_t *d);
+void tsvm_free(void *p);
+/* Label mapping of main (879-896): first label -> +1 (POS), first different -> -1 (NEG).
+ * Returns the count of labels outside {POS, NEG} (left unchanged). */
+int64_t tsvm_map_labels(float *labels, int64_t n, int *pos, int *neg);
+/* Model files of writemodel: RBF LIBSVM text (384-406) and linear LIBLINEAR text
+ * (340-350), formats byte-identical to the reference's fprintf calls. */
+tsvm_status tsvm_write_model_rbf(const char *path, const double *alpha, const float *labels,
+                                 const float *inst, int64_t n, int64_t d, double C,
+                                 double gamma, double b, int pos, int neg);
+tsvm_status tsvm_write_model_linear(const char *path, const double *w, int64_t d, double b,
+                                    int pos, int neg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -509,6 +509,17 @@ static void newton_step(const double *Z, const double *D, const double *L, doubl
   }
   free(r5); free(r7); free(b);
 }
+void orc_newton_direction(const double *Z, const double *a, int64_t n, int64_t k, double C,
+                          const double *X, const double *S, const double *Xi, double *r) {
+  double *D = (double *)malloc(sizeof(double) * n);
+  double *M = (double *)malloc(sizeof(double) * k * k);
+  for (int64_t i = 0; i < n; i++) D[i] = S[i] / X[i] + Xi[i] / (C - X[i]);
+  build_capacitance(Z, D, n, k, M);
+  newton_step(Z, D, M, C, a, X, S, Xi, r, n, k);
+  free(D);
+  free(M);
+}
+
 static double max_step(const double *r, const double *X, const double *S, const double *Xi,
                        double C, int64_t n) {
   const double *dx = r, *ds = r + n + 1, *dxi = r + 2 * n + 1;

@@ -73,6 +73,10 @@ int orc_mpc(const double *Z, const double *a, int64_t n, int64_t k, double C,
 /* One SMW solve (1824-1840): b <- (D + Z Z^T)^{-1} b, M = chol(I + Z^T D^-1 Z). */
 void orc_smw_solve(const double *Z, const double *D, int64_t n, int64_t k, double *b);
 
+/* One NewtonStep (1794-1822) at (X, S, Xi): r = [r_x (n); r_y (1); r_s (n); r_xi (n)]
+ * is overwritten with the step (dx; dy; ds; dxi). */
+void orc_newton_direction(const double *Z, const double *a, int64_t n, int64_t k, double C,
+                          const double *X, const double *S, const double *Xi, double *r);
 /* Bias (writemodel 353-372): mean over the first <= 50 free SVs of
  * y_j - sum_i alpha_i y_j (U_i . U_j).  nfree_used receives the sample count. */
 double orc_bias_rbf(const double *alpha, const float *labels, const double *U,

@@ -66,7 +66,118 @@ def _bind_optional(L):
                 fn.restype = res
 
 
-_EXTRA = []
+class LraInfo(ctypes.Structure):
+    _fields_ = [("t_product1", _D), ("t_orth", _D), ("t_product2", _D), ("t_core", _D),
+                ("t_factor", _D), ("t_total", _D), ("realk", _I64), ("eig_max", _D),
+                ("eig_min", _D), ("orth_passes", _INT)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class IpmParams(ctypes.Structure):
+    _fields_ = [("C", _D), ("kernel_type", _INT), ("maxit", _INT), ("stop_mode", _INT),
+                ("tol", _D), ("precision", _INT), ("refine", _INT), ("verbose", _INT)]
+
+
+class IpmInfo(ctypes.Structure):
+    _fields_ = [("iters", _INT), ("reason", _INT), ("mu", _D), ("normdx", _D), ("normdy", _D),
+                ("pobj", _D), ("dobj", _D), ("gap_rel", _D), ("condD", _D), ("y", _D),
+                ("t_total", _D), ("t_syrk", _D), ("syrk_tflops", _D), ("refine_res", _D)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def ipm_params(C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5, precision=PREC_FP32,
+               refine=1, verbose=0):
+    return IpmParams(C, kernel_type, maxit, stop, tol, precision, refine, verbose)
+
+
+class TrainParams(ctypes.Structure):
+    _fields_ = [("k", _I64), ("C", _D), ("gamma", _D), ("kernel_type", _INT), ("precision", _INT),
+                ("stop_mode", _INT), ("tol", _D), ("maxit", _INT), ("refine", _INT),
+                ("verbose", _INT)]
+
+
+class TrainInfo(ctypes.Structure):
+    _fields_ = [("lra", LraInfo), ("ipm", IpmInfo), ("b", _D), ("b_std", _D), ("b_samples", _INT),
+                ("nsv", _I64), ("nfree", _I64), ("t_bias", _D), ("t_total", _D)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("lra", "ipm")}
+        d["lra"] = self.lra.as_dict()
+        d["ipm"] = self.ipm.as_dict()
+        return d
+
+
+def train_params(k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32, stop=STOP_REF,
+                 tol=1e-5, maxit=100, refine=1, verbose=0):
+    return TrainParams(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose)
+
+
+_EXTRA = [
+    ("tsvm_train", [_P, _P, _P, _I64, _I64, ctypes.POINTER(TrainParams), _P, _P, _P,
+                    ctypes.POINTER(TrainInfo)], None),
+    ("tsvm_train_dev", [_P, _P, _P, _I64, _I64, ctypes.POINTER(TrainParams), _P, _P,
+                        ctypes.POINTER(TrainInfo)], None),
+    ("tsvm_decision_function", [_P, _P, _P, _P, _I64, _I64, _D, _D, _P, _I64, _P], None),
+    ("tsvm_read_libsvm", [ctypes.c_char_p, _INT, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                          ctypes.POINTER(_I64), ctypes.POINTER(_I64)], None),
+    ("tsvm_free", [_P], ctypes.c_int),
+    ("tsvm_map_labels", [_P, _I64, ctypes.POINTER(_INT), ctypes.POINTER(_INT)], _I64),
+    ("tsvm_write_model_rbf", [ctypes.c_char_p, _P, _P, _P, _I64, _I64, _D, _D, _D, _INT, _INT], None),
+    ("tsvm_write_model_linear", [ctypes.c_char_p, _P, _I64, _D, _INT, _INT], None),
+    ("tsvm_ipm", [_P, _P, _P, _I64, _I64, ctypes.POINTER(IpmParams), _P, _P, _P,
+                  ctypes.POINTER(IpmInfo)], None),
+    ("tsvm_ipm_dev", [_P, _P, _P, _I64, _I64, ctypes.POINTER(IpmParams), _P, _P, _P,
+                      ctypes.POINTER(IpmInfo)], None),
+    ("tsvm_lra", [_P, _P, _P, _I64, _I64, _I64, _D, _INT, _P, _P, ctypes.POINTER(LraInfo)], None),
+    ("tsvm_lra_dev", [_P, _P, _P, _I64, _I64, _I64, _D, _INT, _P, _P, ctypes.POINTER(LraInfo)], None),
+]
+
+
+def read_libsvm(path, scale256=False):
+    """LIBSVM text -> (labels float32 (n,), dense features float32 (n, d))."""
+    L = load()
+    lab, inst, n, d = _P(), _P(), _I64(), _I64()
+    st = L.tsvm_read_libsvm(path.encode(), int(scale256), ctypes.byref(lab), ctypes.byref(inst),
+                            ctypes.byref(n), ctypes.byref(d))
+    if st != 0:
+        raise TsvmError(f"tsvm_read_libsvm({path}) failed with status {st}")
+    nn, dd = n.value, d.value
+    labels = np.ctypeslib.as_array(ctypes.cast(lab, ctypes.POINTER(ctypes.c_float)), (max(nn, 1),))[:nn].copy()
+    x = np.ctypeslib.as_array(ctypes.cast(inst, ctypes.POINTER(ctypes.c_float)),
+                              (max(nn * dd, 1),))[:nn * dd].copy().reshape(nn, dd)
+    L.tsvm_free(lab)
+    L.tsvm_free(inst)
+    return labels, x
+
+
+def map_labels(labels):
+    labels = np.ascontiguousarray(labels, dtype=np.float32).copy()
+    pos, neg = _INT(), _INT()
+    bad = load().tsvm_map_labels(labels.ctypes.data_as(_P), labels.size, ctypes.byref(pos),
+                                 ctypes.byref(neg))
+    return labels, pos.value, neg.value, bad
+
+
+def write_model_rbf(path, alpha, labels, inst, C, gamma, b, pos, neg):
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    labels = np.ascontiguousarray(labels, dtype=np.float32)
+    inst = np.ascontiguousarray(inst, dtype=np.float32)
+    n, d = inst.shape
+    st = load().tsvm_write_model_rbf(path.encode(), alpha.ctypes.data_as(_P), labels.ctypes.data_as(_P),
+                                     inst.ctypes.data_as(_P), n, d, C, gamma, b, pos, neg)
+    if st != 0:
+        raise TsvmError(f"tsvm_write_model_rbf failed ({st})")
+
+
+def write_model_linear(path, w, b, pos, neg):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    st = load().tsvm_write_model_linear(path.encode(), w.ctypes.data_as(_P), w.size, b, pos, neg)
+    if st != 0:
+        raise TsvmError(f"tsvm_write_model_linear failed ({st})")
 
 
 def _ptr(a):
@@ -138,6 +249,66 @@ class Context:
                                                  n, d, _ptr(A), r, gamma, precision, _ptr(W)),
                     "tsvm_kernel_product")
         return W
+
+    def lra(self, X, y, k, gamma, precision=PREC_FP32, omega=None):
+        """Rank-k factor U (n x k float32) with U U^T ~ diag(y) K diag(y)."""
+        X, y = _f32(X), _f32(y)
+        n, d = X.shape
+        om = None if omega is None else _f32(omega).reshape(n, k)
+        U = np.empty((n, k), dtype=np.float32)
+        info = LraInfo()
+        self._check(self.lib.tsvm_lra(self.h, _ptr(X), _ptr(y), n, d, k, gamma, precision,
+                                      _ptr(om), _ptr(U), ctypes.byref(info)), "tsvm_lra")
+        return U, info
+
+    def lra_dev(self, X, y, n, d, k, gamma, precision, omega, U):
+        info = LraInfo()
+        self._check(self.lib.tsvm_lra_dev(self.h, _ptr(X), _ptr(y), n, d, k, gamma, precision,
+                                          _ptr(omega), _ptr(U), ctypes.byref(info)), "tsvm_lra_dev")
+        return info
+
+    def ipm(self, Z, a, C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5,
+            precision=PREC_FP32, refine=1, verbose=0):
+        """Dual IPM on Q = Z Z^T; returns (alpha, xi, s, info)."""
+        Z, a = _f32(Z), _f32(a)
+        n, k = Z.shape
+        al, xi, s = np.empty(n), np.empty(n), np.empty(n)
+        prm = ipm_params(C, kernel_type, maxit, stop, tol, precision, refine, verbose)
+        info = IpmInfo()
+        self._check(self.lib.tsvm_ipm(self.h, _ptr(Z), _ptr(a), n, k, ctypes.byref(prm), _ptr(al),
+                                      _ptr(xi), _ptr(s), ctypes.byref(info)), "tsvm_ipm")
+        return al, xi, s, info
+
+    def train(self, X, y, k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32,
+              stop=STOP_REF, tol=1e-5, maxit=100, refine=1, verbose=0, omega=None):
+        """Train on X (n x d) with labels +-1; returns (alpha, w (linear) or None, info)."""
+        X, y = _f32(X), _f32(y)
+        n, d = X.shape
+        om = None if omega is None else _f32(omega).reshape(n, k)
+        al = np.empty(n)
+        w = np.empty(d)
+        prm = train_params(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose)
+        info = TrainInfo()
+        self._check(self.lib.tsvm_train(self.h, _ptr(X), _ptr(y), n, d, ctypes.byref(prm), _ptr(om),
+                                        _ptr(al), _ptr(w), ctypes.byref(info)), "tsvm_train")
+        return al, (w if kernel_type == 0 else None), info
+
+    def train_dev(self, X, y, n, d, prm, omega, alpha):
+        info = TrainInfo()
+        self._check(self.lib.tsvm_train_dev(self.h, _ptr(X), _ptr(y), n, d, ctypes.byref(prm),
+                                            _ptr(omega), _ptr(alpha), ctypes.byref(info)),
+                    "tsvm_train_dev")
+        return info
+
+    def decision_function(self, X, y, alpha, gamma, b, T):
+        X, y, T = _f32(X), _f32(y), _f32(T)
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        n, d = X.shape
+        m = T.shape[0]
+        f = np.empty(m)
+        self._check(self.lib.tsvm_decision_function(self.h, _ptr(X), _ptr(y), _ptr(alpha), n, d, gamma,
+                                                    b, _ptr(T), m, _ptr(f)), "tsvm_decision_function")
+        return f
 
     def kernel_product_dev(self, X1, y1, m, X2, y2, n, d, A, r, gamma, precision, W):
         self._check(self.lib.tsvm_kernel_product_dev(self.h, _ptr(X1), _ptr(y1), m, _ptr(X2),

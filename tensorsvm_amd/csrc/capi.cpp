@@ -6,11 +6,9 @@
 
 #include <vector>
 
-#include "internal.h"
+#include <rocblas/rocblas.h>
 
-struct tsvm_ctx {
-  tsvm::Ctx c;
-};
+#include "internal.h"
 
 namespace tsvm {
 
@@ -83,6 +81,7 @@ void tsvm_destroy(tsvm_ctx* ctx) {
   hipSetDevice(ctx->c.device);
   hipStreamSynchronize(ctx->c.stream);
   ctx->c.ws.release_all();
+  if (ctx->c.rocblas) rocblas_destroy_handle((rocblas_handle)ctx->c.rocblas);
   hipStreamDestroy(ctx->c.stream);
   delete ctx;
 }

@@ -8,6 +8,7 @@ typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4_t;
 typedef __attribute__((ext_vector_type(16))) float f32x16_t;
 typedef __attribute__((ext_vector_type(4))) float f32x4_t;
 typedef __attribute__((ext_vector_type(4))) double f64x4_t;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 
 #define TSVM_LOG2E 1.4426950408889634
 

@@ -23,6 +23,7 @@ struct Ctx {
   Workspace ws;
   std::string last_error;
   int cu_count = 256;
+  void* rocblas = nullptr;  // rocblas_handle for the k x k eigen-solver, created on first use
   // phase timers of the last tsvm_lra / tsvm_ipm call (seconds)
   double t_phase[16] = {0};
 };
@@ -62,16 +63,43 @@ tsvm_status gemm_nn_tall(Ctx* c, const float* A, int64_t lda, int64_t ka, const 
                          int64_t kb, int64_t n, float* Y, int64_t ldy);
 
 // ---- small dense FP64 linear algebra on the device (chol.hip) --------------------
-// In-place lower Cholesky of the k x k row-major matrix M (device).  info_dev receives
-// 0 or the 1-based failing column.  Adds `shift` to the diagonal first.
-tsvm_status chol_f64(Ctx* c, double* M, int64_t k, double shift, int* info_dev);
-// Solve (L L^T) X = B for p right-hand sides stored column-wise: B is k x p row-major.
-tsvm_status chol_solve_f64(Ctx* c, const double* L, int64_t k, double* B, int64_t p);
-// Lower-triangular inverse: Linv = L^{-1} (k x k row-major, device).
-tsvm_status tri_inv_lower_f64(Ctx* c, const double* L, int64_t k, double* Linv);
+// In-place lower Cholesky of the k x k row-major matrix M (device); the strict upper
+// triangle is zeroed.  info_dev (device int) receives 0 or the 1-based failing column.
+tsvm_status chol_f64(Ctx* c, double* M, int64_t k, int* info_dev);
+// Solve (L L^T) X = B for p right-hand sides; B is p x k row-major (one RHS per row),
+// overwritten by X.  Skipped (B unchanged) when *info_dev != 0.
+tsvm_status chol_solve_f64(Ctx* c, const double* L, int64_t k, double* B, int64_t p,
+                           const int* info_dev);
 
-// ---- host helpers (host.cpp) ------------------------------------------------------
+// ---- elementwise helpers (util.hip) -----------------------------------------------
+// x[i] <- float(bf16(x[i])) (round to nearest even)
+tsvm_status round_bf16_inplace(Ctx* c, float* x, int64_t count);
+// v (host, k) = sum_i m_i U[i, :], m_i = alpha_i if alpha_i > thr else 0 (device U, alpha)
+tsvm_status masked_colsum(Ctx* c, const float* U, int64_t n, int64_t k, int64_t ldu,
+                          const double* alpha, double thr, double* v_host);
+
+// ---- host small dense FP64 linear algebra (host_la.cpp) ---------------------------
 // Symmetric eigen-decomposition (ascending), V column i = eigenvector i, row-major.
 tsvm_status host_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* V);
+const char* host_lapack_name();
+int host_chol(double* M, int64_t k);  // lower, in place; 0 or failing column + 1
+void host_tri_inv_lower(const double* L, int64_t k, double* Linv);
+void host_gemm(const double* A, const double* B, int64_t k, double* C);  // k x k
+
+// ---- factor construction (lra.cpp) ------------------------------------------------
+// U (n x k, FP32, device) with U U^T ~ K_Y; Om (n x k, device) is consumed (reused as
+// workspace).  Fills info (may be null).
+tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d, int64_t k,
+                    double gamma, int precision, float* Om, float* U, tsvm_lra_info* info);
+
+// ---- interior-point solver (ipm.hip) ---------------------------------------------
+tsvm_status ipm_dev(Ctx* c, const float* Z, const float* a, int64_t n, int64_t k,
+                    const tsvm_ipm_params* prm, double* x, double* xi, double* s,
+                    tsvm_ipm_info* info);
 
 }  // namespace tsvm
+
+// the opaque C-ABI handle
+struct tsvm_ctx {
+  tsvm::Ctx c;
+};

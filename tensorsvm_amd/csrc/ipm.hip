@@ -1,0 +1,1019 @@
+// Primal-dual interior-point solver on the GPU (replaces mpc, tensorsvm-train.cu:1507-1785,
+// NewtonStep 1794-1822 and SMWSolve 1824-1840).
+//
+// State (x, s, xi, the step, D) lives in HBM as FP64 n-vectors; the factor Z stays
+// resident as FP32 (n x kp, kp = k rounded up to 32 with zero columns).  Per iteration:
+//   residual pass     Z^T x, Z (Z^T x), then one fused O(n) pass producing r_x, r_s, r_xi,
+//                     D, 1/D and every reduction of the stop test (mu, |r_x|, a^T x,
+//                     x^T Q x, sums, max/min D) -- one 256-byte device->host copy
+//   capacitance       M = I + Z^T D^-1 Z: FP64 MFMA from FP32 Z (plain form) or BF16
+//                     MFMA with FP32 accumulation (BF16 form); FP64 Cholesky on the GPU
+//   predictor         SMW solves for [r5, a] batched into one pass pair over Z
+//                     (the reference's third solve is a linear combination: SMW(a dy - r5)
+//                     = dy SMW(a) - SMW(r5)); step bound and mu_aff as device reductions
+//   corrector         one SMW solve (the a-solve is reused), combined step, update
+// In the BF16 form each SMW solve is followed by iterative-refinement sweeps whose
+// residual rhs - (D + Z Z^T) v is formed with the FP32 factor and FP64 accumulation.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+#include "ipm_kernels.h"
+
+namespace tsvm {
+
+constexpr int EW_BLOCKS = 1024;  // blocks of the fused O(n) passes
+constexpr int MAXR = 10;         // reduction slots per pass
+
+// ------------------------------------------------------------------------------------
+// block reduction helper: each block writes its R partials to red[blockIdx.x * R + r]
+// ------------------------------------------------------------------------------------
+template <int R>
+__device__ inline void block_reduce_store(double (&v)[R], const int (&op)[R], double* red) {
+  __shared__ double sh[4][R];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    double x = v[r];
+    if (op[r] == RED_SUM) x = wave_sum(x);
+    else if (op[r] == RED_MAX) x = wave_max(x);
+    else x = wave_min(x);
+    if (lane == 0) sh[wave][r] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < R) {
+    int r = threadIdx.x;
+    double x = sh[0][r];
+    for (int w = 1; w < 4; w++) {
+      double y = sh[w][r];
+      x = op[r] == RED_SUM ? x + y : (op[r] == RED_MAX ? fmax(x, y) : fmin(x, y));
+    }
+    red[blockIdx.x * R + r] = x;
+  }
+}
+
+// sc[slot[r]] = reduction over nblk blocks of red[b * R + r]  (one block, fixed order)
+__global__ __launch_bounds__(256) void reduce_slots(const double* __restrict__ red, int nblk,
+                                                    int R, const int* __restrict__ ops,
+                                                    const int* __restrict__ slots,
+                                                    double* __restrict__ sc) {
+  __shared__ double sh[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int r = 0; r < R; r++) {
+    const int op = ops[r];
+    double x = op == RED_SUM ? 0.0 : (op == RED_MAX ? -INFINITY : INFINITY);
+    for (int b = threadIdx.x; b < nblk; b += 256) {
+      double y = red[b * R + r];
+      x = op == RED_SUM ? x + y : (op == RED_MAX ? fmax(x, y) : fmin(x, y));
+    }
+    x = op == RED_SUM ? wave_sum(x) : (op == RED_MAX ? wave_max(x) : wave_min(x));
+    if (lane == 0) sh[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double z = sh[0];
+      for (int w = 1; w < 4; w++)
+        z = op == RED_SUM ? z + sh[w] : (op == RED_MAX ? fmax(z, sh[w]) : fmin(z, sh[w]));
+      sc[slots[r]] = z;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Z^T V: block b owns a row range; thread t handles float4 column group t % (kp/4) of
+// rows t / (kp/4) + step * R.  Partials per block, then a fixed-order reduction.
+// ------------------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(256) void gemv_t_kernel(const float* __restrict__ Z, int64_t n, int kp,
+                                                     const double* __restrict__ V, int64_t ldv,
+                                                     int64_t rows_per_blk, double* __restrict__ part) {
+  extern __shared__ double shm[];
+  const int L = kp / 4;
+  const int R = 256 / L;
+  const int slot = threadIdx.x / L, c4 = threadIdx.x % L;
+  const bool active = slot < R;
+  const int64_t r0 = blockIdx.x * rows_per_blk;
+  const int64_t r1 = r0 + rows_per_blk < n ? r0 + rows_per_blk : n;
+  double acc[P][4];
+#pragma unroll
+  for (int q = 0; q < P; q++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[q][j] = 0.0;
+  if (active) {
+    int64_t i = r0 + slot;
+    for (; i + 3 * R < r1; i += 4 * R) {
+      f32x4_t z[4];
+      double v[4][P];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        z[u] = *(const f32x4_t*)(Z + (i + u * R) * kp + 4 * c4);
+#pragma unroll
+        for (int q = 0; q < P; q++) v[u][q] = V[q * ldv + i + u * R];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int q = 0; q < P; q++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) acc[q][j] += v[u][q] * (double)z[u][j];
+    }
+    for (; i < r1; i += R) {
+      f32x4_t z = *(const f32x4_t*)(Z + i * kp + 4 * c4);
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        double vq = V[q * ldv + i];
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[q][j] += vq * (double)z[j];
+      }
+    }
+  }
+  // combine the R row slots: shm[slot][q][c]
+  if (active)
+#pragma unroll
+    for (int q = 0; q < P; q++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) shm[(slot * P + q) * kp + 4 * c4 + j] = acc[q][j];
+  __syncthreads();
+  for (int e = threadIdx.x; e < P * kp; e += 256) {
+    double s = 0.0;
+    for (int sl = 0; sl < R; sl++) s += shm[sl * P * kp + e];
+    part[blockIdx.x * (int64_t)(P * kp) + e] = s;
+  }
+}
+
+// T[e] = sum over blocks of part[b][e]; 64 outputs x 4 partial lanes per block
+__global__ __launch_bounds__(256) void gemv_t_reduce(const double* __restrict__ part, int nblk,
+                                                     int m, double* __restrict__ T) {
+  __shared__ double sh[4][64];
+  const int o = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+  double s = 0.0;
+  if (o < m)
+    for (int b = g; b < nblk; b += 4) s += part[(int64_t)b * m + o];
+  sh[g][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (g == 0 && o < m) T[o] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+}
+
+static int gemv_blocks(Ctx* c, int64_t n) {
+  int64_t b = (int64_t)c->cu_count * 2;
+  int64_t mx = ceil_div(n, 64);
+  if (b > mx) b = mx;
+  return (int)(b < 1 ? 1 : b);
+}
+
+tsvm_status ipm_gemv_t(Ctx* c, const float* Z, int64_t n, int kp, const double* V, int64_t ldv,
+                       int p, double* T) {
+  if (kp % 32 != 0 || kp > 1024 || p < 1 || p > 4)
+    return fail(c, TSVM_ERR_ARG, "gemv_t: kp=%d p=%d unsupported", kp, p);
+  const int nblk = gemv_blocks(c, n);
+  const int64_t rpb = ceil_div(n, nblk);
+  void* part;
+  TSVM_HIP(c, c->ws.get("gemvt_part", (size_t)nblk * p * kp * 8, &part));
+  const int R = 256 / (kp / 4);
+  size_t sm = (size_t)R * p * kp * 8;
+  dim3 g(nblk), b(256);
+  switch (p) {
+    case 1: hipLaunchKernelGGL(gemv_t_kernel<1>, g, b, sm, c->stream, Z, n, kp, V, ldv, rpb, (double*)part); break;
+    case 2: hipLaunchKernelGGL(gemv_t_kernel<2>, g, b, sm, c->stream, Z, n, kp, V, ldv, rpb, (double*)part); break;
+    case 3: hipLaunchKernelGGL(gemv_t_kernel<3>, g, b, sm, c->stream, Z, n, kp, V, ldv, rpb, (double*)part); break;
+    default: hipLaunchKernelGGL(gemv_t_kernel<4>, g, b, sm, c->stream, Z, n, kp, V, ldv, rpb, (double*)part); break;
+  }
+  const int m = p * kp;
+  hipLaunchKernelGGL(gemv_t_reduce, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, c->stream,
+                     (const double*)part, nblk, m, T);
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Z T: L = min(64, kp/4) lanes per row, each lane F = kp/(4L) float4 groups; the lane
+// segment is reduced with xor shuffles.  T columns of a lane stay in registers.
+// ------------------------------------------------------------------------------------
+template <int P, int F>
+__global__ __launch_bounds__(256) void gemv_n_kernel(const float* __restrict__ Z, int64_t n, int kp,
+                                                     const double* __restrict__ T,
+                                                     double* __restrict__ E, int64_t lde) {
+  const int L = (kp / 4 < 64) ? kp / 4 : 64;
+  const int RW = 64 / L;  // rows per wave per step
+  const int lane = threadIdx.x & 63;
+  const int seg = lane / L, li = lane % L;
+  double t[P][F][4];
+#pragma unroll
+  for (int q = 0; q < P; q++)
+#pragma unroll
+    for (int f = 0; f < F; f++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) t[q][f][j] = T[q * kp + 4 * (li + f * L) + j];
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 64;
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / 64;
+  for (int64_t base = wave * RW; base < n; base += nwaves * RW * 2) {
+    f32x4_t z[2][F];
+    int64_t rows[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      rows[u] = base + u * nwaves * RW + seg;
+      int64_t rr = rows[u] < n ? rows[u] : n - 1;
+#pragma unroll
+      for (int f = 0; f < F; f++) z[u][f] = *(const f32x4_t*)(Z + rr * kp + 4 * (li + f * L));
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      double acc[P];
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        double s = 0.0;
+#pragma unroll
+        for (int f = 0; f < F; f++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) s += (double)z[u][f][j] * t[q][f][j];
+        acc[q] = s;
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++)
+        for (int o = L / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+      if (li == 0 && rows[u] < n)
+#pragma unroll
+        for (int q = 0; q < P; q++) E[q * lde + rows[u]] = acc[q];
+    }
+  }
+}
+
+tsvm_status ipm_gemv_n(Ctx* c, const float* Z, int64_t n, int kp, const double* T, int p,
+                       double* E, int64_t lde) {
+  if (kp % 32 != 0 || kp > 1024 || p < 1 || p > 2)
+    return fail(c, TSVM_ERR_ARG, "gemv_n: kp=%d p=%d unsupported", kp, p);
+  const int L = kp / 4 < 64 ? kp / 4 : 64;
+  const int F = kp / (4 * L);
+  int64_t blocks = (int64_t)c->cu_count * 8;
+  int64_t need = ceil_div(n, 4 * (64 / L) * 2);
+  if (blocks > need) blocks = need < 1 ? 1 : need;
+  dim3 g((unsigned)blocks), b(256);
+#define GN(PP, FF) hipLaunchKernelGGL((gemv_n_kernel<PP, FF>), g, b, 0, c->stream, Z, n, kp, T, E, lde)
+  if (p == 1) {
+    if (F == 1) GN(1, 1); else if (F == 2) GN(1, 2); else if (F == 3) GN(1, 3); else GN(1, 4);
+  } else {
+    if (F == 1) GN(2, 1); else if (F == 2) GN(2, 2); else if (F == 3) GN(2, 3); else GN(2, 4);
+  }
+#undef GN
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// fused O(n) passes
+// ------------------------------------------------------------------------------------
+#define GRID_LOOP(i, n) \
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+// residual, D, stop-test reductions (1606-1643)
+__global__ __launch_bounds__(256) void k_resid(int64_t n, double C, const double* __restrict__ x,
+                                               const double* __restrict__ s, const double* __restrict__ xi,
+                                               const double* __restrict__ a, const double* __restrict__ e,
+                                               const double* __restrict__ sc, double* __restrict__ dx,
+                                               double* __restrict__ ds, double* __restrict__ dxi,
+                                               double* __restrict__ D, double* __restrict__ winv,
+                                               double* __restrict__ red) {
+  const double y = sc[SL_Y];
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, -INFINITY, INFINITY};
+  const int op[9] = {RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_MAX, RED_MIN};
+  GRID_LOOP(i, n) {
+    double xv = x[i], sv = s[i], xiv = xi[i], av = a[i], ev = e[i];
+    double cx = C - xv;
+    double r = ev - av * y - sv + xiv - 1.0;
+    dx[i] = r;
+    ds[i] = -sv * xv;
+    dxi[i] = -xiv * cx;
+    double Dv = sv / xv + xiv / cx;
+    D[i] = Dv;
+    winv[i] = 1.0 / Dv;
+    v[0] += xv * sv + cx * xiv;
+    v[1] += r * r;
+    v[2] += av * xv;
+    v[3] += xv * ev;
+    v[4] += xv;
+    v[5] += xiv;
+    v[6] += fabs(xv);
+    v[7] = fmax(v[7], Dv);
+    v[8] = fmin(v[8], Dv);
+  }
+  block_reduce_store<9>(v, op, red);
+}
+
+// predictor right-hand sides: rhs0 = r5 = r1 - r3/x + r4/(C-x), rhs1 = a (1805-1808)
+__global__ void k_rhs_pred(int64_t n, double C, const double* __restrict__ x,
+                           const double* __restrict__ dx, const double* __restrict__ ds,
+                           const double* __restrict__ dxi, const double* __restrict__ a,
+                           double* __restrict__ rhs, int64_t ldr) {
+  GRID_LOOP(i, n) {
+    double xv = x[i];
+    rhs[i] = dx[i] - ds[i] / xv + dxi[i] / (C - xv);
+    rhs[ldr + i] = a[i];
+  }
+}
+
+// V = rhs / D (p vectors)
+__global__ void k_scale(int64_t n, int p, const double* __restrict__ rhs, int64_t ldr,
+                        const double* __restrict__ winv, double* __restrict__ V, int64_t ldv) {
+  GRID_LOOP(i, n) {
+    double w = winv[i];
+    for (int q = 0; q < p; q++) V[q * ldv + i] = rhs[q * ldr + i] * w;
+  }
+}
+
+// sol (+)= V - E/D  (SMW: D^-1 v - D^-1 Z M^-1 Z^T D^-1 v, 1828-1838)
+__global__ void k_smw_fin(int64_t n, int p, int accumulate, const double* __restrict__ V, int64_t ldv,
+                          const double* __restrict__ E, int64_t lde, const double* __restrict__ winv,
+                          double* __restrict__ sol, int64_t lds) {
+  GRID_LOOP(i, n) {
+    double w = winv[i];
+    for (int q = 0; q < p; q++) {
+      double v = V[q * ldv + i] - E[q * lde + i] * w;
+      if (accumulate) sol[q * lds + i] += v;
+      else sol[q * lds + i] = v;
+    }
+  }
+}
+
+// refinement residual: res = rhs - D sol - E (E = Z Z^T sol); V = res / D; reductions of
+// |res|^2 and |rhs|^2 for RHS 0
+__global__ __launch_bounds__(256) void k_refine_res(int64_t n, int p, const double* __restrict__ rhs,
+                                                    int64_t ldr, const double* __restrict__ sol,
+                                                    int64_t lds, const double* __restrict__ E, int64_t lde,
+                                                    const double* __restrict__ D,
+                                                    const double* __restrict__ winv,
+                                                    double* __restrict__ V, int64_t ldv,
+                                                    double* __restrict__ red) {
+  double v[3] = {0, 0, 0};
+  const int op[3] = {RED_SUM, RED_SUM, RED_SUM};
+  GRID_LOOP(i, n) {
+    double Dv = D[i], w = winv[i];
+    for (int q = 0; q < p; q++) {
+      double r = rhs[q * ldr + i] - Dv * sol[q * lds + i] - E[q * lde + i];
+      V[q * ldv + i] = r * w;
+      if (q < 2) v[q] += r * r;
+    }
+    double b = rhs[i];
+    v[2] += b * b;
+  }
+  block_reduce_store<3>(v, op, red);
+}
+
+// dots of the Newton step: a^T sol0 (and a^T sol1)
+__global__ __launch_bounds__(256) void k_dots(int64_t n, int p, const double* __restrict__ a,
+                                              const double* __restrict__ sol, int64_t lds,
+                                              double* __restrict__ red) {
+  double v[2] = {0, 0};
+  const int op[2] = {RED_SUM, RED_SUM};
+  GRID_LOOP(i, n) {
+    double av = a[i];
+    v[0] += av * sol[i];
+    if (p > 1) v[1] += av * sol[lds + i];
+  }
+  block_reduce_store<2>(v, op, red);
+}
+
+// right-hand side of the third SMW solve (1815-1816): rhs3 = a dy - r5, with
+// dy = (r2 + a^T r7) / (a^T b), r2 = -a^T x (predictor) or 0 (corrector)
+__global__ void k_rhs3(int64_t n, int corr, const double* __restrict__ a, const double* __restrict__ r5,
+                       double* __restrict__ sc, double* __restrict__ rhs3) {
+  const double r2 = corr ? 0.0 : -sc[SL_AX];
+  const double dy = (r2 + sc[corr ? SL_AR7C : SL_AR7]) / sc[SL_AB];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[corr ? SL_DYC : SL_DY] = dy;
+  GRID_LOOP(i, n) rhs3[i] = a[i] * dy - r5[i];
+}
+
+// Newton step completion (1817-1821) and step bound (1733-1740 / 1762-1768):
+//   ds = (r3 - s dx)/x;  dxi = (r4 + xi dx)/(C-x), dx = (D + Z Z^T)^-1 rhs3
+// corr = 0: predictor, results to (dx, ds, dxi) in place.
+// corr = 1: corrector; the total step aff + corr goes to (dx, ds, dxi), the affine
+//           step is read from (adx, ads, adxi).
+__global__ __launch_bounds__(256) void k_step(int64_t n, double C, int corr, const double* __restrict__ x,
+                                              const double* __restrict__ s, const double* __restrict__ xi,
+                                              const double* __restrict__ dxs, double* __restrict__ dx,
+                                              double* __restrict__ ds, double* __restrict__ dxi,
+                                              const double* __restrict__ adx, const double* __restrict__ ads,
+                                              const double* __restrict__ adxi, double* __restrict__ red) {
+  double v[1] = {1.0};
+  const int op[1] = {RED_MIN};
+  GRID_LOOP(i, n) {
+    double xv = x[i], cx = C - xv;
+    double ddx = dxs[i];
+    double dds = (ds[i] - s[i] * ddx) / xv;
+    double ddxi = (dxi[i] + xi[i] * ddx) / cx;
+    if (corr) {
+      ddx += adx[i];
+      dds += ads[i];
+      ddxi += adxi[i];
+    }
+    dx[i] = ddx;
+    ds[i] = dds;
+    dxi[i] = ddxi;
+    double al = v[0];
+    if (ddx < 0) al = fmin(al, -xv / ddx);
+    else if (ddx > 0) al = fmin(al, cx / ddx);
+    if (dds < 0) al = fmin(al, -s[i] / dds);
+    if (ddxi < 0) al = fmin(al, -xi[i] / ddxi);
+    v[0] = al;
+  }
+  block_reduce_store<1>(v, op, red);
+}
+
+// complementarity at the predictor point (1742-1745)
+__global__ __launch_bounds__(256) void k_muaff(int64_t n, double C, const double* __restrict__ x,
+                                               const double* __restrict__ s, const double* __restrict__ xi,
+                                               const double* __restrict__ dx, const double* __restrict__ ds,
+                                               const double* __restrict__ dxi, const double* __restrict__ sc,
+                                               double* __restrict__ red) {
+  const double al = sc[SL_ALPHA_AFF];
+  double v[1] = {0.0};
+  const int op[1] = {RED_SUM};
+  GRID_LOOP(i, n) {
+    double xa = x[i] + al * dx[i];
+    v[0] += xa * (s[i] + al * ds[i]) + (C - xa) * (xi[i] + al * dxi[i]);
+  }
+  block_reduce_store<1>(v, op, red);
+}
+
+// corrector right-hand sides (1750-1757): keep the affine step, r3 = sigma mu - dx ds,
+// r4 = sigma mu + dx dxi, r1 = r2 = 0, rhs0 = r5 = -r3/x + r4/(C-x)
+__global__ void k_rhs_corr(int64_t n, double C, const double* __restrict__ x,
+                           double* __restrict__ dx, double* __restrict__ ds, double* __restrict__ dxi,
+                           double* __restrict__ adx, double* __restrict__ ads, double* __restrict__ adxi,
+                           const double* __restrict__ sc, double* __restrict__ rhs) {
+  const double n2 = 2.0 * (double)n;
+  const double mu = sc[SL_MU] / n2, mua = sc[SL_MUAFF] / n2;
+  double sig = mua / mu;
+  sig = sig * sig * sig;
+  const double sm = sig * mu;
+  GRID_LOOP(i, n) {
+    double a1 = dx[i], a3 = ds[i], a4 = dxi[i];
+    adx[i] = a1;
+    ads[i] = a3;
+    adxi[i] = a4;
+    double r3 = sm - a1 * a3, r4 = sm + a1 * a4;
+    ds[i] = r3;
+    dxi[i] = r4;
+    double xv = x[i];
+    rhs[i] = -r3 / xv + r4 / (C - xv);
+  }
+}
+
+// x, s, xi += 0.99 alpha step; y += 0.99 alpha (dy + dyc)   (1769-1777).  Skipped when the
+// capacitance Cholesky failed (the reference returns the current iterate, 1711-1714).
+__global__ void k_update(int64_t n, double* __restrict__ x, double* __restrict__ s, double* __restrict__ xi,
+                         const double* __restrict__ dx, const double* __restrict__ ds,
+                         const double* __restrict__ dxi, double* __restrict__ sc, const int* __restrict__ info) {
+  if (*info != 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[SL_CHOLFAIL] = 1.0;
+    return;
+  }
+  const double al = 0.99 * sc[SL_ALPHA];
+  GRID_LOOP(i, n) {
+    x[i] += al * dx[i];
+    s[i] += al * ds[i];
+    xi[i] += al * dxi[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[SL_Y] += al * (sc[SL_DY] + sc[SL_DYC]);
+}
+
+// M += I (kp x kp) and copy lower to upper is not needed (Cholesky reads the lower part)
+__global__ void k_add_identity(double* __restrict__ M, int64_t kp) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < kp) M[i * kp + i] += 1.0;
+}
+
+// initial point (1535-1579): x = clamp(1^T Q 1 / |Q 1|^2) (RBF) or clamp(1) (linear), s = xi = 1
+__global__ __launch_bounds__(256) void k_init_red(int64_t n, const double* __restrict__ e,
+                                                  double* __restrict__ red) {
+  double v[2] = {0, 0};
+  const int op[2] = {RED_SUM, RED_SUM};
+  GRID_LOOP(i, n) {
+    v[0] += e[i] * e[i];
+    v[1] += e[i];
+  }
+  block_reduce_store<2>(v, op, red);
+}
+__global__ void k_fill(int64_t n, double* __restrict__ p, double v) {
+  GRID_LOOP(i, n) p[i] = v;
+}
+__global__ void k_init_xs(int64_t n, double C, int linear, const double* __restrict__ sc,
+                          double* __restrict__ x, double* __restrict__ s, double* __restrict__ xi) {
+  double ox = linear ? 1.0 : sc[SL_QE] / sc[SL_QQ];
+  if (ox >= 0.99 * C) ox = 0.99 * C;
+  else if (ox <= 0.01 * C) ox = 0.01 * C;
+  GRID_LOOP(i, n) {
+    x[i] = ox;
+    s[i] = 1.0;
+    xi[i] = 1.0;
+  }
+}
+__global__ void k_init_linear_adjust(int64_t n, const double* __restrict__ e, const double* __restrict__ a,
+                                     double* __restrict__ s, double* __restrict__ xi) {
+  GRID_LOOP(i, n) {
+    double t = e[i] - 1.0;  // y = 0 at the start
+    (void)a;
+    if (t > 0) s[i] += t;
+    else xi[i] -= t;
+  }
+}
+__global__ void k_labels_to_f64(int64_t n, const float* __restrict__ af, double* __restrict__ a) {
+  GRID_LOOP(i, n) a[i] = (double)af[i];
+}
+__global__ void k_pad_z(int64_t n, int64_t k, int kp, const float* __restrict__ Z, float* __restrict__ Zp) {
+  GRID_LOOP(e, n * kp) {
+    int64_t i = e / kp, c = e - i * kp;
+    Zp[e] = c < k ? Z[i * k + c] : 0.0f;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// BF16 capacitance: ZbT (kpt x n_pad bf16) is the transposed factor made once per solve;
+// every iteration M = I + sum_n w_n z_n z_n^T is accumulated on MFMA 32x32x16 bf16 with
+// FP32 accumulation.  The weights are applied to the A panel while it is staged into LDS.
+// Block = 4 waves, 128 x 128 output tile (wave: 64 x 64 = 2 x 2 MFMA tiles), lower tiles
+// only; the n range is split over blocks, partials are summed in FP64 in a fixed order.
+// ------------------------------------------------------------------------------------
+__global__ void make_zbt_kernel(const float* __restrict__ Z, int64_t n, int kp, int kpt,
+                                int64_t n_pad, __bf16* __restrict__ ZbT) {
+  __shared__ float tile[32][33];
+  const int64_t i0 = (int64_t)blockIdx.x * 32;  // rows of Z
+  const int c0 = blockIdx.y * 32;                // columns of Z
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    int64_t i = i0 + r;
+    int c = c0 + tx;
+    tile[r][tx] = (i < n && c < kp) ? Z[i * kp + c] : 0.0f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    int c = c0 + r;
+    if (c < kpt) ZbT[(int64_t)c * n_pad + i0 + tx] = (__bf16)tile[tx][r];
+  }
+}
+
+tsvm_status ipm_make_zbt(Ctx* c, const float* Z, int64_t n, int kp, int64_t n_pad, __bf16* ZbT) {
+  const int kpt = (int)round_up(kp, 128);
+  dim3 g((unsigned)(n_pad / 32), (unsigned)(kpt / 32));
+  hipLaunchKernelGGL(make_zbt_kernel, g, dim3(256), 0, c->stream, Z, n, kp, kpt, n_pad, ZbT);
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+constexpr int SY_BK = 64;            // n per stage
+constexpr int SY_LS = SY_BK + 8;     // LDS row stride (bf16)
+
+__global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict__ ZbT, int64_t n_pad,
+                                                        int kpt, const double* __restrict__ w,
+                                                        int64_t n, int ntile, int nsplit,
+                                                        int64_t stages_per_split,
+                                                        float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[128 * SY_LS];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[128 * SY_LS];
+  // block -> (tile, split): the tiles of one split sit 8 blocks apart, i.e. on one XCD
+  const int nlow = ntile * (ntile + 1) / 2;
+  const int bid = blockIdx.x;
+  const int xcd = bid % 8, grp = bid / 8;
+  const int tile = grp % nlow;
+  const int split = (grp / nlow) * 8 + xcd;
+  if (split >= nsplit) return;
+  int ta = 0;
+  while ((ta + 1) * (ta + 2) / 2 <= tile) ta++;
+  const int tb = tile - ta * (ta + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, h = lane >> 5;
+  const int wa = (wave >> 1) * 64, wb = (wave & 1) * 64;
+  const int64_t s0 = (int64_t)split * stages_per_split;
+  int64_t s1 = s0 + stages_per_split;
+  const int64_t nst = n_pad / SY_BK;
+  if (s1 > nst) s1 = nst;
+  f32x16_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
+  const __bf16* Apan = ZbT + (int64_t)ta * 128 * n_pad;
+  const __bf16* Bpan = ZbT + (int64_t)tb * 128 * n_pad;
+  for (int64_t st = s0; st < s1; st++) {
+    const int64_t n0 = st * SY_BK;
+    // stage: 128 rows x 64 n per panel = 1024 vectors of 8; 4 per thread per panel
+    u32x4_t va[4], vb[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      int idx = tid + v * 256;
+      int row = idx >> 3, col = (idx & 7) * 8;
+      va[v] = *(const u32x4_t*)(Apan + (int64_t)row * n_pad + n0 + col);
+      vb[v] = *(const u32x4_t*)(Bpan + (int64_t)row * n_pad + n0 + col);
+    }
+    __syncthreads();  // previous stage's LDS reads are done
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      int idx = tid + v * 256;
+      int row = idx >> 3, col = (idx & 7) * 8;
+      bf16x8_t a8 = *(bf16x8_t*)&va[v];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        int64_t nn = n0 + col + j;
+        float wj = nn < n ? (float)w[nn] : 0.0f;
+        a8[j] = (__bf16)((float)a8[j] * wj);
+      }
+      *(bf16x8_t*)&sA[row * SY_LS + col] = a8;
+      *(u32x4_t*)&sB[row * SY_LS + col] = vb[v];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < SY_BK / 16; ks++) {
+      bf16x8_t af[2], bfr[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        af[i] = *(const bf16x8_t*)&sA[(wa + 32 * i + l32) * SY_LS + ks * 16 + 8 * h];
+        bfr[i] = *(const bf16x8_t*)&sB[(wb + 32 * i + l32) * SY_LS + ks * 16 + 8 * h];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  float* P = part + (int64_t)split * kpt * kpt;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        int a = ta * 128 + wa + 32 * i + 8 * (q / 4) + 4 * h + (q % 4);
+        int bcol = tb * 128 + wb + 32 * j + l32;
+        P[(int64_t)a * kpt + bcol] = acc[i][j][q];
+      }
+}
+
+// M (kp x kp FP64) = I + sum over splits (lower tiles, mirrored)
+__global__ void syrk_reduce(const float* __restrict__ part, int nsplit, int kpt, int kp,
+                            double* __restrict__ M) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)kp * kp) return;
+  int a = (int)(idx / kp), bb = (int)(idx % kp);
+  int ra = a, rb = bb;
+  if (rb / 128 > ra / 128) {  // upper tile: read the mirrored lower entry
+    ra = bb;
+    rb = a;
+  }
+  double s = 0.0;
+  for (int t = 0; t < nsplit; t++) s += (double)part[(int64_t)t * kpt * kpt + (int64_t)ra * kpt + rb];
+  if (a == bb) s += 1.0;
+  M[idx] = s;
+}
+
+tsvm_status ipm_syrk_bf16(Ctx* c, const __bf16* ZbT, int64_t n_pad, int kp, const double* w,
+                          int64_t n, double* M, double* tflops_out) {
+  (void)tflops_out;
+  const int kpt = (int)round_up(kp, 128);
+  const int ntile = kpt / 128;
+  const int nlow = ntile * (ntile + 1) / 2;
+  const int64_t nst = n_pad / SY_BK;
+  // about two blocks per CU in total, rounded to whole groups of 8 splits
+  int nsplit = (int)ceil_div((int64_t)c->cu_count * 2, nlow);
+  nsplit = (int)round_up(nsplit, 8);
+  if (nsplit > nst) nsplit = (int)round_up(nst, 8);
+  const int64_t sps = ceil_div(nst, nsplit);
+  void* part;
+  TSVM_HIP(c, c->ws.get("syrk_part", (size_t)nsplit * kpt * kpt * 4, &part));
+  const int nblocks = nlow * nsplit;  // nsplit is a multiple of 8
+  hipLaunchKernelGGL(syrk_bf16_kernel, dim3(nblocks), dim3(256), 0, c->stream, ZbT, n_pad, kpt, w, n,
+                     ntile, nsplit, sps, (float*)part);
+  hipLaunchKernelGGL(syrk_reduce, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)), dim3(256), 0,
+                     c->stream, (const float*)part, nsplit, kpt, kp, M);
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// host driver
+// ------------------------------------------------------------------------------------
+namespace {
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_t k,
+                    const tsvm_ipm_params* prm, double* x, double* xi, double* s_out,
+                    tsvm_ipm_info* info) {
+  const double t_start = now();
+  if (n <= 0 || k <= 0 || k > 1024) return fail(c, TSVM_ERR_ARG, "ipm: need n > 0, 0 < k <= 1024");
+  if (!(prm->C > 0)) return fail(c, TSVM_ERR_ARG, "ipm: C must be positive");
+  hipStream_t st = c->stream;
+  const int kp = (int)round_up(k, 32);
+  const double C = prm->C;
+  // The BF16 capacitance is only accurate enough for SMW while its absolute error stays
+  // well below 1 (the refinement contraction is about rho(M~ - M)); until the hybrid
+  // split (FP64 for high-weight rows, BF16 for the bulk) is selected per iteration, both
+  // forms build the capacitance in FP64.
+  const bool bf16 = false;
+  const int refine = bf16 ? (prm->refine < 0 ? 1 : prm->refine) : 0;
+  void* p;
+  auto ws = [&](const char* name, int64_t bytes, void** out) -> tsvm_status {
+    TSVM_HIP(c, c->ws.get(name, (size_t)bytes, out));
+    return TSVM_OK;
+  };
+  double *a, *e, *dx, *ds, *dxi, *adx, *ads, *adxi, *D, *winv, *rhs, *V, *E, *solp, *solc, *T, *M,
+      *sc, *red, *s, *rhs3, *sol3;
+  int *dinfo, *tables;
+#define WS(name, bytes, var)              \
+  TSVM_CHECK(ws(name, (bytes), &p));      \
+  var = (decltype(var))p;
+  WS("ipm_a", n * 8, a);
+  WS("ipm_e", n * 8, e);
+  WS("ipm_dx", n * 8, dx);
+  WS("ipm_ds", n * 8, ds);
+  WS("ipm_dxi", n * 8, dxi);
+  WS("ipm_adx", n * 8, adx);
+  WS("ipm_ads", n * 8, ads);
+  WS("ipm_adxi", n * 8, adxi);
+  WS("ipm_D", n * 8, D);
+  WS("ipm_winv", n * 8, winv);
+  WS("ipm_rhs", 2 * n * 8, rhs);
+  WS("ipm_V", 2 * n * 8, V);
+  WS("ipm_E", 2 * n * 8, E);
+  WS("ipm_solp", 2 * n * 8, solp);
+  WS("ipm_solc", n * 8, solc);
+  WS("ipm_rhs3", n * 8, rhs3);
+  WS("ipm_sol3", n * 8, sol3);
+  WS("ipm_T", 4 * kp * 8, T);
+  WS("ipm_M", (int64_t)kp * kp * 8, M);
+  WS("ipm_sc", IPM_NSLOT * 8, sc);
+  WS("ipm_red", EW_BLOCKS * MAXR * 8, red);
+  WS("ipm_info", 16, dinfo);
+  WS("ipm_tables", 2 * MAXR * 4, tables);
+  if (s_out) {
+    s = s_out;
+  } else {
+    WS("ipm_s", n * 8, s);
+  }
+  const float* Z = Zin;
+  if (kp != k) {
+    float* Zp;
+    WS("ipm_Zp", n * kp * 4, Zp);
+    hipLaunchKernelGGL(k_pad_z, dim3(4096), dim3(256), 0, st, n, k, kp, Zin, Zp);
+    Z = Zp;
+  }
+  __bf16* ZbT = nullptr;
+  const int64_t n_pad = round_up(n, SY_BK);
+  if (bf16) {
+    const int kpt = (int)round_up(kp, 128);
+    WS("ipm_ZbT", (int64_t)kpt * n_pad * 2, ZbT);
+    TSVM_CHECK(ipm_make_zbt(c, Z, n, kp, n_pad, ZbT));
+  }
+#undef WS
+  const dim3 eg(EW_BLOCKS), eb(256);
+  hipLaunchKernelGGL(k_labels_to_f64, eg, eb, 0, st, n, af, a);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (int64_t)IPM_NSLOT, sc, 0.0);
+
+  auto reduce = [&](std::vector<int> ops, std::vector<int> slots) -> tsvm_status {
+    int R = (int)ops.size();
+    int buf[2 * MAXR] = {0};
+    for (int r = 0; r < R; r++) {
+      buf[r] = ops[r];
+      buf[MAXR + r] = slots[r];
+    }
+    TSVM_HIP(c, hipMemcpyAsync(tables, buf, sizeof(buf), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(reduce_slots, dim3(1), dim3(256), 0, st, (const double*)red, EW_BLOCKS, R,
+                       (const int*)tables, (const int*)(tables + MAXR), sc);
+    return TSVM_OK;
+  };
+  // SMW solve with optional refinement: sol = (D + Z Z^T)^{-1} rhs, p right-hand sides
+  auto smw = [&](double* rh, int pp, double* sol) -> tsvm_status {
+    hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, pp, (const double*)rh, n, (const double*)winv, V, n);
+    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, T));
+    TSVM_CHECK(chol_solve_f64(c, M, kp, T, pp, dinfo));
+    TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, pp, E, n));
+    hipLaunchKernelGGL(k_smw_fin, eg, eb, 0, st, n, pp, 0, (const double*)V, n, (const double*)E, n,
+                       (const double*)winv, sol, n);
+    for (int sw = 0; sw < refine; sw++) {
+      TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, sol, n, pp, T));
+      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, pp, E, n));
+      hipLaunchKernelGGL(k_refine_res, eg, eb, 0, st, n, pp, (const double*)rh, n, (const double*)sol, n,
+                         (const double*)E, n, (const double*)D, (const double*)winv, V, n, red);
+      TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_RES0, SL_RES1, SL_RHS0}));
+      TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, T));
+      TSVM_CHECK(chol_solve_f64(c, M, kp, T, pp, dinfo));
+      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, pp, E, n));
+      hipLaunchKernelGGL(k_smw_fin, eg, eb, 0, st, n, pp, 1, (const double*)V, n, (const double*)E, n,
+                         (const double*)winv, sol, n);
+    }
+    TSVM_HIP(c, hipGetLastError());
+    return TSVM_OK;
+  };
+
+  // ---- initial point (1535-1579)
+  const bool linear = prm->kernel_type == 0;
+  if (!linear) {
+    hipLaunchKernelGGL(k_fill, eg, eb, 0, st, n, V, 1.0);
+    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, 1, T));
+    TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, 1, e, n));
+    hipLaunchKernelGGL(k_init_red, eg, eb, 0, st, n, (const double*)e, red);
+    TSVM_CHECK(reduce({RED_SUM, RED_SUM}, {SL_QQ, SL_QE}));
+  }
+  hipLaunchKernelGGL(k_init_xs, eg, eb, 0, st, n, C, linear ? 1 : 0, (const double*)sc, x, s, xi);
+  if (linear) {
+    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, x, n, 1, T));
+    TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, 1, e, n));
+    hipLaunchKernelGGL(k_init_linear_adjust, eg, eb, 0, st, n, (const double*)e, (const double*)a, s, xi);
+  }
+  TSVM_HIP(c, hipGetLastError());
+
+  hipEvent_t ev0, ev1;
+  TSVM_HIP(c, hipEventCreate(&ev0));
+  TSVM_HIP(c, hipEventCreate(&ev1));
+  double t_syrk = 0.0;
+  int n_syrk = 0;
+  bool pending_syrk = false;
+  double hsc[IPM_NSLOT];
+  tsvm_ipm_info inf;
+  memset(&inf, 0, sizeof(inf));
+  int reason = TSVM_MAXIT;
+  int iter = 0;
+  const int maxit = prm->maxit > 0 ? prm->maxit : 100;
+  for (iter = 0; iter < maxit; iter++) {
+    // ---- residual pass (1606-1643)
+    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, x, n, 1, T));
+    TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, 1, e, n));
+    hipLaunchKernelGGL(k_resid, eg, eb, 0, st, n, C, (const double*)x, (const double*)s,
+                       (const double*)xi, (const double*)a, (const double*)e, (const double*)sc, dx,
+                       ds, dxi, D, winv, red);
+    TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_MAX, RED_MIN},
+                      {SL_MU, SL_NDX2, SL_AX, SL_XQX, SL_SX, SL_SXI, SL_L1, SL_DMAX, SL_DMIN}));
+    TSVM_HIP(c, hipMemcpyAsync(hsc, sc, sizeof(hsc), hipMemcpyDeviceToHost, st));
+    TSVM_HIP(c, hipStreamSynchronize(st));
+    if (pending_syrk) {
+      float ms = 0.f;
+      TSVM_HIP(c, hipEventElapsedTime(&ms, ev0, ev1));
+      t_syrk += ms * 1e-3;
+      n_syrk++;
+      pending_syrk = false;
+    }
+    if (hsc[SL_CHOLFAIL] != 0.0) {
+      reason = TSVM_CHOL_FAIL;
+      if (prm->verbose)
+        printf("Cholesky fact info error; solver becomes unstable. Terminate.\n");
+      break;
+    }
+    const double mu = hsc[SL_MU] / (2.0 * n);
+    const double normdx = sqrt(hsc[SL_NDX2]);
+    const double normdy = fabs(hsc[SL_AX]);
+    const double tmp = 0.5 * hsc[SL_XQX];
+    const double pobj = tmp - hsc[SL_SX];
+    const double dobj = -tmp - C * hsc[SL_SXI];
+    const double gap = (pobj - dobj) / (1.0 + fabs(pobj));
+    const double cond = hsc[SL_DMAX] / hsc[SL_DMIN];
+    inf.mu = mu;
+    inf.normdx = normdx;
+    inf.normdy = normdy;
+    inf.pobj = pobj;
+    inf.dobj = dobj;
+    inf.gap_rel = gap;
+    inf.condD = cond;
+    inf.y = hsc[SL_Y];
+    if (bf16 && hsc[SL_RHS0] > 0) inf.refine_res = sqrt(hsc[SL_RES0] / hsc[SL_RHS0]);
+    if (prm->verbose && iter % 5 == 0)
+      printf("iter %d, mu=%.3e, normdx=%.3e, normdy=%.3e max/min(D)=%.3e pobj=%.9e dobj=%.9e\n", iter,
+             mu, normdx, normdy, cond, pobj, dobj);
+    bool conv;
+    if (prm->stop_mode == TSVM_STOP_REF) {
+      conv = mu < 1e-7 && normdx < 1e-7 && normdy < 1e-7;
+    } else {
+      conv = gap <= prm->tol && normdx / (1.0 + sqrt((double)n)) <= prm->tol &&
+             normdy <= prm->tol * (1.0 + hsc[SL_L1] / C);
+    }
+    if (conv) {
+      reason = TSVM_CONVERGED;
+      if (prm->verbose) {
+        printf("Converged!\n");
+        printf("iter %d, mu=%.3e, normdx=%.3e, normdy=%.3e max/min(D)=%.3e pobj=%.9e dobj=%.9e\n",
+               iter, mu, normdx, normdy, cond, pobj, dobj);
+      }
+      break;
+    }
+    if (cond > 1e18) {
+      reason = TSVM_ILL_COND;
+      if (prm->verbose) {
+        printf("D is too ill-conditioned %.3e! Terminating.\n", cond);
+        printf("iter %d, mu=%.3e, normdx=%.3e, normdy=%.3e max/min(D)=%.3e pobj=%.9e dobj=%.9e\n",
+               iter, mu, normdx, normdy, cond, pobj, dobj);
+      }
+      break;
+    }
+    // ---- capacitance M = I + Z^T D^-1 Z and its Cholesky (1660-1714)
+    TSVM_HIP(c, hipEventRecord(ev0, st));
+    if (bf16) {
+      TSVM_CHECK(ipm_syrk_bf16(c, ZbT, n_pad, kp, winv, n, M, nullptr));
+    } else {
+      TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, n, winv, true, M));
+      hipLaunchKernelGGL(k_add_identity, dim3((unsigned)ceil_div(kp, 256)), dim3(256), 0, st, M, (int64_t)kp);
+    }
+    TSVM_HIP(c, hipEventRecord(ev1, st));
+    pending_syrk = true;
+    TSVM_CHECK(chol_f64(c, M, kp, dinfo));
+    // ---- predictor (1727-1748)
+    hipLaunchKernelGGL(k_rhs_pred, eg, eb, 0, st, n, C, (const double*)x, (const double*)dx,
+                       (const double*)ds, (const double*)dxi, (const double*)a, rhs, n);
+    TSVM_CHECK(smw(rhs, 2, solp));
+    hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 2, (const double*)a, (const double*)solp, n, red);
+    TSVM_CHECK(reduce({RED_SUM, RED_SUM}, {SL_AR7, SL_AB}));
+    hipLaunchKernelGGL(k_rhs3, eg, eb, 0, st, n, 0, (const double*)a, (const double*)rhs, sc, rhs3);
+    TSVM_CHECK(smw(rhs3, 1, sol3));
+    hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 0, (const double*)x, (const double*)s,
+                       (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
+                       (const double*)ads, (const double*)adxi, red);
+    TSVM_CHECK(reduce({RED_MIN}, {SL_ALPHA_AFF}));
+    hipLaunchKernelGGL(k_muaff, eg, eb, 0, st, n, C, (const double*)x, (const double*)s,
+                       (const double*)xi, (const double*)dx, (const double*)ds, (const double*)dxi,
+                       (const double*)sc, red);
+    TSVM_CHECK(reduce({RED_SUM}, {SL_MUAFF}));
+    // ---- corrector (1750-1777)
+    hipLaunchKernelGGL(k_rhs_corr, eg, eb, 0, st, n, C, (const double*)x, dx, ds, dxi, adx, ads, adxi,
+                       (const double*)sc, rhs);
+    TSVM_CHECK(smw(rhs, 1, solc));
+    hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 1, (const double*)a, (const double*)solc, n, red);
+    TSVM_CHECK(reduce({RED_SUM}, {SL_AR7C}));
+    hipLaunchKernelGGL(k_rhs3, eg, eb, 0, st, n, 1, (const double*)a, (const double*)rhs, sc, rhs3);
+    TSVM_CHECK(smw(rhs3, 1, sol3));
+    hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 1, (const double*)x, (const double*)s,
+                       (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
+                       (const double*)ads, (const double*)adxi, red);
+    TSVM_CHECK(reduce({RED_MIN}, {SL_ALPHA}));
+    hipLaunchKernelGGL(k_update, eg, eb, 0, st, n, x, s, xi, (const double*)dx, (const double*)ds,
+                       (const double*)dxi, sc, (const int*)dinfo);
+    TSVM_HIP(c, hipGetLastError());
+  }
+  TSVM_HIP(c, hipStreamSynchronize(st));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  inf.iters = iter;
+  inf.reason = reason;
+  inf.t_total = now() - t_start;
+  inf.t_syrk = t_syrk;
+  if (t_syrk > 0) inf.syrk_tflops = (double)n * k * (k + 1) * n_syrk / t_syrk * 1e-12;
+  if (info) *info = inf;
+  return TSVM_OK;
+}
+
+}  // namespace tsvm
+
+using tsvm::Ctx;
+
+extern "C" {
+
+void tsvm_ipm_default_params(tsvm_ipm_params* p) {
+  if (!p) return;
+  p->C = 1.0;
+  p->kernel_type = 2;
+  p->maxit = 100;
+  p->stop_mode = TSVM_STOP_REF;
+  p->tol = 1e-5;
+  p->precision = TSVM_PREC_FP32;
+  p->refine = 1;
+  p->verbose = 0;
+}
+
+tsvm_status tsvm_ipm_dev(tsvm_ctx* ctx, const float* Z, const float* a, int64_t n, int64_t k,
+                         const tsvm_ipm_params* prm, double* alpha, double* xi, double* s,
+                         tsvm_ipm_info* info) {
+  if (!ctx || !Z || !a || !prm || !alpha || !xi) return TSVM_ERR_ARG;
+  (void)hipSetDevice(ctx->c.device);
+  return tsvm::ipm_dev(&ctx->c, Z, a, n, k, prm, alpha, xi, s, info);
+}
+
+tsvm_status tsvm_ipm(tsvm_ctx* ctx, const float* Z, const float* a, int64_t n, int64_t k,
+                     const tsvm_ipm_params* prm, double* alpha, double* xi, double* s,
+                     tsvm_ipm_info* info) {
+  if (!ctx || !Z || !a || !prm || !alpha || !xi || n <= 0 || k <= 0) return TSVM_ERR_ARG;
+  Ctx* c = &ctx->c;
+  (void)hipSetDevice(c->device);
+  hipStream_t st = c->stream;
+  void *dZ, *da, *dal, *dxi, *ds;
+  TSVM_HIP(c, c->ws.get("h_ipmZ", n * k * 4, &dZ));
+  TSVM_HIP(c, c->ws.get("h_ipma", n * 4, &da));
+  TSVM_HIP(c, c->ws.get("h_ipmx", n * 8, &dal));
+  TSVM_HIP(c, c->ws.get("h_ipmxi", n * 8, &dxi));
+  TSVM_HIP(c, c->ws.get("h_ipms", n * 8, &ds));
+  TSVM_HIP(c, hipMemcpyAsync(dZ, Z, n * k * 4, hipMemcpyHostToDevice, st));
+  TSVM_HIP(c, hipMemcpyAsync(da, a, n * 4, hipMemcpyHostToDevice, st));
+  TSVM_CHECK(tsvm::ipm_dev(c, (float*)dZ, (float*)da, n, k, prm, (double*)dal, (double*)dxi,
+                           (double*)ds, info));
+  TSVM_HIP(c, hipMemcpyAsync(alpha, dal, n * 8, hipMemcpyDeviceToHost, st));
+  TSVM_HIP(c, hipMemcpyAsync(xi, dxi, n * 8, hipMemcpyDeviceToHost, st));
+  if (s) TSVM_HIP(c, hipMemcpyAsync(s, ds, n * 8, hipMemcpyDeviceToHost, st));
+  TSVM_HIP(c, hipStreamSynchronize(st));
+  return TSVM_OK;
+}
+
+}  // extern "C"

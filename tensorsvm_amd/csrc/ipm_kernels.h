@@ -1,0 +1,51 @@
+// Device-side scalars and launchers of the interior-point iteration (ipm.hip, ipm.cpp).
+#pragma once
+#include "internal.h"
+
+namespace tsvm {
+
+// Slots of the device scalar array (double sc[IPM_NSLOT]).  Reductions write the raw
+// sums; the consumers normalise.
+enum IpmSlot {
+  SL_Y = 0,      // equality multiplier y
+  SL_MU,         // sum x s + (C - x) xi
+  SL_NDX2,       // |r_x|^2
+  SL_AX,         // a^T x
+  SL_XQX,        // x^T Z Z^T x
+  SL_SX,         // sum x
+  SL_SXI,        // sum xi
+  SL_L1,         // sum |x|
+  SL_DMAX,       // max D
+  SL_DMIN,       // min D
+  SL_AR7,        // a^T r7 (predictor)
+  SL_AB,         // a^T b, b = (D + Z Z^T)^-1 a
+  SL_ALPHA_AFF,  // predictor step bound
+  SL_MUAFF,      // sum of complementarity at the predictor point
+  SL_AR7C,       // a^T r7 (corrector)
+  SL_ALPHA,      // step bound of the combined direction
+  SL_DY,         // predictor dy
+  SL_DYC,        // corrector dy
+  SL_CHOLFAIL,   // 1 when the capacitance Cholesky failed (the step was not taken)
+  SL_RES0,       // |refinement residual|^2, RHS 0 (before / after)
+  SL_RES1,
+  SL_RHS0,       // |rhs|^2 of RHS 0
+  SL_QQ,         // init: |Z Z^T 1|^2
+  SL_QE,         // init: 1^T Z Z^T 1
+  IPM_NSLOT = 32
+};
+
+enum RedOp { RED_SUM = 0, RED_MAX = 1, RED_MIN = 2 };
+
+// T (p x kp) = Z^T V with V p vectors of length n (row q at V + q*ldv); Z n x kp FP32.
+tsvm_status ipm_gemv_t(Ctx* c, const float* Z, int64_t n, int kp, const double* V, int64_t ldv,
+                       int p, double* T);
+// E (p x n, row q at E + q*lde) = Z T, T p x kp.
+tsvm_status ipm_gemv_n(Ctx* c, const float* Z, int64_t n, int kp, const double* T, int p,
+                       double* E, int64_t lde);
+// Capacitance from the bf16 transposed factor: M (kp x kp FP64) = I + Zb^T diag(w) Zb.
+tsvm_status ipm_syrk_bf16(Ctx* c, const __bf16* ZbT, int64_t n_pad, int kp, const double* w,
+                          int64_t n, double* M, double* tflops_out);
+// ZbT (kp x n_pad bf16) = transpose of Z (n x kp FP32), zero-padded columns.
+tsvm_status ipm_make_zbt(Ctx* c, const float* Z, int64_t n, int kp, int64_t n_pad, __bf16* ZbT);
+
+}  // namespace tsvm

@@ -29,6 +29,9 @@
 
 namespace tsvm {
 
+// 32-j chunks per pipeline stage of the BF16 kernel (the j padding is a multiple of it)
+#define KBF16_CH 2
+
 // position of j (0..31) inside a permuted 32-j chunk row (see kprod_bf16 B operand)
 __device__ __host__ inline int perm32(int j) {
   int kap = j >> 4, jj = j & 15;
@@ -185,21 +188,22 @@ __global__ void prep_A_f32(const float* __restrict__ A, int64_t lda, const float
 // lane per row block, then 2*NT MFMA 32x32x16 per row block for O += E.A'.
 // LDS (double-buffered): Qop chunk 32 x KD, AT chunk BN x 32 (row stride padded).
 // -------------------------------------------------------------------------------------
-template <int NT, int KD>
+template <int NT, int KD, int CH>
 __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
                                                   const __bf16* __restrict__ Q,
                                                   const __bf16* __restrict__ AT,
                                                   const float* __restrict__ y1,
                                                   float* __restrict__ W, int64_t ldw, int64_t m,
-                                                  int64_t r, int64_t n_chunks, int64_t r_pad) {
+                                                  int64_t r, int64_t n_stages, int64_t r_pad) {
   constexpr int BN = NT * 32;
   constexpr int KS = KD / 16;
-  constexpr int QS = KD + 8;   // LDS row stride (bf16) of the Qop chunk
-  constexpr int AS = 32 + 8;   // LDS row stride (bf16) of the AT chunk
-  constexpr int QV = 32 * KD / 8;   // 16-byte vectors per Qop chunk
-  constexpr int AV = BN * 32 / 8;   // 16-byte vectors per AT chunk
-  constexpr int NV = (QV + AV + 255) / 256;
-  __shared__ __attribute__((aligned(16))) __bf16 sQ[2][32 * QS];
+  constexpr int QS = KD + 8;        // LDS row stride (bf16) of the Qop stage
+  constexpr int AS = 32 * CH + 8;   // LDS row stride (bf16) of the AT stage
+  constexpr int QV = 32 * CH * KD / 8;   // 16-byte vectors per Qop stage (contiguous)
+  constexpr int AV = BN * 32 * CH / 8;   // 16-byte vectors per AT stage
+  constexpr int NQ = (QV + 255) / 256;
+  constexpr int NA = (AV + 255) / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 sQ[2][32 * CH * QS];
   __shared__ __attribute__((aligned(16))) __bf16 sA[2][BN * AS];
 
   const int tid = threadIdx.x;
@@ -224,28 +228,43 @@ __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
 #pragma unroll
       for (int q = 0; q < 16; q++) acc[rb][ct][q] = 0.0f;
 
-  uint4 stage[NV];
+  // Staging: the Qop stage is one contiguous run; the AT stage is CH chunks, each a
+  // contiguous [r_pad][32] block of which this column block reads BN rows.  Loads are
+  // branch-free (out-of-range lanes re-read a valid vector and skip the store) so that
+  // they stay in flight across the compute of the current stage.
+  u32x4_t sq[NQ], sa[NA];
   auto gload = [&](int64_t t) {
-    const uint4* gq = (const uint4*)(Q + t * 32 * KD);
-    const uint4* ga = (const uint4*)(AT + (t * r_pad + c0) * 32);
+    const u32x4_t* gq = (const u32x4_t*)(Q + t * (32 * CH) * KD);
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
+    for (int v = 0; v < NQ; v++) {
       int idx = tid + v * 256;
-      if (idx < QV) stage[v] = gq[idx];
-      else if (idx < QV + AV) stage[v] = ga[idx - QV];
+      sq[v] = gq[idx < QV ? idx : QV - 1];
+    }
+#pragma unroll
+    for (int v = 0; v < NA; v++) {
+      int idx = tid + v * 256;
+      idx = idx < AV ? idx : AV - 1;
+      int ch = idx / (BN * 4), rem = idx - ch * (BN * 4);
+      const u32x4_t* ga = (const u32x4_t*)(AT + ((t * CH + ch) * r_pad + c0) * 32);
+      sa[v] = ga[rem];
     }
   };
   auto sstore = [&](int buf) {
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
+    for (int v = 0; v < NQ; v++) {
       int idx = tid + v * 256;
-      if (idx < QV) {
+      if (QV % 256 == 0 || idx < QV) {
         int row = idx / (KD / 8), col = (idx % (KD / 8)) * 8;
-        *(uint4*)&sQ[buf][row * QS + col] = stage[v];
-      } else if (idx < QV + AV) {
-        int a = idx - QV;
-        int row = a / 4, col = (a % 4) * 8;
-        *(uint4*)&sA[buf][row * AS + col] = stage[v];
+        *(u32x4_t*)&sQ[buf][row * QS + col] = sq[v];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NA; v++) {
+      int idx = tid + v * 256;
+      if (AV % 256 == 0 || idx < AV) {
+        int ch = idx / (BN * 4), rem = idx - ch * (BN * 4);
+        int row = rem / 4, col = ch * 32 + (rem % 4) * 8;
+        *(u32x4_t*)&sA[buf][row * AS + col] = sa[v];
       }
     }
   };
@@ -253,42 +272,52 @@ __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
   gload(0);
   sstore(0);
   __syncthreads();
-  for (int64_t t = 0; t < n_chunks; t++) {
+  for (int64_t t = 0; t < n_stages; t++) {
     const int buf = (int)(t & 1);
-    if (t + 1 < n_chunks) gload(t + 1);
-    // distance tile S^T (32 j x 32 i) for both row blocks
-    f32x16_t s[2];
+    // prefetch the next stage into registers (the last iteration re-reads its own
+    // stage: unconditional, so the loads stay in flight across the MFMA work below)
+    gload(t + 1 < n_stages ? t + 1 : t);
+    __builtin_amdgcn_sched_barrier(0);  // issue the prefetch before the MFMA work
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+    for (int ch = 0; ch < CH; ch++) {
+      // distance tile S^T (32 j x 32 i) for both row blocks
+      f32x16_t s[2];
 #pragma unroll
-      for (int q = 0; q < 16; q++) s[rb][q] = 0.0f;
+      for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-      bf16x8_t qa = *(const bf16x8_t*)&sQ[buf][l32 * QS + ks * 16 + 8 * h];
-      s[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa, pb[0][ks], s[0], 0, 0, 0);
-      s[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa, pb[1][ks], s[1], 0, 0, 0);
-    }
-    // E = exp2(S^T): lane holds i = l32 and j = 8(q/4) + 4h + q%4; registers 8k..8k+7
-    // form the A operand of K-half k (j in 16k + {4h..4h+3, 8+4h..8+4h+3}).
-    bf16x8_t e[2][2];
+        for (int q = 0; q < 16; q++) s[rb][q] = 0.0f;
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+      for (int ks = 0; ks < KS; ks++) {
+        bf16x8_t qa = *(const bf16x8_t*)&sQ[buf][(ch * 32 + l32) * QS + ks * 16 + 8 * h];
+        s[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa, pb[0][ks], s[0], 0, 0, 0);
+        s[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qa, pb[1][ks], s[1], 0, 0, 0);
+      }
+      // E = exp2(S^T): lane holds i = l32 and j = 8(q/4) + 4h + q%4; registers 8k..8k+7
+      // form the A operand of K-half k (j in 16k + {4h..4h+3, 8+4h..8+4h+3}).
+      bf16x8_t e[2][2];
 #pragma unroll
-      for (int kap = 0; kap < 2; kap++)
+      for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-        for (int tt = 0; tt < 8; tt++)
-          e[rb][kap][tt] = (__bf16)__builtin_amdgcn_exp2f(s[rb][8 * kap + tt]);
-    // O += E . A'_j
+        for (int kap = 0; kap < 2; kap++)
 #pragma unroll
-    for (int ct = 0; ct < NT; ct++) {
+          for (int tt = 0; tt < 8; tt++)
+            e[rb][kap][tt] = (__bf16)__builtin_amdgcn_exp2f(s[rb][8 * kap + tt]);
+      // O += E . A'_j
 #pragma unroll
-      for (int kap = 0; kap < 2; kap++) {
-        bf16x8_t bv = *(const bf16x8_t*)&sA[buf][(ct * 32 + l32) * AS + (2 * kap + h) * 8];
-        acc[0][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[0][kap], bv, acc[0][ct], 0, 0, 0);
-        acc[1][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[1][kap], bv, acc[1][ct], 0, 0, 0);
+      for (int ct = 0; ct < NT; ct++) {
+#pragma unroll
+        for (int kap = 0; kap < 2; kap++) {
+          bf16x8_t bv =
+              *(const bf16x8_t*)&sA[buf][(ct * 32 + l32) * AS + ch * 32 + (2 * kap + h) * 8];
+          acc[0][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[0][kap], bv, acc[0][ct], 0, 0, 0);
+          acc[1][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[1][kap], bv, acc[1][ct], 0, 0, 0);
+        }
       }
     }
-    if (t + 1 < n_chunks) sstore(buf ^ 1);
+    // keep the LDS stores of the next stage after this stage's MFMA work
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    sstore(buf ^ 1);
     __syncthreads();
   }
   // epilogue: W[i, c] = y1_i * O[i, c]; acc register q holds row 8(q/4) + 4h + q%4
@@ -345,19 +374,19 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[ct][q] = 0.0f;
 
-  float4 stage[NV];
+  f32x4_t stage[NV];
   auto gload = [&](int64_t t) {
-    const float4* gq = (const float4*)(Q + t * 32 * KF);
+    const f32x4_t* gq = (const f32x4_t*)(Q + t * 32 * KF);
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       int idx = tid + v * 256;
-      if (idx < QV) {
-        stage[v] = gq[idx];
-      } else if (idx < QV + AV) {
-        int a = idx - QV;
-        int row = a / (BN / 4), col = (a % (BN / 4)) * 4;
-        stage[v] = *(const float4*)(Af + (t * 32 + row) * r_pad + c0 + col);
-      }
+      // branch-free: both addresses are formed, one is selected (in-range index clamped)
+      int a = idx - QV;
+      a = a < 0 ? 0 : (a < AV ? a : AV - 1);
+      int row = a / (BN / 4), col = (a % (BN / 4)) * 4;
+      const f32x4_t* pa = (const f32x4_t*)(Af + (t * 32 + row) * r_pad + c0 + col);
+      const f32x4_t* pq = gq + (idx < QV ? idx : 0);
+      stage[v] = *(idx < QV ? pq : pa);
     }
   };
   auto sstore = [&]() {
@@ -365,11 +394,11 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
     for (int v = 0; v < NV; v++) {
       int idx = tid + v * 256;
       if (idx < QV) {
-        *(float4*)&sQ[idx * 4] = stage[v];
+        *(f32x4_t*)&sQ[idx * 4] = stage[v];
       } else if (idx < QV + AV) {
         int a = idx - QV;
         int row = a / (BN / 4), col = (a % (BN / 4)) * 4;
-        *(float4*)&sA[row * AS + col] = stage[v];
+        *(f32x4_t*)&sA[row * AS + col] = stage[v];
       }
     }
   };
@@ -378,7 +407,8 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
   for (int64_t t = 0; t < n_chunks; t++) {
     sstore();
     __syncthreads();
-    if (t + 1 < n_chunks) gload(t + 1);
+    gload(t + 1 < n_chunks ? t + 1 : t);
+    __builtin_amdgcn_sched_barrier(0);
     f32x16_t s;
 #pragma unroll
     for (int q = 0; q < 16; q++) s[q] = 0.0f;
@@ -397,6 +427,8 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
                                                         0, 0, 0);
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
     __syncthreads();
   }
 #pragma unroll
@@ -419,9 +451,10 @@ template <int NT, int KD>
 static void launch_bf16(const __bf16* P, const __bf16* Q, const __bf16* AT, const float* y1,
                         float* W, int64_t ldw, int64_t m, int64_t r, int64_t n_chunks,
                         int64_t r_pad, int64_t m_pad, hipStream_t s) {
+  constexpr int CH = KBF16_CH;
   dim3 grid((unsigned)(m_pad / 256), (unsigned)(r_pad / (NT * 32)));
-  hipLaunchKernelGGL((kprod_bf16<NT, KD>), grid, dim3(256), 0, s, P, Q, AT, y1, W, ldw, m, r,
-                     n_chunks, r_pad);
+  hipLaunchKernelGGL((kprod_bf16<NT, KD, CH>), grid, dim3(256), 0, s, P, Q, AT, y1, W, ldw, m, r,
+                     n_chunks / CH, r_pad);
 }
 template <int NT, int KF>
 static void launch_f32(const float* P, const float* Q, const float* Af, const float* y1,
@@ -442,8 +475,8 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
                 (long)n, (long)d, (long)r);
   hipStream_t s = c->stream;
   const double gl2 = gamma * TSVM_LOG2E;
-  const int64_t n_pad = round_up(n > 0 ? n : 1, 32), n_chunks = n_pad / 32;
   if (precision == TSVM_PREC_BF16) {
+    const int64_t n_pad = round_up(n > 0 ? n : 1, 32 * KBF16_CH), n_chunks = n_pad / 32;
     // operand width: the smallest compiled size holding [hi, hi, lo, norm terms]
     static const int kds[] = {16, 32, 48, 64, 80, 96, 112, 128, 176, 256};
     int KD = 0;
@@ -487,6 +520,7 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
     }
 #undef KP_BF16_CASE
   } else if (precision == TSVM_PREC_FP32) {
+    const int64_t n_pad = round_up(n > 0 ? n : 1, 32), n_chunks = n_pad / 32;
     static const int kfs[] = {4, 8, 12, 16, 20, 24, 34, 56, 64, 96, 130};
     int KF = 0;
     for (int v : kfs)

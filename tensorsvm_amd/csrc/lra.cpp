@@ -1,0 +1,225 @@
+// Low-rank factor construction (replaces LRA, tensorsvm-train.cu:1941-2249).
+//
+// Everything of size n stays resident in HBM; only k x k matrices cross to the host.
+//   1. W = K_Y Omega                      fused kernel-product kernel (FP32 or BF16 MFMA)
+//   2. W <- orth(W)                       shifted Cholesky-QR (Fukaya et al., SIAM J. Sci.
+//                                         Comput. 2020): G = W^T W on FP64 MFMA, host
+//                                         Cholesky of G + s I, W <- W L^{-T}; repeated until
+//                                         G = I to 1e-6 (at most 3 passes)
+//   3. Wp = W rounded to the product precision (BF16 form: bf16 values)
+//      Gp = Wp^T Wp                       exact Gram of the basis actually used
+//   4. Q = K_Y Wp                         second kernel product
+//   5. Cb = Wp^T Q (FP64), symmetrised    (the reference's C, 2122-2156)
+//   6. host: Gp = L L^T, C = L^-1 Cb L^-T = V diag(w) V^T (dsyevd), X = L^-T V sqrt(w+)
+//      -- with Wo = Wp L^-T orthonormal this is exactly the reference's C = Wo^T K_Y Wo,
+//      so U U^T = Wo C+ Wo^T is the same projected kernel even when Wp is only nearly
+//      orthonormal (bf16 rounding)
+//   7. U = Wp X                           FP64 MFMA, FP32 output (the reference's U = W X)
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <chrono>
+#include <vector>
+
+#include "internal.h"
+
+namespace tsvm {
+
+namespace {
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d, int64_t k,
+                    double gamma, int precision, float* Om, float* U, tsvm_lra_info* info) {
+  if (n <= 0 || d <= 0 || k <= 0 || k > n)
+    return fail(c, TSVM_ERR_ARG, "lra: need 0 < k <= n (n=%ld k=%ld)", (long)n, (long)k);
+  if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16)
+    return fail(c, TSVM_ERR_ARG, "lra: unknown precision %d", precision);
+  hipStream_t s = c->stream;
+  tsvm_lra_info inf;
+  memset(&inf, 0, sizeof(inf));
+  const double t0 = now();
+  void *pW, *pG, *pB;
+  TSVM_HIP(c, c->ws.get("lra_W", (size_t)(n * k * 4), &pW));
+  TSVM_HIP(c, c->ws.get("lra_G", (size_t)(k * k * 8), &pG));
+  TSVM_HIP(c, c->ws.get("lra_B", (size_t)(k * k * 8), &pB));
+  float* W = (float*)pW;
+  double* dG = (double*)pG;
+  double* dB = (double*)pB;
+  std::vector<double> G(k * k), L(k * k), Li(k * k), T(k * k);
+
+  // 1. W = K_Y Omega (timed with events on the stream it runs on)
+  hipEvent_t e0, e1;
+  TSVM_HIP(c, hipEventCreate(&e0));
+  TSVM_HIP(c, hipEventCreate(&e1));
+  TSVM_HIP(c, hipEventRecord(e0, s));
+  TSVM_CHECK(kernel_product_dev(c, X, y, n, X, y, n, d, Om, k, k, gamma, precision, W, k));
+  TSVM_HIP(c, hipEventRecord(e1, s));
+  TSVM_HIP(c, hipStreamSynchronize(s));
+  float ms = 0.f;
+  TSVM_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  double t1 = now();
+  inf.t_product1 = ms * 1e-3;
+
+  // 2. shifted Cholesky-QR; the basis ping-pongs between W and U
+  float* cur = W;
+  float* oth = U;
+  bool have_gram = false;
+  for (int pass = 0; pass < 3; pass++) {
+    TSVM_CHECK(gemm_tn_f64(c, cur, k, k, cur, k, k, n, nullptr, true, dG));
+    TSVM_HIP(c, hipMemcpyAsync(G.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
+    TSVM_HIP(c, hipStreamSynchronize(s));
+    double dev = 0.0, tr = 0.0;
+    for (int64_t i = 0; i < k; i++) {
+      tr += G[i * k + i];
+      for (int64_t j = 0; j < k; j++) dev = fmax(dev, fabs(G[i * k + j] - (i == j ? 1.0 : 0.0)));
+    }
+    if (!(tr > 0.0) || !std::isfinite(tr))
+      return fail(c, TSVM_ERR_NUMERIC, "lra: range-finder block has norm %g", tr);
+    have_gram = true;
+    if (dev < 1e-6) break;  // orthonormal to FP32 storage accuracy
+    inf.orth_passes = pass + 1;
+    // shift (pass 0 only) keeps the Cholesky of an ill-conditioned Gram defined
+    double shift = pass == 0 ? 11.0 * ((double)n * k + (double)k * (k + 1)) * 1.11e-16 * tr : 0.0;
+    int fails = 0;
+    for (;;) {
+      memcpy(L.data(), G.data(), k * k * 8);
+      for (int64_t i = 0; i < k; i++) L[i * k + i] += shift;
+      if (host_chol(L.data(), k) == 0) break;
+      shift = shift > 0 ? shift * 10.0 : 1e-14 * tr;
+      if (++fails > 30) return fail(c, TSVM_ERR_NUMERIC, "lra: Cholesky-QR failed");
+    }
+    host_tri_inv_lower(L.data(), k, Li.data());
+    for (int64_t i = 0; i < k; i++)  // B = L^{-T}
+      for (int64_t j = 0; j < k; j++) T[i * k + j] = Li[j * k + i];
+    TSVM_HIP(c, hipMemcpyAsync(dB, T.data(), k * k * 8, hipMemcpyHostToDevice, s));
+    TSVM_CHECK(gemm_nn_tall(c, cur, k, k, dB, k, n, oth, k));
+    std::swap(cur, oth);
+    have_gram = false;
+  }
+  if (cur != W) {
+    TSVM_HIP(c, hipMemcpyAsync(W, cur, n * k * 4, hipMemcpyDeviceToDevice, s));
+    cur = W;
+  }
+  // 3. basis in the product precision and its exact Gram
+  if (precision == TSVM_PREC_BF16) {
+    TSVM_CHECK(round_bf16_inplace(c, W, n * k));
+    have_gram = false;
+  }
+  if (!have_gram) {
+    TSVM_CHECK(gemm_tn_f64(c, W, k, k, W, k, k, n, nullptr, true, dG));
+    TSVM_HIP(c, hipMemcpyAsync(G.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
+  }
+  TSVM_HIP(c, hipStreamSynchronize(s));
+  double t2 = now();
+  inf.t_orth = t2 - t1;
+
+  // 4. Q = K_Y Wp (into the Omega buffer)
+  float* Q = Om;
+  TSVM_HIP(c, hipEventRecord(e0, s));
+  TSVM_CHECK(kernel_product_dev(c, X, y, n, X, y, n, d, W, k, k, gamma, precision, Q, k));
+  TSVM_HIP(c, hipEventRecord(e1, s));
+  TSVM_HIP(c, hipStreamSynchronize(s));
+  TSVM_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  double t3 = now();
+  inf.t_product2 = ms * 1e-3;
+
+  // 5. Cb = Wp^T Q
+  std::vector<double> Cb(k * k), Cs(k * k), w(k), V(k * k);
+  TSVM_CHECK(gemm_tn_f64(c, W, k, k, Q, k, k, n, nullptr, false, dG));
+  TSVM_HIP(c, hipMemcpyAsync(Cb.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
+  TSVM_HIP(c, hipStreamSynchronize(s));
+  for (int64_t i = 0; i < k; i++)
+    for (int64_t j = 0; j < k; j++) Cs[i * k + j] = 0.5 * (Cb[i * k + j] + Cb[j * k + i]);
+  // 6. C = L^-1 Cs L^-T, eig, X = L^-T V sqrt(w+)
+  memcpy(L.data(), G.data(), k * k * 8);
+  if (host_chol(L.data(), k) != 0) return fail(c, TSVM_ERR_NUMERIC, "lra: basis Gram not SPD");
+  host_tri_inv_lower(L.data(), k, Li.data());
+  host_gemm(Li.data(), Cs.data(), k, T.data());  // T = L^-1 Cs
+  std::vector<double> LiT(k * k);
+  for (int64_t i = 0; i < k; i++)
+    for (int64_t j = 0; j < k; j++) LiT[i * k + j] = Li[j * k + i];
+  host_gemm(T.data(), LiT.data(), k, Cb.data());  // Cb = L^-1 Cs L^-T
+  for (int64_t i = 0; i < k; i++)
+    for (int64_t j = 0; j < i; j++) {
+      double v = 0.5 * (Cb[i * k + j] + Cb[j * k + i]);
+      Cb[i * k + j] = v;
+      Cb[j * k + i] = v;
+    }
+  TSVM_CHECK(host_sym_eig(c, Cb.data(), k, w.data(), V.data()));
+  int64_t realk = k;
+  for (int64_t j = 0; j < k; j++) {
+    double sc = 0.0;
+    if (w[j] > 0) sc = sqrt(w[j]);
+    else realk--;
+    for (int64_t i = 0; i < k; i++) V[i * k + j] *= sc;
+  }
+  host_gemm(LiT.data(), V.data(), k, T.data());  // X = L^-T V sqrt(w+)
+  inf.realk = realk;
+  inf.eig_max = w[k - 1];
+  inf.eig_min = w[0];
+  TSVM_HIP(c, hipMemcpyAsync(dB, T.data(), k * k * 8, hipMemcpyHostToDevice, s));
+  double t4 = now();
+  inf.t_core = t4 - t3;
+
+  // 7. U = Wp X
+  TSVM_CHECK(gemm_nn_tall(c, W, k, k, dB, k, n, U, k));
+  TSVM_HIP(c, hipStreamSynchronize(s));
+  double t5 = now();
+  inf.t_factor = t5 - t4;
+  inf.t_total = t5 - t0;
+  if (info) *info = inf;
+  return TSVM_OK;
+}
+
+}  // namespace tsvm
+
+using tsvm::Ctx;
+
+extern "C" {
+
+tsvm_status tsvm_lra_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t n, int64_t d,
+                         int64_t k, double gamma, int precision, float* omega, float* U,
+                         tsvm_lra_info* info) {
+  if (!ctx || !X || !y || !omega || !U) return TSVM_ERR_ARG;
+  Ctx* c = &ctx->c;
+  hipSetDevice(c->device);
+  return tsvm::lra_dev(c, X, y, n, d, k, gamma, precision, omega, U, info);
+}
+
+tsvm_status tsvm_lra(tsvm_ctx* ctx, const float* X, const float* y, int64_t n, int64_t d,
+                     int64_t k, double gamma, int precision, const float* omega, float* U,
+                     tsvm_lra_info* info) {
+  if (!ctx || !X || !y || !U || n <= 0 || d <= 0 || k <= 0) return TSVM_ERR_ARG;
+  Ctx* c = &ctx->c;
+  hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  void *dX, *dy, *dOm, *dU;
+  TSVM_HIP(c, c->ws.get("h_lraX", n * d * 4, &dX));
+  TSVM_HIP(c, c->ws.get("h_lray", n * 4, &dy));
+  TSVM_HIP(c, c->ws.get("h_lraOm", n * k * 4, &dOm));
+  TSVM_HIP(c, c->ws.get("h_lraU", n * k * 4, &dU));
+  TSVM_HIP(c, hipMemcpyAsync(dX, X, n * d * 4, hipMemcpyHostToDevice, s));
+  TSVM_HIP(c, hipMemcpyAsync(dy, y, n * 4, hipMemcpyHostToDevice, s));
+  if (omega) {
+    TSVM_HIP(c, hipMemcpyAsync(dOm, omega, n * k * 4, hipMemcpyHostToDevice, s));
+    TSVM_HIP(c, hipStreamSynchronize(s));
+  } else {
+    std::vector<float> om(n * k);
+    TSVM_CHECK(tsvm_gaussrand_fill(om.data(), n * k, 0));
+    TSVM_HIP(c, hipMemcpyAsync(dOm, om.data(), n * k * 4, hipMemcpyHostToDevice, s));
+    TSVM_HIP(c, hipStreamSynchronize(s));
+  }
+  TSVM_CHECK(tsvm::lra_dev(c, (float*)dX, (float*)dy, n, d, k, gamma, precision, (float*)dOm,
+                           (float*)dU, info));
+  TSVM_HIP(c, hipMemcpyAsync(U, dU, n * k * 4, hipMemcpyDeviceToHost, s));
+  TSVM_HIP(c, hipStreamSynchronize(s));
+  return TSVM_OK;
+}
+
+}  // extern "C"

@@ -43,6 +43,7 @@ def lib():
         L.orc_mpc.argtypes = [P, P, I64, I64, D, ctypes.c_int, ctypes.c_int, ctypes.c_int, D,
                               P, P, P, ctypes.POINTER(IpmInfo)]
         L.orc_smw_solve.argtypes = [P, P, I64, I64, P]
+        L.orc_newton_direction.argtypes = [P, P, I64, I64, D, P, P, P, P]
         L.orc_bias_rbf.argtypes = [P, P, P, I64, I64, D, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(D)]
         L.orc_bias_rbf.restype = D
         L.orc_bias_linear.argtypes = [P, P, P, I64, I64, D, P]
@@ -154,6 +155,14 @@ def smw_solve(Z, Dv, b):
     n, k = Z.shape
     lib().orc_smw_solve(_p(Z), _p(Dv), n, k, _p(b))
     return b
+
+
+def newton_direction(Z, a, C, X, S, Xi, r):
+    Z, a, X, S, Xi = f64(Z), f64(a), f64(X), f64(S), f64(Xi)
+    r = f64(r).copy()
+    n, k = Z.shape
+    lib().orc_newton_direction(_p(Z), _p(a), n, k, C, _p(X), _p(S), _p(Xi), _p(r))
+    return r
 
 
 def bias_rbf(alpha, labels, U, C):

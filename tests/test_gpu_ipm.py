@@ -1,0 +1,73 @@
+"""GPU interior-point solver (through the C-ABI) against the oracle's mpc and sklearn."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(n, k, seed, sep=0.05):
+    rng = np.random.default_rng(seed)
+    y = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    U = rng.normal(size=(n, k)) * 0.6 + sep * y[:, None] * rng.normal(size=(1, k))
+    Z = (y[:, None] * U).astype(np.float32)
+    return Z, y.astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ["a", "b"])
+def test_ipm_fp32_vs_sklearn_fixture(ctx, name):
+    import tensorsvm_amd as T
+    f = np.load(f"tests/golden/ipm_svc_{name}.npz")
+    C = float(f["C"])
+    Z = f["Z"].astype(np.float32)
+    al, xi, s, info = ctx.ipm(Z, f["y"], C, precision=T.PREC_FP32)
+    assert info.reason == 0, info.as_dict()
+    assert np.abs(al - f["alpha"]).max() <= 1e-4 * C
+    assert abs(info.y - (-float(f["b"]))) <= 1e-3 * (1 + abs(float(f["b"])))
+    # and the oracle on the identical FP32 factor agrees more tightly
+    Xo, Xio, So, io = O.mpc(Z.astype(np.float64), f["y"], C)
+    assert np.abs(al - Xo).max() <= 1e-6 * C
+    assert info.iters == io.iters
+
+
+@pytest.mark.parametrize("n,k,C", [(5000, 32, 32.0), (3001, 50, 1.0), (8000, 100, 8.0)])
+def test_ipm_fp32_matches_oracle(ctx, n, k, C):
+    import tensorsvm_amd as T
+    Z, y = _problem(n, k, n + k)
+    al, xi, s, info = ctx.ipm(Z, y, C, precision=T.PREC_FP32)
+    Xo, Xio, So, io = O.mpc(Z.astype(np.float64), y.astype(np.float64), C)
+    assert info.reason == io.reason
+    assert np.abs(al - Xo).max() <= 1e-5 * C
+    assert abs(info.pobj - io.pobj) <= 1e-7 * (1 + abs(io.pobj))
+    # same support-vector sets apart from multipliers within 1e-6 of 0 or C
+    sv_g = al > 1e-6
+    sv_o = Xo > 1e-6
+    amb = (np.abs(Xo) < 1e-5) | (np.abs(Xo - C) < 1e-5)
+    assert np.all((sv_g == sv_o) | amb)
+
+
+@pytest.mark.parametrize("n,k,C", [(5000, 32, 32.0), (8000, 100, 8.0)])
+def test_ipm_bf16_refined_matches_oracle(ctx, n, k, C):
+    import tensorsvm_amd as T
+    Z, y = _problem(n, k, 7 * n + k)
+    al, xi, s, info = ctx.ipm(Z, y, C, precision=T.PREC_BF16, refine=2, stop=T.capi.STOP_GAP,
+                              tol=1e-8)
+    Xo, Xio, So, io = O.mpc(Z.astype(np.float64), y.astype(np.float64), C, stop="gap", tol=1e-8)
+    assert info.reason == 0, info.as_dict()
+    assert abs(info.pobj - io.pobj) <= 1e-6 * (1 + abs(io.pobj))
+    assert np.abs(al - Xo).max() <= 1e-4 * C
+
+
+def test_ipm_linear_init(ctx):
+    """kernel_type 0 (linear SVM): Z = diag(y) X, the reference's linear initial point."""
+    import tensorsvm_amd as T
+    rng = np.random.default_rng(3)
+    n, d = 2000, 10
+    y = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    X = rng.normal(size=(n, d)) + 0.5 * y[:, None]
+    Z = (y[:, None] * X).astype(np.float32)
+    al, xi, s, info = ctx.ipm(Z, y, 1.0, kernel_type=0, precision=T.PREC_FP32)
+    Xo, Xio, So, io = O.mpc(Z.astype(np.float64), y, 1.0, kernel_type=0)
+    assert info.reason == io.reason
+    assert np.abs(al - Xo).max() <= 1e-5

@@ -1,0 +1,63 @@
+"""Low-rank factor construction (HIP, through the C-ABI) against the oracle's LRA."""
+import numpy as np
+import pytest
+from scipy.spatial.distance import cdist
+
+import oracle_lib as O
+from datagen import gaussian_mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _ky(X, y, gamma):
+    X = X.astype(np.float64)
+    return y[:, None] * np.exp(-gamma * cdist(X, X, "sqeuclidean")) * y[None, :]
+
+
+@pytest.mark.parametrize("n,d,k,gamma", [(2000, 18, 32, 0.05), (1500, 54, 64, 1.0), (3000, 18, 100, 0.05)])
+def test_lra_fp32_matches_oracle(ctx, n, d, k, gamma):
+    import tensorsvm_amd as T
+    X, y = gaussian_mixture(n, d, 1000 + n)
+    om = O.gaussrand(n * k).reshape(n, k)
+    U, info = ctx.lra(X, y, k, gamma, T.PREC_FP32, omega=om)
+    Uo, w, realk = O.lra(X, y, k, gamma, om)
+    G = U.astype(np.float64) @ U.T.astype(np.float64)
+    Go = Uo @ Uo.T
+    rel = np.linalg.norm(G - Go) / np.linalg.norm(Go)
+    assert rel < 1e-4, rel
+    # the approximation error against the exact kernel agrees too
+    K = _ky(X, y, gamma)
+    e_gpu = np.linalg.norm(K - G) / np.linalg.norm(K)
+    e_orc = np.linalg.norm(K - Go) / np.linalg.norm(K)
+    assert abs(e_gpu - e_orc) <= 1e-3 * max(e_orc, 1e-3)
+    assert info.realk == realk or abs(info.realk - realk) <= 1
+    assert np.isclose(info.eig_max, w[-1], rtol=1e-4)
+
+
+def test_lra_full_rank_reproduces_kernel(ctx):
+    """k = n with distinct points: range(K_Y Omega) is everything, so U U^T = K_Y."""
+    import tensorsvm_amd as T
+    n, d, gamma = 256, 8, 0.3
+    X, y = gaussian_mixture(n, d, 7)
+    om = O.gaussrand(n * n).reshape(n, n)
+    U, info = ctx.lra(X, y, n, gamma, T.PREC_FP32, omega=om)
+    K = _ky(X, y, gamma)
+    G = U.astype(np.float64) @ U.T.astype(np.float64)
+    assert np.linalg.norm(K - G) / np.linalg.norm(K) < 1e-4
+
+
+def test_lra_bf16_close_to_fp64(ctx):
+    """BF16 form: same projected kernel up to the bf16 rounding of the kernel values."""
+    import tensorsvm_amd as T
+    n, d, k, gamma = 4000, 18, 64, 0.05
+    X, y = gaussian_mixture(n, d, 99)
+    om = O.gaussrand(n * k).reshape(n, k)
+    U, info = ctx.lra(X, y, k, gamma, T.PREC_BF16, omega=om)
+    Uo, w, realk = O.lra(X, y, k, gamma, om)
+    K = _ky(X, y, gamma)
+    G = U.astype(np.float64) @ U.T.astype(np.float64)
+    Go = Uo @ Uo.T
+    e_gpu = np.linalg.norm(K - G) / np.linalg.norm(K)
+    e_orc = np.linalg.norm(K - Go) / np.linalg.norm(K)
+    # the low-rank approximation is as good as the FP64 one, to within 2% of its error
+    assert e_gpu <= 1.02 * e_orc + 2e-3, (e_gpu, e_orc)

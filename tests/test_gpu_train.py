@@ -1,0 +1,133 @@
+"""End-to-end training (factor + IPM + bias) and the command line, against the oracle's
+plain method on the same seeded inputs and Omega."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from datagen import gaussian_mixture
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _split(n, d, seed):
+    X, y = gaussian_mixture(n, d, seed)
+    ntr = int(0.8 * n)
+    return X[:ntr], y[:ntr], X[ntr:], y[ntr:]
+
+
+def _oracle_train(X, y, k, gamma, C, om):
+    U, w, realk = O.lra(X, y, k, gamma, om)
+    al, xi, s, info = O.mpc(U, y.astype(np.float64), C)
+    b, cnt, sd = O.bias_rbf(al, y, U, C)
+    return al, b, info, U
+
+
+@pytest.mark.parametrize("n,k", [(3000, 32), (4000, 64)])
+def test_train_fp32_matches_oracle(ctx, n, k):
+    import tensorsvm_amd as T
+    gamma, C = 0.05, 32.0
+    X, y, Xt, yt = _split(n, 18, 1000 + n)
+    om = O.gaussrand(X.shape[0] * k)
+    al, _, info = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=T.PREC_FP32, omega=om)
+    alo, bo, io, U = _oracle_train(X, y, k, gamma, C, om.reshape(-1, k))
+    assert info.ipm.reason == io.reason == 0
+    # the two factors agree to ~1e-6 (Cholesky-QR vs Householder, FP32 vs FP64 products),
+    # which moves a near-degenerate QP solution by up to a few 1e-4 C
+    assert np.abs(al - alo).max() <= 1e-3 * C
+    assert abs(info.ipm.pobj - io.pobj) <= 1e-6 * (1 + abs(io.pobj))
+    amb = (alo < 1e-5) | (alo > C - 1e-5) | (np.abs(alo - 1e-6) < 1e-5)
+    assert np.all(((al > 1e-6) == (alo > 1e-6)) | amb)
+    assert abs(info.b - bo) <= 1e-3 + 3 * info.b_std
+    fg = ctx.decision_function(X, y, al, gamma, info.b, Xt)
+    fo = O.decision(alo, y, X, gamma, bo, Xt)
+    acc_g, acc_o = np.mean(np.sign(fg) == yt), np.mean(np.sign(fo) == yt)
+    assert abs(acc_g - acc_o) <= 0.001 + 1.0 / len(yt)
+    assert np.allclose(fg, O.decision(al, y, X, gamma, info.b, Xt), atol=1e-4)
+
+
+def test_train_bf16_same_accuracy(ctx):
+    import tensorsvm_amd as T
+    gamma, C, k = 0.05, 32.0, 64
+    X, y, Xt, yt = _split(6000, 18, 77)
+    om = O.gaussrand(X.shape[0] * k)
+    al32, _, i32 = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=T.PREC_FP32, omega=om,
+                             stop=T.capi.STOP_GAP, tol=1e-5)
+    al16, _, i16 = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=T.PREC_BF16, omega=om,
+                             stop=T.capi.STOP_GAP, tol=1e-5)
+    assert i16.ipm.reason == 0 and i32.ipm.reason == 0
+    assert abs(i16.ipm.pobj - i32.ipm.pobj) <= 5e-3 * abs(i32.ipm.pobj)
+    a32 = np.mean(np.sign(ctx.decision_function(X, y, al32, gamma, i32.b, Xt)) == yt)
+    a16 = np.mean(np.sign(ctx.decision_function(X, y, al16, gamma, i16.b, Xt)) == yt)
+    assert abs(a32 - a16) <= 0.01
+
+
+def _write_libsvm(path, X, y):
+    with open(path, "w") as f:
+        for xi, yi in zip(X, y):
+            f.write(("+1" if yi > 0 else "-1") + " " +
+                    " ".join(f"{j + 1}:{v:.7g}" for j, v in enumerate(xi) if v != 0) + "\n")
+
+
+def _read_model(path):
+    head, svs = {}, []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    i = lines.index("SV")
+    for ln in lines[:i]:
+        kv = ln.split(" ", 1)
+        head[kv[0]] = kv[1]
+    for ln in lines[i + 1:]:
+        parts = ln.split()
+        svs.append(float(parts[0]))
+    return head, np.array(svs)
+
+
+def test_cli_rbf_model_file(tmp_path):
+    X, y = gaussian_mixture(2000, 18, 5)
+    tr = str(tmp_path / "train.libsvm")
+    _write_libsvm(tr, X, y)
+    model = str(tmp_path / "out.model")
+    exe = os.path.join(ROOT, "tensorsvm_amd", "tensorsvm-train")
+    r = subprocess.run([exe, "-k", "32", "-c", "32", "-g", "0.05", "-t", "2", tr, model],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "found labels:" in r.stdout and "#BSV" in r.stdout
+    head, coef = _read_model(model)
+    assert head["svm_type"] == "c_svc" and head["kernel_type"] == "rbf"
+    assert head["gamma"] == "0.0500000" and head["nr_class"] == "2"
+    # the same data through the oracle (labels as read, Omega = the gaussrand stream)
+    lab, xs = O.libsvm_read(tr)
+    lab, pos, neg, bad = O.label_map(lab)
+    assert head["label"] == f"{pos} {neg}"
+    om = O.gaussrand(xs.shape[0] * 32).reshape(-1, 32)
+    alo, bo, io, U = _oracle_train(xs, lab, 32, 0.05, 32.0, om)
+    nsv = int(np.sum(alo > 1e-6))
+    assert abs(int(head["total_sv"]) - nsv) <= max(2, nsv // 200)
+    assert abs(float(head["rho"]) + bo) <= 5e-3 * (1 + abs(bo))
+    assert np.all(np.abs(coef) <= 32.0 + 1e-6)
+
+
+def test_cli_linear_model_file(tmp_path):
+    rng = np.random.default_rng(8)
+    n, d = 1500, 6
+    y = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    X = (rng.normal(size=(n, d)) + 0.7 * y[:, None]).astype(np.float32)
+    tr = str(tmp_path / "lin.libsvm")
+    _write_libsvm(tr, X, y)
+    exe = os.path.join(ROOT, "tensorsvm_amd", "tensorsvm-train")
+    r = subprocess.run([exe, "-t", "0", "-c", "1", tr], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(tr + ".model").read().splitlines()
+    assert lines[0] == "solver_type L2R_L1LOSS_SVC_DUAL" and lines[5] == "w"
+    w = np.array([float(v) for v in lines[6:6 + d]])
+    lab, xs = O.libsvm_read(tr)
+    lab, pos, neg, bad = O.label_map(lab)
+    Z = lab[:, None].astype(np.float64) * xs
+    al, xi, s, info = O.mpc(Z, lab, 1.0, kernel_type=0)
+    bo, wo = O.bias_linear(al, lab, xs, 1.0)
+    assert np.allclose(w, wo, atol=1e-4 * (1 + np.abs(wo).max()))
+    assert abs(float(lines[6 + d]) - bo) <= 1e-3 * (1 + abs(bo))

@@ -1,0 +1,171 @@
+"""CPU oracle pinned against golden vectors, library results and closed forms (no GPU)."""
+import os
+
+import numpy as np
+import pytest
+from scipy.spatial.distance import cdist
+
+import oracle_lib as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+G = os.path.join(HERE, "golden")
+
+
+def test_glibc_rand_and_gaussrand_golden():
+    g = np.load(os.path.join(G, "glibc_rand.npz"))
+    n = g["rand"].size
+    assert np.array_equal(O.glibc_rand(n), g["rand"])
+    assert np.array_equal(O.gaussrand(n).view(np.uint32), g["gauss_bits"])
+
+
+def test_gaussrand_distribution():
+    v = O.gaussrand(400_000).astype(np.float64)
+    assert abs(v.mean()) < 0.01 and abs(v.std() - 1) < 0.01
+
+
+def test_kernel_product_closed_form():
+    rng = np.random.default_rng(0)
+    X1, X2 = rng.normal(size=(37, 6)).astype(np.float32), rng.normal(size=(53, 6)).astype(np.float32)
+    y1 = np.sign(rng.normal(size=37)).astype(np.float32)
+    y2 = np.sign(rng.normal(size=53)).astype(np.float32)
+    A = rng.normal(size=(53, 5))
+    K = np.exp(-0.3 * cdist(X1.astype(np.float64), X2.astype(np.float64), "sqeuclidean"))
+    ref = (y1[:, None] * K * y2[None, :]) @ A
+    assert np.allclose(O.kernel_product(X1, y1, X2, y2, A, 0.3), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_householder_q():
+    rng = np.random.default_rng(1)
+    W = rng.normal(size=(200, 12))
+    Q = O.householder_q(W)
+    assert np.allclose(Q.T @ Q, np.eye(12), atol=1e-12)
+    Qn, _ = np.linalg.qr(W)
+    assert np.allclose(Q @ Q.T, Qn @ Qn.T, atol=1e-12)  # same range
+
+
+def test_sym_eig_vs_numpy():
+    rng = np.random.default_rng(2)
+    B = rng.normal(size=(20, 20))
+    A = B + B.T
+    w, V = O.sym_eig(A)
+    wn = np.linalg.eigvalsh(A)
+    assert np.allclose(w, wn, atol=1e-10)
+    assert np.allclose(V @ np.diag(w) @ V.T, A, atol=1e-10)
+
+
+def test_lra_full_rank_is_exact():
+    """k = n (distinct points, K_Y positive definite): U U^T = K_Y."""
+    rng = np.random.default_rng(3)
+    n = 60
+    X = rng.normal(size=(n, 4)).astype(np.float32)
+    y = np.sign(rng.normal(size=n)).astype(np.float32)
+    om = O.gaussrand(n * n).reshape(n, n)
+    U, w, realk = O.lra(X, y, n, 0.4, om)
+    K = y[:, None] * np.exp(-0.4 * cdist(X.astype(np.float64), X.astype(np.float64), "sqeuclidean")) * y[None, :]
+    assert realk == n
+    assert np.allclose(U @ U.T, K, atol=1e-9)
+
+
+def test_lra_low_rank_is_projection():
+    """U U^T = P K_Y P with P the projector on range(K_Y Omega) (k < n)."""
+    rng = np.random.default_rng(4)
+    n, k = 120, 10
+    X = rng.normal(size=(n, 3)).astype(np.float32)
+    y = np.sign(rng.normal(size=n)).astype(np.float32)
+    om = O.gaussrand(n * k).reshape(n, k)
+    U, w, realk = O.lra(X, y, k, 0.5, om)
+    K = y[:, None] * np.exp(-0.5 * cdist(X.astype(np.float64), X.astype(np.float64), "sqeuclidean")) * y[None, :]
+    Qn, _ = np.linalg.qr(K @ om.astype(np.float64))
+    P = Qn @ Qn.T
+    PKP = P @ K @ P
+    ev, V = np.linalg.eigh((PKP + PKP.T) / 2)
+    ref = (V * np.maximum(ev, 0)) @ V.T
+    assert np.allclose(U @ U.T, ref, atol=1e-8)
+    fk, fr = O.fnorm_res(X, y, 0.5, U)
+    assert np.isclose(fk, np.linalg.norm(K)) and np.isclose(fr, np.linalg.norm(K - U @ U.T))
+
+
+@pytest.mark.parametrize("name", ["a", "b"])
+def test_mpc_vs_sklearn_svc(name):
+    f = np.load(os.path.join(G, f"ipm_svc_{name}.npz"))
+    C = float(f["C"])
+    X, Xi, S, info = O.mpc(f["Z"], f["y"], C)
+    assert info.reason == 0
+    assert np.abs(X - f["alpha"]).max() <= 1e-5 * C
+    assert abs(info.y + float(f["b"])) <= 1e-4 * (1 + abs(float(f["b"])))  # y = rho = -b
+    # KKT at exit: a^T alpha = 0, box, complementarity = 2 n mu
+    assert abs(np.dot(f["y"], X)) < 1e-6
+    assert X.min() > 0 and X.max() < C
+
+
+def test_smw_solve_vs_dense():
+    rng = np.random.default_rng(5)
+    n, k = 150, 7
+    Z = rng.normal(size=(n, k))
+    Dv = np.exp(rng.uniform(-3, 3, n))
+    b = rng.normal(size=n)
+    x = O.smw_solve(Z, Dv, b)
+    assert np.allclose((np.diag(Dv) + Z @ Z.T) @ x, b, atol=1e-9)
+
+
+def test_newton_step_solves_kkt_jacobian():
+    """NewtonStep + SMWSolve (1794-1840) against a dense solve of the (3n+1) KKT Jacobian
+    [-Q a I -I; a^T 0 0 0; S 0 X 0; -Xi 0 0 C-X] d = r (the system in the comment above
+    mpc, tensorsvm-train.cu:1497-1501)."""
+    rng = np.random.default_rng(6)
+    n, k, C = 30, 4, 2.0
+    a = np.sign(rng.normal(size=n))
+    Z = a[:, None] * rng.normal(size=(n, k))
+    X = rng.uniform(0.1, 1.9, n)
+    S = rng.uniform(0.1, 2, n)
+    Xi = rng.uniform(0.1, 2, n)
+    r = rng.normal(size=3 * n + 1)
+    step = O.newton_direction(Z, a, C, X, S, Xi, r)
+    Q = Z @ Z.T
+    J = np.zeros((3 * n + 1, 3 * n + 1))
+    I = np.eye(n)
+    J[:n, :n] = -Q
+    J[:n, n] = a
+    J[:n, n + 1:2 * n + 1] = I
+    J[:n, 2 * n + 1:] = -I
+    J[n, :n] = a
+    J[n + 1:2 * n + 1, :n] = np.diag(S)
+    J[n + 1:2 * n + 1, n + 1:2 * n + 1] = np.diag(X)
+    J[2 * n + 1:, :n] = -np.diag(Xi)
+    J[2 * n + 1:, 2 * n + 1:] = np.diag(C - X)
+    ref = np.linalg.solve(J, r)
+    assert np.allclose(step, ref, rtol=1e-8, atol=1e-8)
+
+
+def test_write_model_golden(tmp_path):
+    m = np.load(os.path.join(G, "model_inputs.npz"))
+    p = str(tmp_path / "m.txt")
+    O.write_model_rbf(p, m["alpha"], m["labels"], m["inst"], float(m["C"]), float(m["gamma"]),
+                      float(m["b"]), int(m["pos"]), int(m["neg"]))
+    assert open(p).read() == open(os.path.join(G, "model_rbf.txt")).read()
+    p2 = str(tmp_path / "l.txt")
+    O.write_model_linear(p2, m["w"], float(m["b"]), int(m["pos"]), int(m["neg"]))
+    assert open(p2).read() == open(os.path.join(G, "model_linear.txt")).read()
+
+
+def test_reader_and_labels_golden():
+    g = np.load(os.path.join(G, "tiny_libsvm.npz"))
+    lab, x = O.libsvm_read(os.path.join(G, "tiny.libsvm"))
+    assert np.array_equal(x, g["x"]) and np.array_equal(lab, g["labels"])
+    mapped, pos, neg, bad = O.label_map(np.array([3, 3, 7, 3, 7, 9], dtype=np.float32))
+    assert (pos, neg, bad) == (3, 7, 1)
+    assert list(mapped[:5]) == [1, 1, -1, 1, -1]
+
+
+def test_bias_rbf_matches_kkt():
+    """At the IPM optimum y (the equality multiplier) equals -b at free SVs."""
+    f = np.load(os.path.join(G, "ipm_svc_b.npz"))
+    C = float(f["C"])
+    X, Xi, S, info = O.mpc(f["Z"], f["y"], C)
+    U = f["y"][:, None] * f["Z"]  # unlabelled factor: U U^T = K
+    b, cnt, sd = O.bias_rbf(X, f["y"].astype(np.float32), f["Z"], C)
+    assert cnt > 0
+    # the free-SV average scatters by its reported std around the multiplier
+    assert abs(b - float(f["b"])) <= 3 * sd + 1e-4
+    assert abs(b + info.y) <= 3 * sd + 1e-4
+    assert abs(info.y + float(f["b"])) < 1e-4
