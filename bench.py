@@ -91,7 +91,7 @@ def main():
         dOm.copy_(dOm0)  # the factor construction consumes Omega
         torch.cuda.synchronize()
         prm = T.capi.train_params(k=k, C=C, gamma=gamma, kernel_type=2, precision=precision,
-                                  stop=T.capi.STOP_GAP, tol=1e-5, maxit=100, refine=1)
+                                  stop=T.capi.STOP_GAP, tol=1e-5, maxit=300, refine=1)
         t0 = time.perf_counter()
         info = ctx.train_dev(dX, dy, ntr, d, prm, dOm, dal)
         torch.cuda.synchronize()

@@ -110,8 +110,11 @@ typedef struct {
   double tol;        /* for TSVM_STOP_GAP */
   int precision;     /* TSVM_PREC_FP32: FP64 capacitance; TSVM_PREC_BF16: BF16 MFMA
                         capacitance + iterative refinement against the FP32 factor */
-  int refine;        /* refinement sweeps per Newton solve (BF16 form), default 1 */
+  int refine;        /* refinement sweeps of each Newton step's bordered system, default 1 */
   int verbose;       /* print the reference's iteration line every 5 iterations */
+  int init;          /* 0: the reference's initial point for kernel_type; 1: the same x with
+                        s, xi shifted so the initial dual residual is zero (what the linear
+                        branch does, 1569-1578) -- far fewer iterations when Q is large */
 } tsvm_ipm_params;
 typedef struct {
   int iters, reason;
@@ -147,8 +150,9 @@ typedef struct {
   int stop_mode;      /* TSVM_STOP_REF (default) or TSVM_STOP_GAP */
   double tol;         /* gap tolerance for TSVM_STOP_GAP */
   int maxit;          /* IPM iterations (100) */
-  int refine;         /* refinement sweeps (BF16 capacitance) */
+  int refine;         /* refinement sweeps of each Newton step */
   int verbose;        /* reference-style progress lines on stdout */
+  int init;           /* IPM initial point, as tsvm_ipm_params.init */
 } tsvm_train_params;
 typedef struct {
   tsvm_lra_info lra;

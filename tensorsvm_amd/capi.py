@@ -77,7 +77,8 @@ class LraInfo(ctypes.Structure):
 
 class IpmParams(ctypes.Structure):
     _fields_ = [("C", _D), ("kernel_type", _INT), ("maxit", _INT), ("stop_mode", _INT),
-                ("tol", _D), ("precision", _INT), ("refine", _INT), ("verbose", _INT)]
+                ("tol", _D), ("precision", _INT), ("refine", _INT), ("verbose", _INT),
+                ("init", _INT)]
 
 
 class IpmInfo(ctypes.Structure):
@@ -90,14 +91,14 @@ class IpmInfo(ctypes.Structure):
 
 
 def ipm_params(C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5, precision=PREC_FP32,
-               refine=1, verbose=0):
-    return IpmParams(C, kernel_type, maxit, stop, tol, precision, refine, verbose)
+               refine=1, verbose=0, init=0):
+    return IpmParams(C, kernel_type, maxit, stop, tol, precision, refine, verbose, init)
 
 
 class TrainParams(ctypes.Structure):
     _fields_ = [("k", _I64), ("C", _D), ("gamma", _D), ("kernel_type", _INT), ("precision", _INT),
                 ("stop_mode", _INT), ("tol", _D), ("maxit", _INT), ("refine", _INT),
-                ("verbose", _INT)]
+                ("verbose", _INT), ("init", _INT)]
 
 
 class TrainInfo(ctypes.Structure):
@@ -112,8 +113,8 @@ class TrainInfo(ctypes.Structure):
 
 
 def train_params(k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32, stop=STOP_REF,
-                 tol=1e-5, maxit=100, refine=1, verbose=0):
-    return TrainParams(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose)
+                 tol=1e-5, maxit=100, refine=1, verbose=0, init=0):
+    return TrainParams(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init)
 
 
 _EXTRA = [
@@ -268,26 +269,26 @@ class Context:
         return info
 
     def ipm(self, Z, a, C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5,
-            precision=PREC_FP32, refine=1, verbose=0):
+            precision=PREC_FP32, refine=1, verbose=0, init=0):
         """Dual IPM on Q = Z Z^T; returns (alpha, xi, s, info)."""
         Z, a = _f32(Z), _f32(a)
         n, k = Z.shape
         al, xi, s = np.empty(n), np.empty(n), np.empty(n)
-        prm = ipm_params(C, kernel_type, maxit, stop, tol, precision, refine, verbose)
+        prm = ipm_params(C, kernel_type, maxit, stop, tol, precision, refine, verbose, init)
         info = IpmInfo()
         self._check(self.lib.tsvm_ipm(self.h, _ptr(Z), _ptr(a), n, k, ctypes.byref(prm), _ptr(al),
                                       _ptr(xi), _ptr(s), ctypes.byref(info)), "tsvm_ipm")
         return al, xi, s, info
 
     def train(self, X, y, k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32,
-              stop=STOP_REF, tol=1e-5, maxit=100, refine=1, verbose=0, omega=None):
+              stop=STOP_REF, tol=1e-5, maxit=100, refine=1, verbose=0, omega=None, init=0):
         """Train on X (n x d) with labels +-1; returns (alpha, w (linear) or None, info)."""
         X, y = _f32(X), _f32(y)
         n, d = X.shape
         om = None if omega is None else _f32(omega).reshape(n, k)
         al = np.empty(n)
         w = np.empty(d)
-        prm = train_params(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose)
+        prm = train_params(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init)
         info = TrainInfo()
         self._check(self.lib.tsvm_train(self.h, _ptr(X), _ptr(y), n, d, ctypes.byref(prm), _ptr(om),
                                         _ptr(al), _ptr(w), ctypes.byref(info)), "tsvm_train")

@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -387,6 +388,52 @@ __global__ void k_rhs3(int64_t n, int corr, const double* __restrict__ a, const 
   GRID_LOOP(i, n) rhs3[i] = a[i] * dy - r5[i];
 }
 
+// Residual of the bordered Newton system (A dx - a dy = -r5, a^T dx = r2) for dx = sol,
+// A dx = D dx + E (E = Z Z^T dx).  Writes r5' = -rho_1 = A dx - a dy + r5 into out and
+// reduces a^T dx (for rho_2 = r2 - a^T dx), |rho_1|^2 and |r5|^2.
+__global__ __launch_bounds__(256) void k_border_res(int64_t n, int corr, const double* __restrict__ a,
+                                                    const double* __restrict__ r5,
+                                                    const double* __restrict__ sol,
+                                                    const double* __restrict__ E,
+                                                    const double* __restrict__ D,
+                                                    const double* __restrict__ sc,
+                                                    double* __restrict__ out, double* __restrict__ red) {
+  const double dy = sc[corr ? SL_DYC : SL_DY];
+  const double r2 = corr ? 0.0 : -sc[SL_AX];
+  double v[3] = {0, 0, 0};
+  const int op[3] = {RED_SUM, RED_SUM, RED_SUM};
+  GRID_LOOP(i, n) {
+    double ax = D[i] * sol[i] + E[i];
+    double rho1 = -r5[i] - (ax - a[i] * dy);
+    out[i] = -rho1;
+    v[0] += a[i] * sol[i];
+    v[1] += rho1 * rho1;
+    v[2] += r5[i] * r5[i];
+  }
+  (void)r2;
+  block_reduce_store<3>(v, op, red);
+}
+
+// correction right-hand side: rhs = a ddy - r5', ddy = (r2 - a^T dx + a^T S r5') / a^T b
+__global__ void k_border_rhs(int64_t n, int corr, const double* __restrict__ a,
+                             const double* __restrict__ r5p, double* __restrict__ sc,
+                             double* __restrict__ rhs) {
+  const double r2 = corr ? 0.0 : -sc[SL_AX];
+  const double rho2 = r2 - sc[SL_R2B];
+  const double ddy = (rho2 + sc[SL_AR7B]) / sc[SL_AB];
+  GRID_LOOP(i, n) rhs[i] = a[i] * ddy - r5p[i];
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[SL_DYB] = ddy;
+    sc[corr ? SL_DYC : SL_DY] += ddy;
+  }
+}
+
+// dx += ddx
+__global__ void k_axpy1(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  GRID_LOOP(i, n) y[i] += x[i];
+}
+
 // Newton step completion (1817-1821) and step bound (1733-1740 / 1762-1768):
 //   ds = (r3 - s dx)/x;  dxi = (r4 + xi dx)/(C-x), dx = (D + Z Z^T)^-1 rhs3
 // corr = 0: predictor, results to (dx, ds, dxi) in place.
@@ -718,7 +765,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   // split (FP64 for high-weight rows, BF16 for the bulk) is selected per iteration, both
   // forms build the capacitance in FP64.
   const bool bf16 = false;
-  const int refine = bf16 ? (prm->refine < 0 ? 1 : prm->refine) : 0;
+  // refinement sweeps of every SMW solve against the exact operator D + Z Z^T (FP64
+  // accumulation of the FP32 factor); SMW alone loses accuracy like eps lambda_max(M)
+  // cond(M), which is large at n in the millions
+  const int refine = 0;
   void* p;
   auto ws = [&](const char* name, int64_t bytes, void** out) -> tsvm_status {
     TSVM_HIP(c, c->ws.get(name, (size_t)bytes, out));
@@ -813,6 +863,30 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     return TSVM_OK;
   };
 
+  // Refinement of the bordered Newton system for the step in sol3 (r5 in rhs row 0):
+  //   rho = [-r5 - (A dx - a dy); r2 - a^T dx];  (ddx, ddy) from the same Schur
+  //   elimination; dx += ddx, dy += ddy.
+  const int border = prm->refine < 0 ? 1 : (prm->refine > 0 ? prm->refine : 0);
+  auto border_refine = [&](int corr) -> tsvm_status {
+    for (int sw = 0; sw < border; sw++) {
+      TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, sol3, n, 1, T));
+      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, 1, E, n));
+      hipLaunchKernelGGL(k_border_res, eg, eb, 0, st, n, corr, (const double*)a, (const double*)rhs,
+                         (const double*)sol3, (const double*)E, (const double*)D, (const double*)sc,
+                         V + n, red);
+      TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_R2B, SL_BRES, SL_BRHS}));
+      // r7' = S r5'
+      TSVM_CHECK(smw(V + n, 1, solc + 0));
+      hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 1, (const double*)a, (const double*)solc, n, red);
+      TSVM_CHECK(reduce({RED_SUM}, {SL_AR7B}));
+      hipLaunchKernelGGL(k_border_rhs, eg, eb, 0, st, n, corr, (const double*)a, (const double*)(V + n),
+                         sc, E + n);
+      TSVM_CHECK(smw(E + n, 1, solc));
+      hipLaunchKernelGGL(k_axpy1, eg, eb, 0, st, n, (const double*)solc, sol3);
+    }
+    return TSVM_OK;
+  };
+
   // ---- initial point (1535-1579)
   const bool linear = prm->kernel_type == 0;
   if (!linear) {
@@ -823,7 +897,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_CHECK(reduce({RED_SUM, RED_SUM}, {SL_QQ, SL_QE}));
   }
   hipLaunchKernelGGL(k_init_xs, eg, eb, 0, st, n, C, linear ? 1 : 0, (const double*)sc, x, s, xi);
-  if (linear) {
+  if (linear || prm->init == 1) {
     TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, x, n, 1, T));
     TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, 1, e, n));
     hipLaunchKernelGGL(k_init_linear_adjust, eg, eb, 0, st, n, (const double*)e, (const double*)a, s, xi);
@@ -882,10 +956,11 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     inf.gap_rel = gap;
     inf.condD = cond;
     inf.y = hsc[SL_Y];
-    if (bf16 && hsc[SL_RHS0] > 0) inf.refine_res = sqrt(hsc[SL_RES0] / hsc[SL_RHS0]);
-    if (prm->verbose && iter % 5 == 0)
-      printf("iter %d, mu=%.3e, normdx=%.3e, normdy=%.3e max/min(D)=%.3e pobj=%.9e dobj=%.9e\n", iter,
-             mu, normdx, normdy, cond, pobj, dobj);
+    if (hsc[SL_BRHS] > 0) inf.refine_res = sqrt(hsc[SL_BRES] / hsc[SL_BRHS]);
+    if (prm->verbose && (iter % 5 == 0 || prm->verbose > 1))
+      printf("iter %d, mu=%.3e, normdx=%.3e, normdy=%.3e max/min(D)=%.3e pobj=%.9e dobj=%.9e%s\n", iter,
+             mu, normdx, normdy, cond, pobj, dobj,
+             prm->verbose > 1 ? (std::string(" newton-res=") + std::to_string(inf.refine_res) + " ay-res=" + std::to_string(hsc[SL_R2B])).c_str() : "");
     bool conv;
     if (prm->stop_mode == TSVM_STOP_REF) {
       conv = mu < 1e-7 && normdx < 1e-7 && normdy < 1e-7;
@@ -930,6 +1005,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_CHECK(reduce({RED_SUM, RED_SUM}, {SL_AR7, SL_AB}));
     hipLaunchKernelGGL(k_rhs3, eg, eb, 0, st, n, 0, (const double*)a, (const double*)rhs, sc, rhs3);
     TSVM_CHECK(smw(rhs3, 1, sol3));
+    TSVM_CHECK(border_refine(0));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 0, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
                        (const double*)ads, (const double*)adxi, red);
@@ -946,6 +1022,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_CHECK(reduce({RED_SUM}, {SL_AR7C}));
     hipLaunchKernelGGL(k_rhs3, eg, eb, 0, st, n, 1, (const double*)a, (const double*)rhs, sc, rhs3);
     TSVM_CHECK(smw(rhs3, 1, sol3));
+    TSVM_CHECK(border_refine(1));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 1, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
                        (const double*)ads, (const double*)adxi, red);
@@ -982,6 +1059,7 @@ void tsvm_ipm_default_params(tsvm_ipm_params* p) {
   p->precision = TSVM_PREC_FP32;
   p->refine = 1;
   p->verbose = 0;
+  p->init = 0;
 }
 
 tsvm_status tsvm_ipm_dev(tsvm_ctx* ctx, const float* Z, const float* a, int64_t n, int64_t k,

@@ -31,6 +31,11 @@ enum IpmSlot {
   SL_RHS0,       // |rhs|^2 of RHS 0
   SL_QQ,         // init: |Z Z^T 1|^2
   SL_QE,         // init: 1^T Z Z^T 1
+  SL_R2B,        // bordered refinement: a^T dx of the step being refined
+  SL_AR7B,       // a^T S(r5')
+  SL_DYB,        // bordered refinement correction of dy
+  SL_BRES,       // |rho_1|^2 of the bordered residual
+  SL_BRHS,       // |r5|^2
   IPM_NSLOT = 32
 };
 

@@ -31,6 +31,7 @@ void to_ipm_params(const tsvm_train_params* p, tsvm_ipm_params* q) {
   q->precision = p->precision;
   q->refine = p->refine;
   q->verbose = p->verbose;
+  q->init = p->init;
 }
 }  // namespace
 
@@ -123,6 +124,7 @@ void tsvm_train_default_params(tsvm_train_params* p) {
   p->maxit = 100;
   p->refine = 1;
   p->verbose = 0;
+  p->init = 0;
 }
 
 tsvm_status tsvm_train_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t n, int64_t d,
@@ -164,6 +166,7 @@ tsvm_status tsvm_train(tsvm_ctx* ctx, const float* X, const float* y, int64_t n,
     ip.precision = prm->precision;
     ip.refine = prm->refine;
     ip.verbose = prm->verbose;
+    ip.init = prm->init;
     void* pxi;
     TSVM_HIP(c, c->ws.get("tr_xi", (size_t)(n * 8), &pxi));
     tsvm_train_info inf;

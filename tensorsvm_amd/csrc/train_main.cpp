@@ -42,6 +42,7 @@ void help_msg() {
       "-tol x : tolerance of -stop gap (default 1e-5)\n"
       "-refine n : refinement sweeps per Newton solve (default 1)\n"
       "-maxit n : IPM iteration limit (default 100)\n"
+      "-init ref|feasible : IPM initial point (feasible: zero initial dual residual)\n"
       "-q : quiet\n"
       "-json file : write a JSON summary of the run\n");
 }
@@ -104,6 +105,7 @@ int main(int argc, char* argv[]) {
     } else if (a == "-tol") prm.tol = atof(next());
     else if (a == "-refine") prm.refine = atoi(next());
     else if (a == "-maxit") prm.maxit = atoi(next());
+    else if (a == "-init") prm.init = std::string(next()) == "feasible" ? 1 : 0;
     else if (a == "-q") prm.verbose = 0;
     else if (a == "-json") json_path = next();
     else if (!train_path) train_path = argv[i];
