@@ -45,9 +45,10 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
   if (k <= 0 || k > n) return fail(c, TSVM_ERR_ARG, "train: need 0 < k <= n");
   tsvm_train_info inf;
   memset(&inf, 0, sizeof(inf));
-  void *pU, *pxi;
+  void *pU, *pxi, *ps;
   TSVM_HIP(c, c->ws.get("tr_U", (size_t)(n * k * 4), &pU));
   TSVM_HIP(c, c->ws.get("tr_xi", (size_t)(n * 8), &pxi));
+  TSVM_HIP(c, c->ws.get("tr_s", (size_t)(n * 8), &ps));
   float* U = (float*)pU;
   if (prm->verbose) printf("\x1b[34mRBF kernel: gamma=%.3e, C=%.3e Approximation Rank K=%ld\x1b[39m\n",
                            prm->gamma, prm->C, (long)k);
@@ -59,12 +60,17 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
   }
   tsvm_ipm_params ip;
   to_ipm_params(prm, &ip);
-  TSVM_CHECK(ipm_dev(c, U, y, n, k, &ip, alpha, (double*)pxi, nullptr, &inf.ipm));
+  TSVM_CHECK(ipm_dev(c, U, y, n, k, &ip, alpha, (double*)pxi, (double*)ps, &inf.ipm));
   if (prm->verbose) printf("\x1b[95mMPC elapsed time = %.3f seconds\x1b[39m\n", inf.ipm.t_total);
-  // bias (writemodel 353-376)
+  // bias (writemodel 353-376).  The samples are the first <= 50 free SVs
+  // (1e-6 < alpha < C - 1e-6); a multiplier counts as free only where the IPM's slacks
+  // agree (s < alpha and xi < C - alpha), which at a converged point is the same set and
+  // at a looser stop (relative gap) keeps not-yet-vanished alphas of non-SVs out.
   const double tb = now();
-  std::vector<double> ha(n);
+  std::vector<double> ha(n), hs(n), hxi(n);
   TSVM_HIP(c, hipMemcpyAsync(ha.data(), alpha, n * 8, hipMemcpyDeviceToHost, c->stream));
+  TSVM_HIP(c, hipMemcpyAsync(hs.data(), ps, n * 8, hipMemcpyDeviceToHost, c->stream));
+  TSVM_HIP(c, hipMemcpyAsync(hxi.data(), pxi, n * 8, hipMemcpyDeviceToHost, c->stream));
   TSVM_HIP(c, hipStreamSynchronize(c->stream));
   std::vector<int64_t> freeidx;
   for (int64_t i = 0; i < n; i++)
@@ -72,7 +78,7 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
       inf.nsv++;
       if (ha[i] < prm->C - 1e-6) {
         inf.nfree++;
-        if (freeidx.size() < 50) freeidx.push_back(i);
+        if (freeidx.size() < 50 && hs[i] < ha[i] && hxi[i] < prm->C - ha[i]) freeidx.push_back(i);
       }
     }
   std::vector<double> v(k);
