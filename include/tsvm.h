@@ -29,7 +29,15 @@ typedef int tsvm_status;
 
 /* Arithmetic of the kernel-product and Z^T D^-1 Z GEMMs. */
 #define TSVM_PREC_FP32 0       /* plain form: FP32 MFMA products, FP64 capacitance */
-#define TSVM_PREC_BF16 1       /* BF16 MFMA (FP32 accumulate) + FP64 Cholesky + refinement */
+#define TSVM_PREC_BF16 1       /* low-precision MFMA form: kernel products with the distance
+                                  in split bf16 and E.A on the fp16 MFMA (the factor's sketch
+                                  in TSVM_PREC_FP16X2, its second product in TSVM_PREC_FP16),
+                                  FP32 accumulate; FP64 Cholesky + refinement in the IPM */
+/* Kernel-product-only forms (tsvm_kernel_product[_dev]); the distance GEMM is the BF16
+ * form's, the product E.A runs on the fp16 MFMA with A under one power-of-two scale: */
+#define TSVM_PREC_FP16X2 2     /* E as two fp16 terms (~2^-22), A fp16 (2^-11) */
+#define TSVM_PREC_FP16 3       /* E and A fp16 (2^-11 each), one MFMA per tile */
+#define TSVM_PREC_FP16R2 4     /* E fp16 (2^-11), A as two fp16 terms (~2^-22) */
 
 typedef struct tsvm_ctx tsvm_ctx;
 

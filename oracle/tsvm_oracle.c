@@ -647,7 +647,8 @@ int orc_mpc(const double *Z, const double *a, int64_t n, int64_t k, double C,
 /* Bias and model files (writemodel 273-417).                                  */
 /* ------------------------------------------------------------------------- */
 double orc_bias_rbf(const double *alpha, const float *labels, const double *U, int64_t n,
-                    int64_t k, double C, int *nfree_used, double *std_out) {
+                    int64_t k, double C, const double *S, const double *Xi, int *nfree_used,
+                    double *std_out) {
   int64_t nSV = 0, nfree = 0;
   for (int64_t i = 0; i < n; i++)
     if (alpha[i] > 1e-6) { nSV++; if (alpha[i] < C - 1e-6) nfree++; }
@@ -655,8 +656,12 @@ double orc_bias_rbf(const double *alpha, const float *labels, const double *U, i
   int64_t *iF = (int64_t *)malloc(sizeof(int64_t) * (nfree + 1));
   int64_t s = 0, f = 0;
   for (int64_t i = 0; i < n; i++)
-    if (alpha[i] > 1e-6) { iSV[s++] = i; if (alpha[i] < C - 1e-6) iF[f++] = i; }
-  int64_t ns = nfree < 50 ? nfree : 50;
+    if (alpha[i] > 1e-6) {
+      iSV[s++] = i;
+      /* with slacks given, a free SV must also have s < alpha and xi < C - alpha */
+      if (alpha[i] < C - 1e-6 && (!S || (S[i] < alpha[i] && Xi[i] < C - alpha[i]))) iF[f++] = i;
+    }
+  int64_t ns = f < 50 ? f : 50;
   double acc = 0.0;
   double *bs = (double *)malloc(sizeof(double) * (ns + 1));
   for (int64_t j = 0; j < ns; j++) { /* 355-372 */

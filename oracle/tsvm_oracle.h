@@ -78,9 +78,13 @@ void orc_smw_solve(const double *Z, const double *D, int64_t n, int64_t k, doubl
 void orc_newton_direction(const double *Z, const double *a, int64_t n, int64_t k, double C,
                           const double *X, const double *S, const double *Xi, double *r);
 /* Bias (writemodel 353-372): mean over the first <= 50 free SVs of
- * y_j - sum_i alpha_i y_j (U_i . U_j).  nfree_used receives the sample count. */
+ * y_j - sum_i alpha_i y_j (U_i . U_j).  S and Xi NULL: the reference's free set
+ * (1e-6 < alpha < C - 1e-6); given the IPM's final slacks, a free SV must also have
+ * S < alpha and Xi < C - alpha (the trainer's rule).  nfree_used receives the sample
+ * count. */
 double orc_bias_rbf(const double *alpha, const float *labels, const double *U,
-                    int64_t n, int64_t k, double C, int *nfree_used, double *std_out);
+                    int64_t n, int64_t k, double C, const double *S, const double *Xi,
+                    int *nfree_used, double *std_out);
 /* Bias of the linear branch (301-331). */
 double orc_bias_linear(const double *alpha, const float *labels, const float *inst,
                        int64_t n, int64_t d, double C, double *w_out);

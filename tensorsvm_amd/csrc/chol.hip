@@ -171,15 +171,24 @@ __global__ __launch_bounds__(256) void chol_solve_kernel(const double* __restric
   for (int64_t b = 0; b < nb; b++) {
     const int64_t j0 = b * CB;
     const int jb = (int)(k - j0 < CB ? k - j0 : CB);
+    // forward: 8 consecutive threads share a row of L (coalesced along t)
+    const int fr = tid >> 3, fg = tid & 7;
     double acc[4] = {0, 0, 0, 0};
-    if (r < jb) {
-      const double* Lr = L + (j0 + r) * k;
-      for (int64_t t = lane8; t < j0; t += 8) {
+    if (fr < jb) {
+      const double* Lr = L + (j0 + fr) * k;
+      int64_t t = fg;
+      for (; t + 24 < j0; t += 32) {
+        double l0 = Lr[t], l1 = Lr[t + 8], l2 = Lr[t + 16], l3 = Lr[t + 24];
+        for (int q = 0; q < p; q++)
+          acc[q] += l0 * X[q * k + t] + l1 * X[q * k + t + 8] + l2 * X[q * k + t + 16] +
+                    l3 * X[q * k + t + 24];
+      }
+      for (; t < j0; t += 8) {
         double l = Lr[t];
         for (int q = 0; q < p; q++) acc[q] += l * X[q * k + t];
       }
     }
-    for (int q = 0; q < 4; q++) part[(lane8 * CB + r) * 4 + q] = acc[q];
+    for (int q = 0; q < 4; q++) part[(fg * CB + fr) * 4 + q] = acc[q];
     __syncthreads();
     if (tid < CB * 4) {
       int rr = tid / 4, q = tid % 4;
@@ -207,7 +216,15 @@ __global__ __launch_bounds__(256) void chol_solve_kernel(const double* __restric
     const int jb = (int)(k - j0 < CB ? k - j0 : CB);
     double acc[4] = {0, 0, 0, 0};
     if (r < jb) {
-      for (int64_t t = j0 + CB + lane8; t < k; t += 8) {
+      int64_t t = j0 + CB + lane8;
+      for (; t + 24 < k; t += 32) {
+        double l0 = L[t * k + j0 + r], l1 = L[(t + 8) * k + j0 + r];
+        double l2 = L[(t + 16) * k + j0 + r], l3 = L[(t + 24) * k + j0 + r];
+        for (int q = 0; q < p; q++)
+          acc[q] += l0 * X[q * k + t] + l1 * X[q * k + t + 8] + l2 * X[q * k + t + 16] +
+                    l3 * X[q * k + t + 24];
+      }
+      for (; t < k; t += 8) {
         double l = L[t * k + j0 + r];
         for (int q = 0; q < p; q++) acc[q] += l * X[q * k + t];
       }

@@ -16,48 +16,95 @@
 namespace tsvm {
 
 // 256 threads; the block owns a 64 x 64 output tile, wave w a 32 x 32 quarter
-// (2 x 2 MFMA tiles of 16 x 16).  Rows of one split are walked 4 at a time (MFMA K = 4).
+// (2 x 2 MFMA tiles of 16 x 16).  Rows are staged 32 at a time through LDS as FP64
+// (the A panel already multiplied by the row weight), double-buffered: the next stage's
+// global loads are in flight while the MFMAs consume the current one.
+constexpr int TN_R = 32;  // rows per stage
+
 __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     const float* __restrict__ A, int64_t lda, int64_t ka, const float* __restrict__ B,
     int64_t ldb, int64_t kb, int64_t n, const double* __restrict__ w, int lower_only,
     int64_t rows_per_split, double* __restrict__ part) {
   const int ta = blockIdx.x, tb = blockIdx.y;
   if (lower_only && tb > ta) return;
+  __shared__ double sA[2][TN_R][64];
+  __shared__ double sB[2][TN_R][64];
   const int64_t split = blockIdx.z;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
-  const int64_t a0 = (int64_t)ta * 64 + (wave >> 1) * 32;
-  const int64_t b0 = (int64_t)tb * 64 + (wave & 1) * 32;
+  const int wa = (wave >> 1) * 32, wb = (wave & 1) * 32;
+  const int64_t a0 = (int64_t)ta * 64, b0 = (int64_t)tb * 64;
   const int64_t r0 = split * rows_per_split;
   const int64_t r1 = r0 + rows_per_split < n ? r0 + rows_per_split : n;
+  const int64_t nst = (r1 - r0 + TN_R - 1) / TN_R;
+  // staging map: thread -> (row, 4 columns) for two passes of 16 rows
+  const int sr = tid >> 4, sc4 = (tid & 15) * 4;
+  const bool avec = (lda % 4 == 0) && (a0 + 64 <= ka);
+  const bool bvec = (ldb % 4 == 0) && (b0 + 64 <= kb);
+  f32x4_t ra[2], rb[2];
+  double rw[2];
+  auto gload = [&](int64_t st) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      int64_t row = r0 + st * TN_R + sr + 16 * h;
+      bool ok = row < r1;
+      int64_t rr = ok ? row : r0;
+      rw[h] = ok ? (w ? w[rr] : 1.0) : 0.0;
+      const float* pa = A + rr * lda + a0 + sc4;
+      const float* pb = B + rr * ldb + b0 + sc4;
+      if (avec) {
+        ra[h] = *(const f32x4_t*)pa;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) ra[h][j] = (a0 + sc4 + j < ka) ? pa[j] : 0.0f;
+      }
+      if (bvec) {
+        rb[h] = *(const f32x4_t*)pb;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) rb[h][j] = (b0 + sc4 + j < kb) ? pb[j] : 0.0f;
+      }
+      if (!ok) rb[h] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        sA[buf][sr + 16 * h][sc4 + j] = rw[h] * (double)ra[h][j];
+        sB[buf][sr + 16 * h][sc4 + j] = (double)rb[h][j];
+      }
+  };
   f64x4_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = (f64x4_t){0, 0, 0, 0};
-  const bool va0 = a0 + l16 < ka, va1 = a0 + 16 + l16 < ka;
-  const bool vb0 = b0 + l16 < kb, vb1 = b0 + 16 + l16 < kb;
-  for (int64_t rr = r0; rr < r1; rr += 16) {
-    double av[4][2], bv[4][2];
+  if (nst > 0) {
+    gload(0);
+    sstore(0);
+    __syncthreads();
+  }
+  for (int64_t st = 0; st < nst; st++) {
+    const int buf = (int)(st & 1);
+    gload(st + 1 < nst ? st + 1 : st);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      int64_t row = rr + 4 * s + kq;
-      bool ok = row < r1;
-      double ws = ok ? (w ? w[row] : 1.0) : 0.0;
-      const float* pa = A + (ok ? row : r0) * lda + a0 + l16;
-      const float* pb = B + (ok ? row : r0) * ldb + b0 + l16;
-      av[s][0] = va0 ? ws * (double)pa[0] : 0.0;
-      av[s][1] = va1 ? ws * (double)pa[16] : 0.0;
-      bv[s][0] = (vb0 && ok) ? (double)pb[0] : 0.0;
-      bv[s][1] = (vb1 && ok) ? (double)pb[16] : 0.0;
+    for (int s4 = 0; s4 < TN_R / 4; s4++) {
+      const int row = 4 * s4 + kq;
+      double av0 = sA[buf][row][wa + l16], av1 = sA[buf][row][wa + 16 + l16];
+      double bv0 = sB[buf][row][wb + l16], bv1 = sB[buf][row][wb + 16 + l16];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv1, acc[1][1], 0, 0, 0);
     }
-#pragma unroll
-    for (int s = 0; s < 4; s++)
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s][i], bv[s][j], acc[i][j], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    sstore(buf ^ 1);
+    __syncthreads();
   }
   // D layout of v_mfma_f64_16x16x4f64: item q of a lane holds row lane/16 + 4q,
   // column lane%16
@@ -68,7 +115,7 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        int64_t a = a0 + 16 * i + kq + 4 * q, b = b0 + 16 * j + l16;
+        int64_t a = a0 + wa + 16 * i + kq + 4 * q, b = b0 + wb + 16 * j + l16;
         if (a < ka && b < kb) P[a * kb + b] = acc[i][j][q];
       }
 }
@@ -98,7 +145,7 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   int64_t max_split = ceil_div(n > 0 ? n : 1, 256);
   if (nsplit > max_split) nsplit = max_split;
   if (nsplit < 1) nsplit = 1;
-  int64_t rps = round_up(ceil_div(n > 0 ? n : 1, nsplit), 16);
+  int64_t rps = round_up(ceil_div(n > 0 ? n : 1, nsplit), TN_R);
   nsplit = ceil_div(n > 0 ? n : 1, rps);
   void* part;
   TSVM_HIP(c, c->ws.get("gemm_part", (size_t)(nsplit * ka * kb * 8), &part));

@@ -72,8 +72,6 @@ tsvm_status chol_solve_f64(Ctx* c, const double* L, int64_t k, double* B, int64_
                            const int* info_dev);
 
 // ---- elementwise helpers (util.hip) -----------------------------------------------
-// x[i] <- float(bf16(x[i])) (round to nearest even)
-tsvm_status round_bf16_inplace(Ctx* c, float* x, int64_t count);
 // v (host, k) = sum_i m_i U[i, :], m_i = alpha_i if alpha_i > thr else 0 (device U, alpha)
 tsvm_status masked_colsum(Ctx* c, const float* U, int64_t n, int64_t k, int64_t ldu,
                           const double* alpha, double thr, double* v_host);

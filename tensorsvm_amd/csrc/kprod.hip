@@ -16,14 +16,22 @@
 // g' = gamma*log2(e), so exp2 gives the kernel directly.  In the BF16 form each FP32
 // value is split into bf16 hi + lo parts and the GEMM carries hi*hi + hi*lo + lo*hi
 // (about 16 mantissa bits for the distance) while the big product E.A' is plain BF16
-// with FP32 accumulation.  The FP32 form uses v_mfma_f32_32x32x2f32 throughout.
+// with FP32 accumulation.  The fp16 forms keep the same distance GEMM and run E.A' on
+// the fp16 MFMA (same rate as bf16, 3 more mantissa bits; E lies in (0, 1] and A' is
+// brought into range by one power-of-two scale): FP16 plain, FP16X2 with E split into
+// two fp16 terms, FP16R2 with A' split into two fp16 terms (two MFMA per tile for the
+// split forms).  The FP32 form uses v_mfma_f32_32x32x2f32 throughout.
 //
 // Layout produced by the prep kernels (padding is zero-filled):
 //   BF16: Pop  [m_pad][KD] bf16, Qop [n_pad][KD] bf16, KD = roundup(3d+4, 16)
 //         AT   [n_pad/32][r_pad][32] bf16 = y2_j A[j,c], j permuted inside each chunk
-//              so one 16-byte LDS read is a whole MFMA B operand.
+//              so one 16-byte LDS read is a whole MFMA B operand (FP16X2: fp16 of
+//              sc * y2_j A[j,c], sc = 2^e with sc * max|A| in [2^13, 2^14))
 //   FP32: Pop  [m_pad][KF] f32, Qop [n_pad][KF] f32, KF = roundup(d+2, 2)
 //         Af   [n_pad][r_pad] f32 = y2_j A[j,c]
+#include <algorithm>
+#include <type_traits>
+
 #include "common.h"
 #include "internal.h"
 
@@ -131,6 +139,73 @@ __global__ void prep_A_bf16(const float* __restrict__ A, int64_t lda, const floa
   }
 }
 
+// max |y_j A[j, c]| over the n x r block, as the bits of a non-negative float
+// (atomicMax on the int view orders non-negative floats correctly)
+__global__ void amax_kernel(const float* __restrict__ A, int64_t lda, int64_t n, int64_t r,
+                            unsigned int* __restrict__ out) {
+  float mx = 0.0f;
+  const int64_t total = n * r;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = idx / r, c = idx - j * r;
+    mx = fmaxf(mx, fabsf(A[j * lda + c]));
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  __shared__ float part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) mx = fmaxf(mx, part[w]);
+    atomicMax(out, __float_as_uint(mx));
+  }
+}
+
+// power-of-two scale sc with sc * amax in [2^13, 2^14) (1 for an all-zero A): slot 0
+// holds sc, slot 1 holds 1/sc (both exact)
+__global__ void fp16_scale_kernel(const unsigned int* __restrict__ amax_bits,
+                                  float* __restrict__ sc) {
+  float a = __uint_as_float(*amax_bits);
+  int e = 0;
+  if (a > 0.0f && a < INFINITY) {
+    int ex;
+    (void)frexpf(a, &ex);  // a = f 2^ex, f in [0.5, 1)
+    e = 14 - ex;
+    e = e > 100 ? 100 : (e < -100 ? -100 : e);
+  }
+  sc[0] = ldexpf(1.0f, e);
+  sc[1] = ldexpf(1.0f, -e);
+}
+
+// fp16 right-hand side: AT[chunk][c][perm32(jj)] = fp16(sc * y_j A[j, c]); with R2 the
+// row carries 64 values, hi at [0, 32) and lo = fp16(sc * y_j A[j, c] - hi) at [32, 64)
+template <bool R2>
+__global__ void prep_A_f16(const float* __restrict__ A, int64_t lda, const float* __restrict__ y,
+                           int64_t n, int64_t r, int64_t r_pad, const float* __restrict__ sc,
+                           _Float16* __restrict__ AT) {
+  constexpr int AW = R2 ? 64 : 32;
+  __shared__ float tile[32][33];
+  const int64_t chunk = blockIdx.x;
+  const int64_t cb = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const float scale = sc[0];
+  for (int jj = ty; jj < 32; jj += 8) {
+    int64_t j = chunk * 32 + jj, c = cb + tx;
+    float v = 0.0f;
+    if (j < n && c < r) v = y[j] * A[j * lda + c];
+    tile[jj][tx] = v * scale;
+  }
+  __syncthreads();
+  for (int cc = ty; cc < 32; cc += 8) {
+    int64_t c = cb + cc;
+    if (c >= r_pad) continue;
+    const float v = tile[tx][cc];
+    const _Float16 hi = (_Float16)v;
+    _Float16* row = AT + (chunk * r_pad + c) * AW;
+    row[perm32(tx)] = hi;
+    if (R2) row[32 + perm32(tx)] = (_Float16)(v - (float)hi);
+  }
+}
+
 __global__ void prep_rows_f32(const float* __restrict__ X, int64_t m, int64_t m_pad, int d, int KF,
                               double gl2, float* __restrict__ P) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -182,29 +257,43 @@ __global__ void prep_A_f32(const float* __restrict__ A, int64_t lda, const float
 }
 
 // -------------------------------------------------------------------------------------
-// BF16 main kernel.  256 threads = 4 waves; wave w owns rows [64w, 64w+64) of the
-// block's 256 rows (two 32-row MFMA blocks) and all BN = 32*NT columns of the block.
-// Per 32-j chunk: S^T via KD/16 MFMA 32x32x16 per row block, exp2 on 16 values per
-// lane per row block, then 2*NT MFMA 32x32x16 per row block for O += E.A'.
-// LDS (double-buffered): Qop chunk 32 x KD, AT chunk BN x 32 (row stride padded).
+// Low-precision main kernel.  256 threads = 4 waves; wave w owns rows [64w, 64w+64) of
+// the block's 256 rows (two 32-row MFMA blocks) and all BN = 32*NT columns of the block.
+// Per 32-j chunk: S^T via KD/16 MFMA 32x32x16 bf16 per row block, exp2 on 16 values per
+// lane per row block, then the E.A' MFMA (2*NT per row block and term) for O += E.A'.
+// LDS (double-buffered): Qop chunk 32 x KD, AT chunk BN x AW (row stride padded).
+//
+// EM selects the E.A' arithmetic:
+//   EM_BF16   E bf16, A' bf16                      1 MFMA per tile, E to 2^-9
+//   EM_F16    E fp16, A' fp16 (scaled)             1 MFMA per tile, E and A' to 2^-11
+//   EM_F16X2  E = e_hi + e_lo fp16, A' fp16        2 MFMA per tile, E to ~2^-22
+//   EM_F16R2  E fp16, A' = a_hi + a_lo fp16        2 MFMA per tile, A' to ~2^-22
 // -------------------------------------------------------------------------------------
-template <int NT, int KD, int CH>
+enum { EM_BF16 = 0, EM_F16 = 1, EM_F16X2 = 2, EM_F16R2 = 3 };
+
+template <int NT, int KD, int CH, int EM>
 __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
                                                   const __bf16* __restrict__ Q,
-                                                  const __bf16* __restrict__ AT,
+                                                  const void* __restrict__ ATv,
                                                   const float* __restrict__ y1,
                                                   float* __restrict__ W, int64_t ldw, int64_t m,
-                                                  int64_t r, int64_t n_stages, int64_t r_pad) {
+                                                  int64_t r, int64_t n_stages, int64_t r_pad,
+                                                  const float* __restrict__ sc) {
+  using ET = typename std::conditional<EM == EM_BF16, __bf16, _Float16>::type;
+  using ET8 = typename std::conditional<EM == EM_BF16, bf16x8_t, f16x8_t>::type;
+  const ET* __restrict__ AT = (const ET*)ATv;
+  constexpr int AW = EM == EM_F16R2 ? 64 : 32;  // AT elements per (chunk, column): hi | lo
   constexpr int BN = NT * 32;
   constexpr int KS = KD / 16;
   constexpr int QS = KD + 8;        // LDS row stride (bf16) of the Qop stage
-  constexpr int AS = 32 * CH + 8;   // LDS row stride (bf16) of the AT stage
+  constexpr int AS = AW * CH + 8;   // LDS row stride (elements) of the AT stage
   constexpr int QV = 32 * CH * KD / 8;   // 16-byte vectors per Qop stage (contiguous)
-  constexpr int AV = BN * 32 * CH / 8;   // 16-byte vectors per AT stage
+  constexpr int AV = BN * AW * CH / 8;   // 16-byte vectors per AT stage
+  constexpr int AVC = BN * AW / 8;       // 16-byte vectors per AT chunk
   constexpr int NQ = (QV + 255) / 256;
   constexpr int NA = (AV + 255) / 256;
   __shared__ __attribute__((aligned(16))) __bf16 sQ[2][32 * CH * QS];
-  __shared__ __attribute__((aligned(16))) __bf16 sA[2][BN * AS];
+  __shared__ __attribute__((aligned(16))) ET sA[2][BN * AS];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -229,7 +318,7 @@ __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
       for (int q = 0; q < 16; q++) acc[rb][ct][q] = 0.0f;
 
   // Staging: the Qop stage is one contiguous run; the AT stage is CH chunks, each a
-  // contiguous [r_pad][32] block of which this column block reads BN rows.  Loads are
+  // contiguous [r_pad][AW] block of which this column block reads BN rows.  Loads are
   // branch-free (out-of-range lanes re-read a valid vector and skip the store) so that
   // they stay in flight across the compute of the current stage.
   u32x4_t sq[NQ], sa[NA];
@@ -244,8 +333,8 @@ __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
     for (int v = 0; v < NA; v++) {
       int idx = tid + v * 256;
       idx = idx < AV ? idx : AV - 1;
-      int ch = idx / (BN * 4), rem = idx - ch * (BN * 4);
-      const u32x4_t* ga = (const u32x4_t*)(AT + ((t * CH + ch) * r_pad + c0) * 32);
+      int ch = idx / AVC, rem = idx - ch * AVC;
+      const u32x4_t* ga = (const u32x4_t*)(AT + ((t * CH + ch) * r_pad + c0) * AW);
       sa[v] = ga[rem];
     }
   };
@@ -262,8 +351,8 @@ __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
     for (int v = 0; v < NA; v++) {
       int idx = tid + v * 256;
       if (AV % 256 == 0 || idx < AV) {
-        int ch = idx / (BN * 4), rem = idx - ch * (BN * 4);
-        int row = rem / 4, col = ch * 32 + (rem % 4) * 8;
+        int ch = idx / AVC, rem = idx - ch * AVC;
+        int row = rem / (AW / 8), col = ch * AW + (rem % (AW / 8)) * 8;
         *(u32x4_t*)&sA[buf][row * AS + col] = sa[v];
       }
     }
@@ -294,23 +383,41 @@ __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
       }
       // E = exp2(S^T): lane holds i = l32 and j = 8(q/4) + 4h + q%4; registers 8k..8k+7
       // form the A operand of K-half k (j in 16k + {4h..4h+3, 8+4h..8+4h+3}).
-      bf16x8_t e[2][2];
+      ET8 e[2][2], el[2][2];
 #pragma unroll
       for (int rb = 0; rb < 2; rb++)
 #pragma unroll
         for (int kap = 0; kap < 2; kap++)
 #pragma unroll
-          for (int tt = 0; tt < 8; tt++)
-            e[rb][kap][tt] = (__bf16)__builtin_amdgcn_exp2f(s[rb][8 * kap + tt]);
-      // O += E . A'_j
+          for (int tt = 0; tt < 8; tt++) {
+            const float v = __builtin_amdgcn_exp2f(s[rb][8 * kap + tt]);
+            const ET hi = (ET)v;
+            e[rb][kap][tt] = hi;
+            if constexpr (EM == EM_F16X2) el[rb][kap][tt] = (ET)(v - (float)hi);
+          }
+      // O += E . A'_j (per term of the split forms)
 #pragma unroll
       for (int ct = 0; ct < NT; ct++) {
 #pragma unroll
         for (int kap = 0; kap < 2; kap++) {
-          bf16x8_t bv =
-              *(const bf16x8_t*)&sA[buf][(ct * 32 + l32) * AS + ch * 32 + (2 * kap + h) * 8];
-          acc[0][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[0][kap], bv, acc[0][ct], 0, 0, 0);
-          acc[1][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[1][kap], bv, acc[1][ct], 0, 0, 0);
+          const int base = (ct * 32 + l32) * AS + ch * AW + (2 * kap + h) * 8;
+          const ET8 bv = *(const ET8*)&sA[buf][base];
+          if constexpr (EM == EM_BF16) {
+            acc[0][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[0][kap], bv, acc[0][ct], 0, 0, 0);
+            acc[1][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[1][kap], bv, acc[1][ct], 0, 0, 0);
+          } else {
+            acc[0][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[0][kap], bv, acc[0][ct], 0, 0, 0);
+            acc[1][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[1][kap], bv, acc[1][ct], 0, 0, 0);
+          }
+          if constexpr (EM == EM_F16X2) {
+            acc[0][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(el[0][kap], bv, acc[0][ct], 0, 0, 0);
+            acc[1][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(el[1][kap], bv, acc[1][ct], 0, 0, 0);
+          }
+          if constexpr (EM == EM_F16R2) {
+            const ET8 bl = *(const ET8*)&sA[buf][base + 32];
+            acc[0][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[0][kap], bl, acc[0][ct], 0, 0, 0);
+            acc[1][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[1][kap], bl, acc[1][ct], 0, 0, 0);
+          }
         }
       }
     }
@@ -320,14 +427,16 @@ __global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
     sstore(buf ^ 1);
     __syncthreads();
   }
-  // epilogue: W[i, c] = y1_i * O[i, c]; acc register q holds row 8(q/4) + 4h + q%4
+  // epilogue: W[i, c] = y1_i * O[i, c]; acc register q holds row 8(q/4) + 4h + q%4.  The
+  // fp16 forms undo the right-hand side's power-of-two scale (exact).
+  const float unscale = EM == EM_BF16 ? 1.0f : sc[1];
 #pragma unroll
   for (int rb = 0; rb < 2; rb++) {
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       int64_t i = rowbase + 32 * rb + 8 * (q / 4) + 4 * h + (q % 4);
       if (i >= m) continue;
-      float yi = y1[i];
+      float yi = y1[i] * unscale;
 #pragma unroll
       for (int ct = 0; ct < NT; ct++) {
         int64_t c = c0 + ct * 32 + l32;
@@ -448,13 +557,30 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
 // host launchers
 // -------------------------------------------------------------------------------------
 template <int NT, int KD>
-static void launch_bf16(const __bf16* P, const __bf16* Q, const __bf16* AT, const float* y1,
-                        float* W, int64_t ldw, int64_t m, int64_t r, int64_t n_chunks,
-                        int64_t r_pad, int64_t m_pad, hipStream_t s) {
+static void launch_bf16(int em, const __bf16* P, const __bf16* Q, const void* AT,
+                        const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
+                        int64_t n_chunks, int64_t r_pad, int64_t m_pad, const float* sc,
+                        hipStream_t s) {
   constexpr int CH = KBF16_CH;
   dim3 grid((unsigned)(m_pad / 256), (unsigned)(r_pad / (NT * 32)));
-  hipLaunchKernelGGL((kprod_bf16<NT, KD, CH>), grid, dim3(256), 0, s, P, Q, AT, y1, W, ldw, m, r,
-                     n_chunks / CH, r_pad);
+  switch (em) {
+    case EM_BF16:
+      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_BF16>), grid, dim3(256), 0, s, P, Q, AT, y1,
+                         W, ldw, m, r, n_chunks / CH, r_pad, sc);
+      break;
+    case EM_F16:
+      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16>), grid, dim3(256), 0, s, P, Q, AT, y1,
+                         W, ldw, m, r, n_chunks / CH, r_pad, sc);
+      break;
+    case EM_F16X2:
+      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16X2>), grid, dim3(256), 0, s, P, Q, AT, y1,
+                         W, ldw, m, r, n_chunks / CH, r_pad, sc);
+      break;
+    default:  // EM_F16R2: one chunk per stage (its AT rows are twice as wide)
+      hipLaunchKernelGGL((kprod_bf16<NT, KD, 1, EM_F16R2>), grid, dim3(256), 0, s, P, Q, AT, y1,
+                         W, ldw, m, r, n_chunks, r_pad, sc);
+      break;
+  }
 }
 template <int NT, int KF>
 static void launch_f32(const float* P, const float* Q, const float* Af, const float* y1,
@@ -475,7 +601,13 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
                 (long)n, (long)d, (long)r);
   hipStream_t s = c->stream;
   const double gl2 = gamma * TSVM_LOG2E;
-  if (precision == TSVM_PREC_BF16) {
+  if (precision == TSVM_PREC_BF16 || precision == TSVM_PREC_FP16 ||
+      precision == TSVM_PREC_FP16X2 || precision == TSVM_PREC_FP16R2) {
+    const int em = precision == TSVM_PREC_BF16     ? EM_BF16
+                   : precision == TSVM_PREC_FP16   ? EM_F16
+                   : precision == TSVM_PREC_FP16X2 ? EM_F16X2
+                                                   : EM_F16R2;
+    const int64_t aw = em == EM_F16R2 ? 64 : 32;
     const int64_t n_pad = round_up(n > 0 ? n : 1, 32 * KBF16_CH), n_chunks = n_pad / 32;
     // operand width: the smallest compiled size holding [hi, hi, lo, norm terms]
     static const int kds[] = {16, 32, 48, 64, 80, 96, 112, 128, 176, 256};
@@ -484,25 +616,51 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
       if (v >= 3 * d + 4) { KD = v; break; }
     if (KD == 0)
       return fail(c, TSVM_ERR_ARG, "kernel_product(bf16): d=%ld exceeds 84 features", (long)d);
-    const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : 4);
+    const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : (r <= 128 ? 4 : 8));
     const int64_t BN = NT * 32, m_pad = round_up(m, 256), r_pad = round_up(r, BN);
     void *P, *Q, *AT;
     TSVM_HIP(c, c->ws.get("kp_P", m_pad * KD * 2, &P));
     TSVM_HIP(c, c->ws.get("kp_Q", n_pad * KD * 2, &Q));
-    TSVM_HIP(c, c->ws.get("kp_A", n_pad * r_pad * 2, &AT));
+    TSVM_HIP(c, c->ws.get("kp_A", n_pad * r_pad * aw * 2 / 32, &AT));
     hipLaunchKernelGGL(prep_rows_bf16, dim3((unsigned)ceil_div(m_pad, 256)), dim3(256), 0, s, X1, m,
                        m_pad, (int)d, KD, gl2, (__bf16*)P);
     hipLaunchKernelGGL(prep_cols_bf16, dim3((unsigned)ceil_div(n_pad, 256)), dim3(256), 0, s, X2, n,
                        n_pad, (int)d, KD, gl2, (__bf16*)Q);
-    hipLaunchKernelGGL(prep_A_bf16, dim3((unsigned)n_chunks, (unsigned)(r_pad / 32)), dim3(256), 0,
-                       s, A, lda, y2, n, r, r_pad, (__bf16*)AT);
+    float* sc = nullptr;
+    if (em != EM_BF16) {
+      // one power-of-two scale brings max |y_j A[j,c]| to [2^13, 2^14) in fp16
+      void* scv;
+      TSVM_HIP(c, c->ws.get("kp_sc", 16, &scv));
+      sc = (float*)scv;
+      unsigned int* amax = (unsigned int*)(sc + 2);
+      TSVM_HIP(c, hipMemsetAsync(amax, 0, 4, s));
+      const int64_t tot = (n > 0 ? n : 1) * r;
+      const unsigned nb = (unsigned)std::min<int64_t>(ceil_div(tot, 256), 4096);
+      hipLaunchKernelGGL(amax_kernel, dim3(nb), dim3(256), 0, s, A, lda, n, r, amax);
+      hipLaunchKernelGGL(fp16_scale_kernel, dim3(1), dim3(1), 0, s, amax, sc);
+      if (em == EM_F16R2)
+        hipLaunchKernelGGL(prep_A_f16<true>, dim3((unsigned)n_chunks, (unsigned)(r_pad / 32)),
+                           dim3(256), 0, s, A, lda, y2, n, r, r_pad, sc, (_Float16*)AT);
+      else
+        hipLaunchKernelGGL(prep_A_f16<false>, dim3((unsigned)n_chunks, (unsigned)(r_pad / 32)),
+                           dim3(256), 0, s, A, lda, y2, n, r, r_pad, sc, (_Float16*)AT);
+    } else {
+      hipLaunchKernelGGL(prep_A_bf16, dim3((unsigned)n_chunks, (unsigned)(r_pad / 32)), dim3(256),
+                         0, s, A, lda, y2, n, r, r_pad, (__bf16*)AT);
+    }
     TSVM_HIP(c, hipGetLastError());
-    const __bf16 *Pb = (const __bf16*)P, *Qb = (const __bf16*)Q, *Ab = (const __bf16*)AT;
-#define KP_BF16_CASE(kd)                                                                   \
-  case kd:                                                                                 \
-    if (NT == 1) launch_bf16<1, kd>(Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s); \
-    else if (NT == 2) launch_bf16<2, kd>(Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s); \
-    else launch_bf16<4, kd>(Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s);       \
+    const __bf16 *Pb = (const __bf16*)P, *Qb = (const __bf16*)Q;
+    const void* Ab = AT;
+#define KP_BF16_CASE(kd)                                                                        \
+  case kd:                                                                                      \
+    if (NT == 1)                                                                                \
+      launch_bf16<1, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, s);   \
+    else if (NT == 2)                                                                           \
+      launch_bf16<2, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, s);   \
+    else if (NT == 4)                                                                           \
+      launch_bf16<4, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, s);   \
+    else                                                                                        \
+      launch_bf16<8, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, s);   \
     break;
     switch (KD) {
       KP_BF16_CASE(16)

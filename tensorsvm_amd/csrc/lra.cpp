@@ -1,20 +1,28 @@
 // Low-rank factor construction (replaces LRA, tensorsvm-train.cu:1941-2249).
 //
 // Everything of size n stays resident in HBM; only k x k matrices cross to the host.
-//   1. W = K_Y Omega                      fused kernel-product kernel (FP32 or BF16 MFMA)
+//   1. W = K_Y Omega                      fused kernel-product kernel (FP32 MFMA; the BF16
+//                                         form runs this sketch in the split FP16X2 product:
+//                                         rounding noise in the sketch adds non-kernel
+//                                         directions to range(W) and costs captured energy
+//                                         in its trailing directions)
 //   2. W <- orth(W)                       shifted Cholesky-QR (Fukaya et al., SIAM J. Sci.
 //                                         Comput. 2020): G = W^T W on FP64 MFMA, host
 //                                         Cholesky of G + s I, W <- W L^{-T}; repeated until
 //                                         G = I to 1e-6 (at most 3 passes)
-//   3. Wp = W rounded to the product precision (BF16 form: bf16 values)
-//      Gp = Wp^T Wp                       exact Gram of the basis actually used
-//   4. Q = K_Y Wp                         second kernel product
-//   5. Cb = Wp^T Q (FP64), symmetrised    (the reference's C, 2122-2156)
+//   3. Gp = W^T W                         Gram of the FP32 basis as stored
+//   4. Q = K_Y W                          second kernel product (BF16 form: the fp16 product;
+//                                         the basis itself stays FP32 -- rounding it would
+//                                         rotate span(W) by ~u and move the projected
+//                                         kernel by ~u sigma_1, while the product's rounding
+//                                         only perturbs C inside span(W), where the
+//                                         trailing block sees ~u sigma_tail)
+//   5. Cb = W^T Q (FP64), symmetrised     (the reference's C, 2122-2156)
 //   6. host: Gp = L L^T, C = L^-1 Cb L^-T = V diag(w) V^T (dsyevd), X = L^-T V sqrt(w+)
-//      -- with Wo = Wp L^-T orthonormal this is exactly the reference's C = Wo^T K_Y Wo,
-//      so U U^T = Wo C+ Wo^T is the same projected kernel even when Wp is only nearly
-//      orthonormal (bf16 rounding)
-//   7. U = Wp X                           FP64 MFMA, FP32 output (the reference's U = W X)
+//      -- with Wo = W L^-T orthonormal this is exactly the reference's C = Wo^T K_Y Wo,
+//      so U U^T = Wo C+ Wo^T is the projected kernel even where W is only nearly
+//      orthonormal
+//   7. U = W X                            FP64 MFMA, FP32 output (the reference's U = W X)
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -56,7 +64,8 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   TSVM_HIP(c, hipEventCreate(&e0));
   TSVM_HIP(c, hipEventCreate(&e1));
   TSVM_HIP(c, hipEventRecord(e0, s));
-  TSVM_CHECK(kernel_product_dev(c, X, y, n, X, y, n, d, Om, k, k, gamma, precision, W, k));
+  const int sketch_prec = precision == TSVM_PREC_BF16 ? TSVM_PREC_FP16X2 : precision;
+  TSVM_CHECK(kernel_product_dev(c, X, y, n, X, y, n, d, Om, k, k, gamma, sketch_prec, W, k));
   TSVM_HIP(c, hipEventRecord(e1, s));
   TSVM_HIP(c, hipStreamSynchronize(s));
   float ms = 0.f;
@@ -104,11 +113,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
     TSVM_HIP(c, hipMemcpyAsync(W, cur, n * k * 4, hipMemcpyDeviceToDevice, s));
     cur = W;
   }
-  // 3. basis in the product precision and its exact Gram
-  if (precision == TSVM_PREC_BF16) {
-    TSVM_CHECK(round_bf16_inplace(c, W, n * k));
-    have_gram = false;
-  }
+  // 3. Gram of the basis as stored
   if (!have_gram) {
     TSVM_CHECK(gemm_tn_f64(c, W, k, k, W, k, k, n, nullptr, true, dG));
     TSVM_HIP(c, hipMemcpyAsync(G.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
@@ -117,10 +122,11 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   double t2 = now();
   inf.t_orth = t2 - t1;
 
-  // 4. Q = K_Y Wp (into the Omega buffer)
+  // 4. Q = K_Y W (into the Omega buffer)
   float* Q = Om;
   TSVM_HIP(c, hipEventRecord(e0, s));
-  TSVM_CHECK(kernel_product_dev(c, X, y, n, X, y, n, d, W, k, k, gamma, precision, Q, k));
+  const int q_prec = precision == TSVM_PREC_BF16 ? TSVM_PREC_FP16 : precision;
+  TSVM_CHECK(kernel_product_dev(c, X, y, n, X, y, n, d, W, k, k, gamma, q_prec, Q, k));
   TSVM_HIP(c, hipEventRecord(e1, s));
   TSVM_HIP(c, hipStreamSynchronize(s));
   TSVM_HIP(c, hipEventElapsedTime(&ms, e0, e1));
@@ -129,7 +135,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   double t3 = now();
   inf.t_product2 = ms * 1e-3;
 
-  // 5. Cb = Wp^T Q
+  // 5. Cb = W^T Q
   std::vector<double> Cb(k * k), Cs(k * k), w(k), V(k * k);
   TSVM_CHECK(gemm_tn_f64(c, W, k, k, Q, k, k, n, nullptr, false, dG));
   TSVM_HIP(c, hipMemcpyAsync(Cb.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
@@ -167,7 +173,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   double t4 = now();
   inf.t_core = t4 - t3;
 
-  // 7. U = Wp X
+  // 7. U = W X
   TSVM_CHECK(gemm_nn_tall(c, W, k, k, dB, k, n, U, k));
   TSVM_HIP(c, hipStreamSynchronize(s));
   double t5 = now();

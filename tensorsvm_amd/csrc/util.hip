@@ -1,27 +1,8 @@
-// Small elementwise device helpers shared by the factor construction and the IPM.
+// Small device helpers of the trainer (bias column sums).
 #include "common.h"
 #include "internal.h"
 
 #include <vector>
-
-namespace tsvm {
-
-__global__ void round_bf16_kernel(float* __restrict__ x, int64_t count) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < count; i += stride) x[i] = (float)(__bf16)x[i];
-}
-
-tsvm_status round_bf16_inplace(Ctx* c, float* x, int64_t count) {
-  if (count <= 0) return TSVM_OK;
-  int64_t blocks = ceil_div(count, 256);
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(round_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, x, count);
-  TSVM_HIP(c, hipGetLastError());
-  return TSVM_OK;
-}
-
-}  // namespace tsvm
 
 namespace tsvm {
 

@@ -44,7 +44,7 @@ def lib():
                               P, P, P, ctypes.POINTER(IpmInfo)]
         L.orc_smw_solve.argtypes = [P, P, I64, I64, P]
         L.orc_newton_direction.argtypes = [P, P, I64, I64, D, P, P, P, P]
-        L.orc_bias_rbf.argtypes = [P, P, P, I64, I64, D, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(D)]
+        L.orc_bias_rbf.argtypes = [P, P, P, I64, I64, D, P, P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(D)]
         L.orc_bias_rbf.restype = D
         L.orc_bias_linear.argtypes = [P, P, P, I64, I64, D, P]
         L.orc_bias_linear.restype = D
@@ -165,11 +165,18 @@ def newton_direction(Z, a, C, X, S, Xi, r):
     return r
 
 
-def bias_rbf(alpha, labels, U, C):
+def bias_rbf(alpha, labels, U, C, S=None, Xi=None):
+    """Reference free set without slacks; with the IPM's final S and Xi, the trainer's
+    slack-confirmed free set."""
     alpha, labels, U = f64(alpha), f32(labels), f64(U)
     n, k = U.shape
     cnt, sd = ctypes.c_int(), D()
-    b = lib().orc_bias_rbf(_p(alpha), _p(labels), _p(U), n, k, C, ctypes.byref(cnt), ctypes.byref(sd))
+    S = None if S is None else f64(S)
+    Xi = None if Xi is None else f64(Xi)
+    ps = None if S is None else _p(S)
+    px = None if Xi is None else _p(Xi)
+    b = lib().orc_bias_rbf(_p(alpha), _p(labels), _p(U), n, k, C, ps, px, ctypes.byref(cnt),
+                           ctypes.byref(sd))
     return b, cnt.value, sd.value
 
 

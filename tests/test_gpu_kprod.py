@@ -61,3 +61,36 @@ def test_kprod_gaussian_mixture_gamma(ctx):
         W = ctx.kernel_product(X, y, X, y, A, g, T.PREC_FP32)
         ref = O.kernel_product(X, y, X, y, A, g)
         assert np.abs(W - ref).max() <= 3e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("m,n,d,r", [(300, 257, 18, 40), (1000, 1000, 18, 32), (513, 700, 54, 100),
+                                     (129, 64, 32, 256), (77, 33, 5, 7), (256, 2048, 18, 512)])
+def test_kprod_fp16x2(ctx, m, n, d, r):
+    """Split form: kernel values as fp16 hi + lo (about 22 bits), the right-hand side
+    rounded to fp16 under one power-of-two scale (relative 2^-11), distance as in BF16."""
+    import tensorsvm_amd as T
+    X1, y1, X2, y2, A = _case(m, n, d, r, 0.05, m + n + 2)
+    W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, T.PREC_FP16X2)
+    ref = O.kernel_product(X1, y1, X2, y2, A, 0.05)
+    Kabs = O.kernel_product(X1, np.ones_like(y1), X2, np.ones_like(y2), np.abs(A), 0.05)
+    err = np.abs(W - ref)
+    bound = (2.0 ** -11 + 2.0 ** -13) * Kabs
+    assert (err <= 1.05 * bound + 1e-6 * np.abs(ref).max()).all()
+    # and well inside the plain BF16 form's error on the same inputs
+    Wb = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, T.PREC_BF16)
+    assert np.median(err) < 0.35 * np.median(np.abs(Wb - ref))
+
+
+def test_kprod_fp16x2_exact_rhs(ctx):
+    """With A = I (exact in fp16) every output is one labelled kernel value, so only the
+    split of E and the distance remain: the per-entry error must sit well below one fp16
+    rounding of E (median 2^-12 relative), which a lost low term would leave."""
+    import tensorsvm_amd as T
+    m, n, d = 700, 96, 18
+    X1, y1, X2, y2, _ = _case(m, n, d, n, 0.05, 99)
+    A = np.eye(n, dtype=np.float32)
+    W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, T.PREC_FP16X2)
+    ref = O.kernel_product(X1, y1, X2, y2, A, 0.05)
+    rel = np.abs(W - ref) / np.abs(ref)
+    assert np.median(rel) < 2.0 ** -14
+    assert rel.max() < 2.0 ** -12

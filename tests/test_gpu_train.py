@@ -22,7 +22,7 @@ def _split(n, d, seed):
 def _oracle_train(X, y, k, gamma, C, om):
     U, w, realk = O.lra(X, y, k, gamma, om)
     al, xi, s, info = O.mpc(U, y.astype(np.float64), C)
-    b, cnt, sd = O.bias_rbf(al, y, U, C)
+    b, cnt, sd = O.bias_rbf(al, y, U, C, s, xi)
     return al, b, info, U
 
 
