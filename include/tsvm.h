@@ -102,6 +102,24 @@ tsvm_status tsvm_lra_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t 
                          int64_t k, double gamma, int precision, float *omega, float *U,
                          tsvm_lra_info *info);
 
+/* Weighted Gram of the interior-point capacitance, M = I + Z^T diag(w) Z (replaces the
+ * D^{-1/2} scaling + cblas_dsyrk / chunked cublasDsyrk of mpc, tensorsvm-train.cu:
+ * 1660-1706).  Z: n x k FP32 row-major, w: n FP64, M: k x k FP64 row-major (all
+ * device pointers; synchronised on return).  TSVM_PREC_FP32: FP64 MFMA over all rows.
+ * TSVM_PREC_BF16 (hybrid): rows with small t_i = w_i |z_i|^2 run on BF16 MFMA (FP32
+ * accumulate) while a bound on sqrt(sum t_i^2) over them stays within budget (<= 0: 4,
+ * the interior-point default), the others on FP64 MFMA.  The BF16 part's error is
+ * about 2^-9 times that bound in norm, against lambda_min(M) >= 1. */
+typedef struct {
+  int64_t n_fp64_rows; /* rows computed in FP64 */
+  double bf16_bound;   /* bound on sqrt(sum t_i^2) over the BF16 rows */
+  double t_bf16;       /* s in the BF16 part */
+  double bf16_tflops;  /* n k (k+1) / t_bf16 */
+} tsvm_syrk_info;
+tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int64_t k,
+                                   const double *w, int precision, double budget, double *M,
+                                   tsvm_syrk_info *info);
+
 /* Interior-point solver parameters (the reference's constants are the defaults). */
 #define TSVM_STOP_REF 0 /* mu, |r_x|, |r_y| < 1e-7 (tensorsvm-train.cu:1647) */
 #define TSVM_STOP_GAP 1 /* (pobj-dobj)/(1+|pobj|) <= tol, |r_x|/(1+sqrt n) <= tol,
@@ -116,8 +134,9 @@ typedef struct {
   int maxit;         /* MAX_MPC_ITER = 100 (62) */
   int stop_mode;     /* TSVM_STOP_REF or TSVM_STOP_GAP */
   double tol;        /* for TSVM_STOP_GAP */
-  int precision;     /* TSVM_PREC_FP32: FP64 capacitance; TSVM_PREC_BF16: BF16 MFMA
-                        capacitance + iterative refinement against the FP32 factor */
+  int precision;     /* TSVM_PREC_FP32: FP64 capacitance; TSVM_PREC_BF16: hybrid capacitance
+                        (tsvm_weighted_syrk_dev, bulk rows on BF16 MFMA) + refinement of
+                        the Newton step against the FP32 factor */
   int refine;        /* refinement sweeps of each Newton step's bordered system, default 1 */
   int verbose;       /* print the reference's iteration line every 5 iterations */
   int init;          /* 0: the reference's initial point for kernel_type; 1: the same x with
@@ -131,7 +150,12 @@ typedef struct {
   double t_total;    /* s */
   double t_syrk;     /* s spent building Z^T D^-1 Z (all iterations) */
   double syrk_tflops;/* algorithmic n k (k+1) flops / t_syrk */
-  double refine_res; /* last relative refinement residual (BF16 form) */
+  double refine_res; /* last relative refinement residual of the bordered Newton system */
+  /* low-precision form: the capacitance's bulk rows on BF16 MFMA (hybrid split) */
+  double t_syrk_bf16;      /* s spent in the BF16 part (all iterations) */
+  double syrk_bf16_tflops; /* n k (k+1) flops per BF16 part / its average time */
+  double bf16_row_frac;    /* rows in the BF16 part, averaged over iterations */
+  int syrk_bf16_iters;     /* iterations with a BF16 part */
 } tsvm_ipm_info;
 void tsvm_ipm_default_params(tsvm_ipm_params *p);
 /* Mehrotra predictor-corrector for the SVM dual with Q = Z Z^T, every O(n) and O(nk)

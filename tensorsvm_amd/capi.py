@@ -40,6 +40,14 @@ def load():
     if _LIB is not None:
         return _LIB
     path = library_path()
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 under the same
+    # SONAME as ROCm's, and whichever is loaded first serves both.  Loading torch's first
+    # lets device tensors and libtsvm share it (the other order leaves torch unable to
+    # initialise its devices).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(path):
         raise TsvmError(f"{path} is missing: build it with `make -C tensorsvm_amd` "
                         "(or __graft_entry__.build())")
@@ -87,7 +95,16 @@ class IpmParams(ctypes.Structure):
 class IpmInfo(ctypes.Structure):
     _fields_ = [("iters", _INT), ("reason", _INT), ("mu", _D), ("normdx", _D), ("normdy", _D),
                 ("pobj", _D), ("dobj", _D), ("gap_rel", _D), ("condD", _D), ("y", _D),
-                ("t_total", _D), ("t_syrk", _D), ("syrk_tflops", _D), ("refine_res", _D)]
+                ("t_total", _D), ("t_syrk", _D), ("syrk_tflops", _D), ("refine_res", _D),
+                ("t_syrk_bf16", _D), ("syrk_bf16_tflops", _D), ("bf16_row_frac", _D),
+                ("syrk_bf16_iters", _INT)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SyrkInfo(ctypes.Structure):
+    _fields_ = [("n_fp64_rows", _I64), ("bf16_bound", _D), ("t_bf16", _D), ("bf16_tflops", _D)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -138,6 +155,7 @@ _EXTRA = [
                       ctypes.POINTER(IpmInfo)], None),
     ("tsvm_lra", [_P, _P, _P, _I64, _I64, _I64, _D, _INT, _P, _P, ctypes.POINTER(LraInfo)], None),
     ("tsvm_lra_dev", [_P, _P, _P, _I64, _I64, _I64, _D, _INT, _P, _P, ctypes.POINTER(LraInfo)], None),
+    ("tsvm_weighted_syrk_dev", [_P, _P, _I64, _I64, _P, _INT, _D, _P, ctypes.POINTER(SyrkInfo)], None),
 ]
 
 
@@ -313,6 +331,14 @@ class Context:
         self._check(self.lib.tsvm_decision_function(self.h, _ptr(X), _ptr(y), _ptr(alpha), n, d, gamma,
                                                     b, _ptr(T), m, _ptr(f)), "tsvm_decision_function")
         return f
+
+    def weighted_syrk_dev(self, Z, n, k, w, precision, M, budget=0.0):
+        """M (device, k x k FP64) = I + Z^T diag(w) Z for device Z (n x k FP32), w (FP64)."""
+        info = SyrkInfo()
+        self._check(self.lib.tsvm_weighted_syrk_dev(self.h, _ptr(Z), n, k, _ptr(w), precision,
+                                                    budget, _ptr(M), ctypes.byref(info)),
+                    "tsvm_weighted_syrk_dev")
+        return info
 
     def kernel_product_dev(self, X1, y1, m, X2, y2, n, d, A, r, gamma, precision, W):
         self._check(self.lib.tsvm_kernel_product_dev(self.h, _ptr(X1), _ptr(y1), m, _ptr(X2),

@@ -106,6 +106,14 @@ tsvm_status tsvm_kernel_product_dev(tsvm_ctx* ctx, const float* X1, const float*
                                   W, r);
 }
 
+tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx* ctx, const float* Z, int64_t n, int64_t k,
+                                   const double* w, int precision, double budget, double* M,
+                                   tsvm_syrk_info* info) {
+  if (!ctx) return TSVM_ERR_ARG;
+  hipSetDevice(ctx->c.device);
+  return tsvm::weighted_syrk_dev(&ctx->c, Z, n, k, w, precision, budget, M, info);
+}
+
 tsvm_status tsvm_kernel_product(tsvm_ctx* ctx, const float* X1, const float* y1, int64_t m,
                                 const float* X2, const float* y2, int64_t n, int64_t d,
                                 const float* A, int64_t r, double gamma, int precision,

@@ -24,7 +24,7 @@ constexpr int TN_R = 32;  // rows per stage
 __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     const float* __restrict__ A, int64_t lda, int64_t ka, const float* __restrict__ B,
     int64_t ldb, int64_t kb, int64_t n, const double* __restrict__ w, int lower_only,
-    int64_t rows_per_split, double* __restrict__ part) {
+    int64_t rows_per_split, const int* __restrict__ rows, double* __restrict__ part) {
   const int ta = blockIdx.x, tb = blockIdx.y;
   if (lower_only && tb > ta) return;
   __shared__ double sA[2][TN_R][64];
@@ -50,6 +50,7 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
       int64_t row = r0 + st * TN_R + sr + 16 * h;
       bool ok = row < r1;
       int64_t rr = ok ? row : r0;
+      if (rows) rr = rows[rr];
       rw[h] = ok ? (w ? w[rr] : 1.0) : 0.0;
       const float* pa = A + rr * lda + a0 + sc4;
       const float* pb = B + rr * ldb + b0 + sc4;
@@ -126,16 +127,18 @@ __global__ void reduce_splits_kernel(const double* __restrict__ part, int64_t ns
   int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= ka * kb) return;
   int64_t a = idx / kb, b = idx - a * kb;
-  if (lower_only && (b / 64) > (a / 64)) return;
+  // lower_only: the lower triangle is summed and mirrored entry by entry, so C is
+  // exactly symmetric (inside a diagonal tile (w z_a) z_b and (w z_b) z_a round apart)
+  if (lower_only && b > a) return;
   double s = 0.0;
   for (int64_t t = 0; t < nsplit; t++) s += part[t * ka * kb + idx];
   C[idx] = s;
-  if (lower_only && (b / 64) < (a / 64)) C[b * kb + a] = s;
+  if (lower_only && b < a) C[b * kb + a] = s;
 }
 
 tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const float* B,
                         int64_t ldb, int64_t kb, int64_t n, const double* w, bool lower_only,
-                        double* C) {
+                        double* C, const int* rows) {
   if (ka <= 0 || kb <= 0) return TSVM_OK;
   if (lower_only && ka != kb) return fail(c, TSVM_ERR_ARG, "gemm_tn_f64: lower_only needs ka==kb");
   const int64_t ta = ceil_div(ka, 64), tb = ceil_div(kb, 64);
@@ -151,7 +154,7 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   TSVM_HIP(c, c->ws.get("gemm_part", (size_t)(nsplit * ka * kb * 8), &part));
   dim3 grid((unsigned)ta, (unsigned)tb, (unsigned)nsplit);
   hipLaunchKernelGGL(gemm_tn_f64_kernel, grid, dim3(256), 0, c->stream, A, lda, ka, B, ldb, kb,
-                     n, w, lower_only ? 1 : 0, rps, (double*)part);
+                     n, w, lower_only ? 1 : 0, rps, rows, (double*)part);
   hipLaunchKernelGGL(reduce_splits_kernel, dim3((unsigned)ceil_div(ka * kb, 256)), dim3(256), 0,
                      c->stream, (const double*)part, nsplit, ka, kb, lower_only ? 1 : 0, C);
   TSVM_HIP(c, hipGetLastError());

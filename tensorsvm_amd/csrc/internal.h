@@ -54,10 +54,38 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
 // ---- dense tall-skinny GEMMs (gemm.hip) -----------------------------------------
 // C (ka x kb, FP64, row-major) = A^T diag(w) B over n rows; A n x ka (lda), B n x kb
 // (ldb) FP32 row-major; w FP64 or nullptr (= ones).  lower_only: only blocks with
-// block-row >= block-col are computed (for SYRK; the caller mirrors).
+// block-row >= block-col are computed (for SYRK; the caller mirrors).  rows (device,
+// optional): the n rows taken are rows[0..n) of A, B and w (a gather) instead of 0..n.
 tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const float* B,
                         int64_t ldb, int64_t kb, int64_t n, const double* w, bool lower_only,
-                        double* C);
+                        double* C, const int* rows = nullptr);
+
+// ---- weighted Gram M = I + Z^T diag(w) Z of the capacitance (syrk.hip) -----------
+// Hybrid form: bulk rows (small w_i |z_i|^2) on BF16 MFMA, the rest on FP64 MFMA.
+struct HybridSyrk {
+  int64_t n = 0, n_pad = 0;
+  int kp = 0, kpt = 0;
+  void* ZbT = nullptr;     // kpt x n_pad bf16, Z^T rounded once
+  double* nz = nullptr;    // |z_i|^2
+  float* wb = nullptr;     // FP32 weights of the bulk rows (0 for FP64 rows), n_pad
+  int* rows = nullptr;     // indices of the FP64 rows
+  int* ints = nullptr;     // split scratch (histograms, counts, offsets, cut, count)
+};
+// Workspace and the once-per-solve operands (ZbT, nz) for Z (n x kp FP32, device).
+tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridSyrk* hs);
+// Splits the rows for weights w (device): slots[0] = sqrt(sum over bulk rows of t_i^2)
+// bound, slots[1] = number of FP64 rows (device doubles); fills rows and wb.
+tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double budget,
+                              double* slots);
+// M = I + FP64 part over the nh gathered rows + BF16 part over the bulk.  ev (optional,
+// two events) brackets the BF16 part.
+tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const double* w,
+                             int64_t nh, double* M, hipEvent_t* ev);
+// flops of one BF16 part, SYRK convention (rows streamed x k (k + 1))
+double hybrid_syrk_flops(const HybridSyrk* hs);
+// tsvm_weighted_syrk_dev (synchronises before returning)
+tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, const double* w,
+                              int precision, double budget, double* M, tsvm_syrk_info* info);
 // Y (n x kb, FP32, ldy) = A (n x ka FP32, lda) * B (ka x kb FP64 row-major).
 tsvm_status gemm_nn_tall(Ctx* c, const float* A, int64_t lda, int64_t ka, const double* B,
                          int64_t kb, int64_t n, float* Y, int64_t ldy);

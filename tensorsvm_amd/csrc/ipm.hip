@@ -30,6 +30,11 @@ namespace tsvm {
 
 constexpr int EW_BLOCKS = 1024;  // blocks of the fused O(n) passes
 constexpr int MAXR = 10;         // reduction slots per pass
+// Error budget of the hybrid capacitance: the BF16 rows satisfy sqrt(sum t_i^2) <= 4
+// (t_i = w_i |z_i|^2), which keeps |M~ - M|_2, and with lambda_min(M) >= 1 the
+// refinement's contraction factor, at a few 1e-2 (three bf16 roundings per term,
+// zero-mean and independent across rows)
+constexpr double IPM_SYRK_BUDGET = 4.0;
 
 // ------------------------------------------------------------------------------------
 // block reduction helper: each block writes its R partials to red[blockIdx.x * R + r]
@@ -579,170 +584,6 @@ __global__ void k_pad_z(int64_t n, int64_t k, int kp, const float* __restrict__ 
 }
 
 // ------------------------------------------------------------------------------------
-// BF16 capacitance: ZbT (kpt x n_pad bf16) is the transposed factor made once per solve;
-// every iteration M = I + sum_n w_n z_n z_n^T is accumulated on MFMA 32x32x16 bf16 with
-// FP32 accumulation.  The weights are applied to the A panel while it is staged into LDS.
-// Block = 4 waves, 128 x 128 output tile (wave: 64 x 64 = 2 x 2 MFMA tiles), lower tiles
-// only; the n range is split over blocks, partials are summed in FP64 in a fixed order.
-// ------------------------------------------------------------------------------------
-__global__ void make_zbt_kernel(const float* __restrict__ Z, int64_t n, int kp, int kpt,
-                                int64_t n_pad, __bf16* __restrict__ ZbT) {
-  __shared__ float tile[32][33];
-  const int64_t i0 = (int64_t)blockIdx.x * 32;  // rows of Z
-  const int c0 = blockIdx.y * 32;                // columns of Z
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int r = ty; r < 32; r += 8) {
-    int64_t i = i0 + r;
-    int c = c0 + tx;
-    tile[r][tx] = (i < n && c < kp) ? Z[i * kp + c] : 0.0f;
-  }
-  __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
-    int c = c0 + r;
-    if (c < kpt) ZbT[(int64_t)c * n_pad + i0 + tx] = (__bf16)tile[tx][r];
-  }
-}
-
-tsvm_status ipm_make_zbt(Ctx* c, const float* Z, int64_t n, int kp, int64_t n_pad, __bf16* ZbT) {
-  const int kpt = (int)round_up(kp, 128);
-  dim3 g((unsigned)(n_pad / 32), (unsigned)(kpt / 32));
-  hipLaunchKernelGGL(make_zbt_kernel, g, dim3(256), 0, c->stream, Z, n, kp, kpt, n_pad, ZbT);
-  TSVM_HIP(c, hipGetLastError());
-  return TSVM_OK;
-}
-
-constexpr int SY_BK = 64;            // n per stage
-constexpr int SY_LS = SY_BK + 8;     // LDS row stride (bf16)
-
-__global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict__ ZbT, int64_t n_pad,
-                                                        int kpt, const double* __restrict__ w,
-                                                        int64_t n, int ntile, int nsplit,
-                                                        int64_t stages_per_split,
-                                                        float* __restrict__ part) {
-  __shared__ __attribute__((aligned(16))) __bf16 sA[128 * SY_LS];
-  __shared__ __attribute__((aligned(16))) __bf16 sB[128 * SY_LS];
-  // block -> (tile, split): the tiles of one split sit 8 blocks apart, i.e. on one XCD
-  const int nlow = ntile * (ntile + 1) / 2;
-  const int bid = blockIdx.x;
-  const int xcd = bid % 8, grp = bid / 8;
-  const int tile = grp % nlow;
-  const int split = (grp / nlow) * 8 + xcd;
-  if (split >= nsplit) return;
-  int ta = 0;
-  while ((ta + 1) * (ta + 2) / 2 <= tile) ta++;
-  const int tb = tile - ta * (ta + 1) / 2;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int l32 = lane & 31, h = lane >> 5;
-  const int wa = (wave >> 1) * 64, wb = (wave & 1) * 64;
-  const int64_t s0 = (int64_t)split * stages_per_split;
-  int64_t s1 = s0 + stages_per_split;
-  const int64_t nst = n_pad / SY_BK;
-  if (s1 > nst) s1 = nst;
-  f32x16_t acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
-  const __bf16* Apan = ZbT + (int64_t)ta * 128 * n_pad;
-  const __bf16* Bpan = ZbT + (int64_t)tb * 128 * n_pad;
-  for (int64_t st = s0; st < s1; st++) {
-    const int64_t n0 = st * SY_BK;
-    // stage: 128 rows x 64 n per panel = 1024 vectors of 8; 4 per thread per panel
-    u32x4_t va[4], vb[4];
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      int idx = tid + v * 256;
-      int row = idx >> 3, col = (idx & 7) * 8;
-      va[v] = *(const u32x4_t*)(Apan + (int64_t)row * n_pad + n0 + col);
-      vb[v] = *(const u32x4_t*)(Bpan + (int64_t)row * n_pad + n0 + col);
-    }
-    __syncthreads();  // previous stage's LDS reads are done
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      int idx = tid + v * 256;
-      int row = idx >> 3, col = (idx & 7) * 8;
-      bf16x8_t a8 = *(bf16x8_t*)&va[v];
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        int64_t nn = n0 + col + j;
-        float wj = nn < n ? (float)w[nn] : 0.0f;
-        a8[j] = (__bf16)((float)a8[j] * wj);
-      }
-      *(bf16x8_t*)&sA[row * SY_LS + col] = a8;
-      *(u32x4_t*)&sB[row * SY_LS + col] = vb[v];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < SY_BK / 16; ks++) {
-      bf16x8_t af[2], bfr[2];
-#pragma unroll
-      for (int i = 0; i < 2; i++) {
-        af[i] = *(const bf16x8_t*)&sA[(wa + 32 * i + l32) * SY_LS + ks * 16 + 8 * h];
-        bfr[i] = *(const bf16x8_t*)&sB[(wb + 32 * i + l32) * SY_LS + ks * 16 + 8 * h];
-      }
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-    }
-  }
-  float* P = part + (int64_t)split * kpt * kpt;
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        int a = ta * 128 + wa + 32 * i + 8 * (q / 4) + 4 * h + (q % 4);
-        int bcol = tb * 128 + wb + 32 * j + l32;
-        P[(int64_t)a * kpt + bcol] = acc[i][j][q];
-      }
-}
-
-// M (kp x kp FP64) = I + sum over splits (lower tiles, mirrored)
-__global__ void syrk_reduce(const float* __restrict__ part, int nsplit, int kpt, int kp,
-                            double* __restrict__ M) {
-  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)kp * kp) return;
-  int a = (int)(idx / kp), bb = (int)(idx % kp);
-  int ra = a, rb = bb;
-  if (rb / 128 > ra / 128) {  // upper tile: read the mirrored lower entry
-    ra = bb;
-    rb = a;
-  }
-  double s = 0.0;
-  for (int t = 0; t < nsplit; t++) s += (double)part[(int64_t)t * kpt * kpt + (int64_t)ra * kpt + rb];
-  if (a == bb) s += 1.0;
-  M[idx] = s;
-}
-
-tsvm_status ipm_syrk_bf16(Ctx* c, const __bf16* ZbT, int64_t n_pad, int kp, const double* w,
-                          int64_t n, double* M, double* tflops_out) {
-  (void)tflops_out;
-  const int kpt = (int)round_up(kp, 128);
-  const int ntile = kpt / 128;
-  const int nlow = ntile * (ntile + 1) / 2;
-  const int64_t nst = n_pad / SY_BK;
-  // about two blocks per CU in total, rounded to whole groups of 8 splits
-  int nsplit = (int)ceil_div((int64_t)c->cu_count * 2, nlow);
-  nsplit = (int)round_up(nsplit, 8);
-  if (nsplit > nst) nsplit = (int)round_up(nst, 8);
-  const int64_t sps = ceil_div(nst, nsplit);
-  void* part;
-  TSVM_HIP(c, c->ws.get("syrk_part", (size_t)nsplit * kpt * kpt * 4, &part));
-  const int nblocks = nlow * nsplit;  // nsplit is a multiple of 8
-  hipLaunchKernelGGL(syrk_bf16_kernel, dim3(nblocks), dim3(256), 0, c->stream, ZbT, n_pad, kpt, w, n,
-                     ntile, nsplit, sps, (float*)part);
-  hipLaunchKernelGGL(syrk_reduce, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)), dim3(256), 0,
-                     c->stream, (const float*)part, nsplit, kpt, kp, M);
-  TSVM_HIP(c, hipGetLastError());
-  return TSVM_OK;
-}
-
-// ------------------------------------------------------------------------------------
 // host driver
 // ------------------------------------------------------------------------------------
 namespace {
@@ -760,11 +601,11 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   hipStream_t st = c->stream;
   const int kp = (int)round_up(k, 32);
   const double C = prm->C;
-  // The BF16 capacitance is only accurate enough for SMW while its absolute error stays
-  // well below 1 (the refinement contraction is about rho(M~ - M)); until the hybrid
-  // split (FP64 for high-weight rows, BF16 for the bulk) is selected per iteration, both
-  // forms build the capacitance in FP64.
-  const bool bf16 = false;
+  // Low-precision form: the capacitance's bulk rows on BF16 MFMA, the high-weight rows
+  // on FP64 (split per iteration so that the BF16 error bound stays within
+  // IPM_SYRK_BUDGET; lambda_min(M) >= 1 makes that bound the refinement's contraction
+  // factor up to a constant).  The plain form builds it in FP64 throughout.
+  const bool bf16 = prm->precision == TSVM_PREC_BF16;
   // refinement sweeps of every SMW solve against the exact operator D + Z Z^T (FP64
   // accumulation of the FP32 factor); SMW alone loses accuracy like eps lambda_max(M)
   // cond(M), which is large at n in the millions
@@ -815,13 +656,8 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     hipLaunchKernelGGL(k_pad_z, dim3(4096), dim3(256), 0, st, n, k, kp, Zin, Zp);
     Z = Zp;
   }
-  __bf16* ZbT = nullptr;
-  const int64_t n_pad = round_up(n, SY_BK);
-  if (bf16) {
-    const int kpt = (int)round_up(kp, 128);
-    WS("ipm_ZbT", (int64_t)kpt * n_pad * 2, ZbT);
-    TSVM_CHECK(ipm_make_zbt(c, Z, n, kp, n_pad, ZbT));
-  }
+  HybridSyrk hs;
+  if (bf16) TSVM_CHECK(hybrid_syrk_setup(c, Z, n, kp, &hs));
 #undef WS
   const dim3 eg(EW_BLOCKS), eb(256);
   hipLaunchKernelGGL(k_labels_to_f64, eg, eb, 0, st, n, af, a);
@@ -907,9 +743,12 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   hipEvent_t ev0, ev1;
   TSVM_HIP(c, hipEventCreate(&ev0));
   TSVM_HIP(c, hipEventCreate(&ev1));
-  double t_syrk = 0.0;
-  int n_syrk = 0;
-  bool pending_syrk = false;
+  hipEvent_t evb[2];
+  TSVM_HIP(c, hipEventCreate(&evb[0]));
+  TSVM_HIP(c, hipEventCreate(&evb[1]));
+  double t_syrk = 0.0, t_bf16 = 0.0, row_frac_sum = 0.0;
+  int n_syrk = 0, n_bf16 = 0;
+  bool pending_syrk = false, pending_bf16 = false;
   double hsc[IPM_NSLOT];
   tsvm_ipm_info inf;
   memset(&inf, 0, sizeof(inf));
@@ -925,6 +764,9 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
                        ds, dxi, D, winv, red);
     TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_MAX, RED_MIN},
                       {SL_MU, SL_NDX2, SL_AX, SL_XQX, SL_SX, SL_SXI, SL_L1, SL_DMAX, SL_DMIN}));
+    // hybrid capacitance: split the rows for this iteration's weights (the FP64 row
+    // count comes back with the scalars)
+    if (bf16) TSVM_CHECK(hybrid_syrk_split(c, &hs, winv, IPM_SYRK_BUDGET, sc + SL_HSB));
     TSVM_HIP(c, hipMemcpyAsync(hsc, sc, sizeof(hsc), hipMemcpyDeviceToHost, st));
     TSVM_HIP(c, hipStreamSynchronize(st));
     if (pending_syrk) {
@@ -932,7 +774,13 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
       TSVM_HIP(c, hipEventElapsedTime(&ms, ev0, ev1));
       t_syrk += ms * 1e-3;
       n_syrk++;
+      if (pending_bf16) {
+        TSVM_HIP(c, hipEventElapsedTime(&ms, evb[0], evb[1]));
+        t_bf16 += ms * 1e-3;
+        n_bf16++;
+      }
       pending_syrk = false;
+      pending_bf16 = false;
     }
     if (hsc[SL_CHOLFAIL] != 0.0) {
       reason = TSVM_CHOL_FAIL;
@@ -989,7 +837,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     // ---- capacitance M = I + Z^T D^-1 Z and its Cholesky (1660-1714)
     TSVM_HIP(c, hipEventRecord(ev0, st));
     if (bf16) {
-      TSVM_CHECK(ipm_syrk_bf16(c, ZbT, n_pad, kp, winv, n, M, nullptr));
+      const int64_t nh = (int64_t)hsc[SL_HSN];
+      TSVM_CHECK(hybrid_syrk_form(c, &hs, Z, winv, nh, M, evb));
+      pending_bf16 = nh < n;
+      row_frac_sum += 1.0 - (double)nh / (double)n;
     } else {
       TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, n, winv, true, M));
       hipLaunchKernelGGL(k_add_identity, dim3((unsigned)ceil_div(kp, 256)), dim3(256), 0, st, M, (int64_t)kp);
@@ -1034,11 +885,17 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   TSVM_HIP(c, hipStreamSynchronize(st));
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
+  (void)hipEventDestroy(evb[0]);
+  (void)hipEventDestroy(evb[1]);
   inf.iters = iter;
   inf.reason = reason;
   inf.t_total = now() - t_start;
   inf.t_syrk = t_syrk;
   if (t_syrk > 0) inf.syrk_tflops = (double)n * k * (k + 1) * n_syrk / t_syrk * 1e-12;
+  inf.t_syrk_bf16 = t_bf16;
+  inf.syrk_bf16_iters = n_bf16;
+  if (t_bf16 > 0) inf.syrk_bf16_tflops = hybrid_syrk_flops(&hs) * n_bf16 / t_bf16 * 1e-12;
+  if (n_syrk > 0) inf.bf16_row_frac = row_frac_sum / n_syrk;
   if (info) *info = inf;
   return TSVM_OK;
 }

@@ -36,6 +36,8 @@ enum IpmSlot {
   SL_DYB,        // bordered refinement correction of dy
   SL_BRES,       // |rho_1|^2 of the bordered residual
   SL_BRHS,       // |r5|^2
+  SL_HSB,        // hybrid capacitance: error bound sqrt(sum t_i^2) of the BF16 rows
+  SL_HSN,        // hybrid capacitance: number of FP64 rows (must follow SL_HSB)
   IPM_NSLOT = 32
 };
 
@@ -47,10 +49,5 @@ tsvm_status ipm_gemv_t(Ctx* c, const float* Z, int64_t n, int kp, const double* 
 // E (p x n, row q at E + q*lde) = Z T, T p x kp.
 tsvm_status ipm_gemv_n(Ctx* c, const float* Z, int64_t n, int kp, const double* T, int p,
                        double* E, int64_t lde);
-// Capacitance from the bf16 transposed factor: M (kp x kp FP64) = I + Zb^T diag(w) Zb.
-tsvm_status ipm_syrk_bf16(Ctx* c, const __bf16* ZbT, int64_t n_pad, int kp, const double* w,
-                          int64_t n, double* M, double* tflops_out);
-// ZbT (kp x n_pad bf16) = transpose of Z (n x kp FP32), zero-padded columns.
-tsvm_status ipm_make_zbt(Ctx* c, const float* Z, int64_t n, int kp, int64_t n_pad, __bf16* ZbT);
 
 }  // namespace tsvm

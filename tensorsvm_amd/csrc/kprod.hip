@@ -271,8 +271,8 @@ __global__ void prep_A_f32(const float* __restrict__ A, int64_t lda, const float
 // -------------------------------------------------------------------------------------
 enum { EM_BF16 = 0, EM_F16 = 1, EM_F16X2 = 2, EM_F16R2 = 3 };
 
-template <int NT, int KD, int CH, int EM>
-__global__ __launch_bounds__(256) void kprod_bf16(const __bf16* __restrict__ P,
+template <int NT, int KD, int CH, int EM, int OCC>
+__global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict__ P,
                                                   const __bf16* __restrict__ Q,
                                                   const void* __restrict__ ATv,
                                                   const float* __restrict__ y1,
@@ -556,29 +556,33 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
 // -------------------------------------------------------------------------------------
 // host launchers
 // -------------------------------------------------------------------------------------
+// Tiling per arithmetic (measured on the 1.6M x 1.6M x 512 sketch shape): the
+// one-term forms run 128-column tiles at two waves per SIMD, the two-term forms
+// 256-column tiles (one distance tile per 256 columns) with one chunk per stage.
 template <int NT, int KD>
 static void launch_bf16(int em, const __bf16* P, const __bf16* Q, const void* AT,
                         const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
                         int64_t n_chunks, int64_t r_pad, int64_t m_pad, const float* sc,
                         hipStream_t s) {
   constexpr int CH = KBF16_CH;
+  constexpr int OCC = NT <= 4 ? 2 : 1;
   dim3 grid((unsigned)(m_pad / 256), (unsigned)(r_pad / (NT * 32)));
   switch (em) {
     case EM_BF16:
-      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_BF16>), grid, dim3(256), 0, s, P, Q, AT, y1,
-                         W, ldw, m, r, n_chunks / CH, r_pad, sc);
+      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_BF16, OCC>), grid, dim3(256), 0, s, P, Q, AT,
+                         y1, W, ldw, m, r, n_chunks / CH, r_pad, sc);
       break;
     case EM_F16:
-      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16>), grid, dim3(256), 0, s, P, Q, AT, y1,
-                         W, ldw, m, r, n_chunks / CH, r_pad, sc);
+      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16, OCC>), grid, dim3(256), 0, s, P, Q, AT,
+                         y1, W, ldw, m, r, n_chunks / CH, r_pad, sc);
       break;
     case EM_F16X2:
-      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16X2>), grid, dim3(256), 0, s, P, Q, AT, y1,
-                         W, ldw, m, r, n_chunks / CH, r_pad, sc);
+      hipLaunchKernelGGL((kprod_bf16<NT, KD, 1, EM_F16X2, 1>), grid, dim3(256), 0, s, P, Q, AT,
+                         y1, W, ldw, m, r, n_chunks, r_pad, sc);
       break;
-    default:  // EM_F16R2: one chunk per stage (its AT rows are twice as wide)
-      hipLaunchKernelGGL((kprod_bf16<NT, KD, 1, EM_F16R2>), grid, dim3(256), 0, s, P, Q, AT, y1,
-                         W, ldw, m, r, n_chunks, r_pad, sc);
+    default:  // EM_F16R2 (its AT rows are twice as wide)
+      hipLaunchKernelGGL((kprod_bf16<NT, KD, 1, EM_F16R2, 1>), grid, dim3(256), 0, s, P, Q, AT,
+                         y1, W, ldw, m, r, n_chunks, r_pad, sc);
       break;
   }
 }
@@ -616,7 +620,8 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
       if (v >= 3 * d + 4) { KD = v; break; }
     if (KD == 0)
       return fail(c, TSVM_ERR_ARG, "kernel_product(bf16): d=%ld exceeds 84 features", (long)d);
-    const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : (r <= 128 ? 4 : 8));
+    const bool two_term = em == EM_F16X2 || em == EM_F16R2;
+    const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : (r <= 128 || !two_term ? 4 : 8));
     const int64_t BN = NT * 32, m_pad = round_up(m, 256), r_pad = round_up(r, BN);
     void *P, *Q, *AT;
     TSVM_HIP(c, c->ws.get("kp_P", m_pad * KD * 2, &P));

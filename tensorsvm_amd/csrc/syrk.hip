@@ -1,0 +1,452 @@
+// Weighted Gram of the interior-point capacitance, M = I + Z^T diag(w) Z
+// (replaces the Zs = D^{-1/2} Z scaling + cblas_dsyrk / chunked cublasDsyrk of
+// tensorsvm-train.cu:1660-1706).
+//
+// Plain form: one FP64-MFMA GEMM over all n rows (gemm_tn_f64).
+//
+// Low-precision (hybrid) form: the rows are split by t_i = w_i |z_i|^2.  The bulk B
+// (small t) goes through a BF16 MFMA SYRK with FP32 accumulation; the rows that carry
+// the large weights (the free support vectors late in the solve) go through the FP64
+// GEMM, gathered by an index list.  The split keeps the BF16 part's error below a
+// budget: each bulk term w_i z_i z_i^T is computed with three bf16 roundings (z twice,
+// w z once), a relative error of about 3 * 2^-9 that is zero-mean and independent
+// across rows, so |E|_2 ~ c 2^-9 sqrt(sum_B t_i^2).  Because M = I + ... has
+// lambda_min(M) >= 1, |M^-1 E| <= |E|, which is the contraction factor of the
+// refinement sweeps that restore the Newton step; sqrt(sum_B t_i^2) <= budget keeps it
+// well below one.  sum_B t_i^2 is bounded from integer counts of t_i per binary
+// exponent (t in [2^e, 2^(e+1)) counts as 4^(e+1)), so the split is deterministic.
+//
+// Data: ZbT (kpt x n_pad bf16) = Z^T rounded once per solve (Z does not change during
+// the interior-point iterations), nz = |z_i|^2 (FP64), per iteration wb (FP32 weights
+// of the bulk rows, 0 for the FP64 rows) and rows (indices of the FP64 rows, in order).
+#include "common.h"
+#include "internal.h"
+
+namespace tsvm {
+
+constexpr int HS_BLOCKS = 1024;  // blocks of the split passes (each a contiguous row range)
+constexpr int HS_BINS = 128;     // bins of t_i by binary exponent: bin = floor(log2 t) + 64
+constexpr int SY_BK = 64;        // rows (the GEMM's K) per SYRK stage
+constexpr int SY_LS = SY_BK + 8;  // LDS row stride (bf16) of a staged panel
+
+__device__ inline int hs_bin(double t) {
+  if (!(t > 0.0)) return 0;
+  int e = ilogb(t) + 64;
+  return e < 0 ? 0 : (e > HS_BINS - 1 ? HS_BINS - 1 : e);
+}
+
+// ZbT[c][i] = bf16(Z[i][c]) for c < kp, 0 for kp <= c < kpt or i >= n (32 x 32 tiles)
+__global__ void make_zbt_kernel(const float* __restrict__ Z, int64_t n, int kp, int kpt,
+                                int64_t n_pad, __bf16* __restrict__ ZbT) {
+  __shared__ float tile[32][33];
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  const int c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    int64_t i = i0 + r;
+    int c = c0 + tx;
+    tile[r][tx] = (i < n && c < kp) ? Z[i * kp + c] : 0.0f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    int c = c0 + r;
+    if (c < kpt) ZbT[(int64_t)c * n_pad + i0 + tx] = (__bf16)tile[tx][r];
+  }
+}
+
+// nz[i] = |z_i|^2 in FP64; one wave per row, lanes stride the columns
+__global__ __launch_bounds__(256) void rownorm2_kernel(const float* __restrict__ Z, int64_t n,
+                                                       int kp, double* __restrict__ nz) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t i = w0; i < n; i += nw) {
+    double s = 0.0;
+    for (int c = lane; c < kp; c += 64) {
+      double v = Z[i * kp + c];
+      s += v * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) nz[i] = s;
+  }
+}
+
+// per-block counts of rows per bin of t_i = w_i nz_i (contiguous row ranges)
+__global__ __launch_bounds__(256) void hs_hist_kernel(const double* __restrict__ w,
+                                                      const double* __restrict__ nz, int64_t n,
+                                                      int64_t rpb, int* __restrict__ part) {
+  __shared__ int h[HS_BINS];
+  for (int b = threadIdx.x; b < HS_BINS; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const int64_t r0 = blockIdx.x * rpb, r1 = r0 + rpb < n ? r0 + rpb : n;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) atomicAdd(&h[hs_bin(w[i] * nz[i])], 1);
+  __syncthreads();
+  for (int b = threadIdx.x; b < HS_BINS; b += blockDim.x) part[blockIdx.x * HS_BINS + b] = h[b];
+}
+
+// one block of HS_BINS threads: total counts per bin, then the cutoff bin: the bulk is
+// every bin below `cut`, the largest prefix whose bound sqrt(sum cnt_b 4^(b-63)) stays
+// within the budget.  out: cut (int), slots[0] = bound of the bulk
+__global__ void hs_cutoff_kernel(const int* __restrict__ part, int nblocks, double budget,
+                                 int* __restrict__ cut_out, double* __restrict__ slots) {
+  __shared__ long long cnt[HS_BINS];
+  const int b = threadIdx.x;
+  long long s = 0;
+  for (int k = 0; k < nblocks; k++) s += part[k * HS_BINS + b];
+  cnt[b] = s;
+  __syncthreads();
+  if (b == 0) {
+    double cum = 0.0, kept = 0.0;
+    int cut = HS_BINS;
+    for (int e = 0; e < HS_BINS; e++) {
+      if (cnt[e] == 0) continue;
+      double add = (double)cnt[e] * ldexp(1.0, 2 * (e - 63));
+      if (sqrt(cum + add) > budget) {
+        cut = e;
+        break;
+      }
+      cum += add;
+    }
+    kept = sqrt(cum);
+    *cut_out = cut;
+    slots[0] = kept;
+  }
+}
+
+// per-block counts of the FP64 rows (bin >= cut)
+__global__ __launch_bounds__(256) void hs_count_kernel(const double* __restrict__ w,
+                                                       const double* __restrict__ nz, int64_t n,
+                                                       int64_t rpb, const int* __restrict__ cut,
+                                                       int* __restrict__ cnt) {
+  __shared__ int tot;
+  if (threadIdx.x == 0) tot = 0;
+  __syncthreads();
+  const int cb = *cut;
+  const int64_t r0 = blockIdx.x * rpb, r1 = r0 + rpb < n ? r0 + rpb : n;
+  int c = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) c += hs_bin(w[i] * nz[i]) >= cb;
+  atomicAdd(&tot, c);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+// one block: exclusive scan of the block counts; slots[1] = number of FP64 rows
+__global__ void hs_scan_kernel(const int* __restrict__ cnt, int nblocks, int* __restrict__ off,
+                               double* __restrict__ slots, int* __restrict__ nh) {
+  if (threadIdx.x != 0) return;
+  int64_t s = 0;
+  for (int b = 0; b < nblocks; b++) {
+    off[b] = (int)s;
+    s += cnt[b];
+  }
+  slots[1] = (double)s;
+  *nh = (int)s;
+}
+
+// rows of the FP64 part in increasing order; wb = FP32 weight of the bulk rows, 0 for
+// the FP64 rows and the padding
+__global__ __launch_bounds__(256) void hs_scatter_kernel(const double* __restrict__ w,
+                                                         const double* __restrict__ nz, int64_t n,
+                                                         int64_t n_pad, int64_t rpb,
+                                                         const int* __restrict__ cut,
+                                                         const int* __restrict__ off,
+                                                         int* __restrict__ rows,
+                                                         float* __restrict__ wb) {
+  __shared__ int wsum[4];
+  const int cb = *cut;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t r0 = blockIdx.x * rpb;
+  int64_t r1 = r0 + rpb < n_pad ? r0 + rpb : n_pad;
+  int base = off[blockIdx.x];
+  for (int64_t c0 = r0; c0 < r1; c0 += 256) {
+    const int64_t i = c0 + threadIdx.x;
+    bool hi = false;
+    float wf = 0.0f;
+    if (i < r1 && i < n) {
+      const double wi = w[i];
+      hi = hs_bin(wi * nz[i]) >= cb;
+      wf = hi ? 0.0f : (float)wi;
+    }
+    if (i < r1) wb[i] = wf;
+    const unsigned long long m = __ballot(hi);
+    const int below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wave] = __popcll(m);
+    __syncthreads();
+    int pre = 0;
+    for (int k = 0; k < wave; k++) pre += wsum[k];
+    const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (hi) rows[base + pre + below] = (int)i;
+    base += tot;
+    __syncthreads();
+  }
+}
+
+// BF16 SYRK: part[split] (kpt x kpt, lower 128 x 128 tiles) = sum over the split's rows
+// of wb_i z_i z_i^T.  Block = 4 waves, 128 x 128 output tile (wave 64 x 64 = 2 x 2 MFMA
+// 32x32x16 tiles); SY_BK rows per stage, double-buffered LDS with the next stage's
+// global loads in flight; the weights scale the A panel while it is staged.  The tiles
+// of one split sit 8 blocks apart (on one XCD) so that its rows are read once into L2.
+__global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict__ ZbT,
+                                                        int64_t n_pad, int kpt,
+                                                        const float* __restrict__ wb, int ntile,
+                                                        int nsplit, int64_t stages_per_split,
+                                                        float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][128 * SY_LS];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][128 * SY_LS];
+  const int nlow = ntile * (ntile + 1) / 2;
+  const int bid = blockIdx.x;
+  const int xcd = bid % 8, grp = bid / 8;
+  const int tile = grp % nlow;
+  const int split = (grp / nlow) * 8 + xcd;
+  if (split >= nsplit) return;
+  int ta = 0;
+  while ((ta + 1) * (ta + 2) / 2 <= tile) ta++;
+  const int tb = tile - ta * (ta + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, h = lane >> 5;
+  const int wa = (wave >> 1) * 64, wbo = (wave & 1) * 64;
+  const int64_t s0 = (int64_t)split * stages_per_split;
+  const int64_t nst_all = n_pad / SY_BK;
+  const int64_t s1 = s0 + stages_per_split < nst_all ? s0 + stages_per_split : nst_all;
+  const int64_t nst = s1 > s0 ? s1 - s0 : 0;
+  f32x16_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
+  const __bf16* Apan = ZbT + (int64_t)ta * 128 * n_pad;
+  const __bf16* Bpan = ZbT + (int64_t)tb * 128 * n_pad;
+  // staging map: thread -> (panel row = idx >> 3, 8 consecutive K at (idx & 7) * 8) for
+  // idx = tid + 256 v; the K offset depends on tid only, so 8 weights cover all four
+  const int scol = (tid & 7) * 8;
+  u32x4_t va[4], vb[4];
+  f32x4_t wv[2];
+  auto gload = [&](int64_t st) {
+    const int64_t n0 = (s0 + st) * SY_BK;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = (tid + v * 256) >> 3;
+      va[v] = *(const u32x4_t*)(Apan + (int64_t)row * n_pad + n0 + scol);
+      vb[v] = *(const u32x4_t*)(Bpan + (int64_t)row * n_pad + n0 + scol);
+    }
+    wv[0] = *(const f32x4_t*)(wb + n0 + scol);
+    wv[1] = *(const f32x4_t*)(wb + n0 + scol + 4);
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = (tid + v * 256) >> 3;
+      bf16x8_t a8 = __builtin_bit_cast(bf16x8_t, va[v]);
+#pragma unroll
+      for (int j = 0; j < 8; j++) a8[j] = (__bf16)((float)a8[j] * wv[j >> 2][j & 3]);
+      *(bf16x8_t*)&sA[buf][row * SY_LS + scol] = a8;
+      *(u32x4_t*)&sB[buf][row * SY_LS + scol] = vb[v];
+    }
+  };
+  if (nst > 0) {
+    gload(0);
+    sstore(0);
+    __syncthreads();
+  }
+  for (int64_t st = 0; st < nst; st++) {
+    const int buf = (int)(st & 1);
+    gload(st + 1 < nst ? st + 1 : st);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ks = 0; ks < SY_BK / 16; ks++) {
+      bf16x8_t af[2], bfr[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        af[i] = *(const bf16x8_t*)&sA[buf][(wa + 32 * i + l32) * SY_LS + ks * 16 + 8 * h];
+        bfr[i] = *(const bf16x8_t*)&sB[buf][(wbo + 32 * i + l32) * SY_LS + ks * 16 + 8 * h];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    sstore(buf ^ 1);
+    __syncthreads();
+  }
+  // C layout of the 32x32 MFMA: register q holds row 8(q/4) + 4h + q%4, column l32
+  float* P = part + (int64_t)split * kpt * kpt;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int a = ta * 128 + wa + 32 * i + 8 * (q / 4) + 4 * h + (q % 4);
+        const int bcol = tb * 128 + wbo + 32 * j + l32;
+        P[(int64_t)a * kpt + bcol] = acc[i][j][q];
+      }
+}
+
+// M (kp x kp FP64) += sum over splits of the BF16 partials, in split order; every upper
+// entry reads its mirrored lower entry (M stays exactly symmetric)
+__global__ void syrk_accum_kernel(const float* __restrict__ part, int nsplit, int kpt, int kp,
+                                  double* __restrict__ M) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)kp * kp) return;
+  const int a = (int)(idx / kp), bb = (int)(idx % kp);
+  int ra = a, rb = bb;
+  if (rb > ra) {
+    ra = bb;
+    rb = a;
+  }
+  double s = 0.0;
+  for (int t = 0; t < nsplit; t++) s += (double)part[(int64_t)t * kpt * kpt + (int64_t)ra * kpt + rb];
+  M[idx] += s;
+}
+
+__global__ void add_identity_kernel(double* __restrict__ M, int64_t k) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < k) M[i * k + i] += 1.0;
+}
+
+// ---- host side ----------------------------------------------------------------------
+
+tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridSyrk* hs) {
+  hs->n = n;
+  hs->kp = kp;
+  hs->kpt = (int)round_up(kp, 128);
+  hs->n_pad = round_up(n > 0 ? n : 1, SY_BK);
+  void* p;
+  TSVM_HIP(c, c->ws.get("hs_ZbT", (size_t)hs->kpt * hs->n_pad * 2, &p));
+  hs->ZbT = p;
+  TSVM_HIP(c, c->ws.get("hs_nz", (size_t)(n > 0 ? n : 1) * 8, &p));
+  hs->nz = (double*)p;
+  TSVM_HIP(c, c->ws.get("hs_wb", (size_t)hs->n_pad * 4, &p));
+  hs->wb = (float*)p;
+  TSVM_HIP(c, c->ws.get("hs_rows", (size_t)(n > 0 ? n : 1) * 4, &p));
+  hs->rows = (int*)p;
+  TSVM_HIP(c, c->ws.get("hs_ints", (size_t)(HS_BLOCKS * (HS_BINS + 2) + 8) * 4, &p));
+  hs->ints = (int*)p;
+  hipStream_t s = c->stream;
+  dim3 g((unsigned)(hs->n_pad / 32), (unsigned)(hs->kpt / 32));
+  hipLaunchKernelGGL(make_zbt_kernel, g, dim3(256), 0, s, Z, n, kp, hs->kpt, hs->n_pad,
+                     (__bf16*)hs->ZbT);
+  hipLaunchKernelGGL(rownorm2_kernel, dim3(4096), dim3(256), 0, s, Z, n, kp, hs->nz);
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double budget,
+                              double* slots) {
+  hipStream_t s = c->stream;
+  const int64_t n = hs->n;
+  int* part = hs->ints;                       // HS_BLOCKS x HS_BINS
+  int* cnt = part + HS_BLOCKS * HS_BINS;      // HS_BLOCKS
+  int* off = cnt + HS_BLOCKS;                 // HS_BLOCKS
+  int* cut = off + HS_BLOCKS;                 // 1
+  int* nh = cut + 1;                          // 1
+  const int64_t rpb = round_up(ceil_div(hs->n_pad, HS_BLOCKS), 256);
+  const int nb = (int)ceil_div(hs->n_pad, rpb);
+  hipLaunchKernelGGL(hs_hist_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb, part);
+  hipLaunchKernelGGL(hs_cutoff_kernel, dim3(1), dim3(HS_BINS), 0, s, (const int*)part, nb, budget,
+                     cut, slots);
+  hipLaunchKernelGGL(hs_count_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb,
+                     (const int*)cut, cnt);
+  hipLaunchKernelGGL(hs_scan_kernel, dim3(1), dim3(64), 0, s, (const int*)cnt, nb, off, slots, nh);
+  hipLaunchKernelGGL(hs_scatter_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, hs->n_pad, rpb,
+                     (const int*)cut, (const int*)off, hs->rows, hs->wb);
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const double* w,
+                             int64_t nh, double* M, hipEvent_t* ev_bf16) {
+  hipStream_t s = c->stream;
+  const int64_t n = hs->n;
+  const int kp = hs->kp, kpt = hs->kpt;
+  // FP64 rows (all rows when every row is FP64: no gather)
+  if (nh >= n)
+    TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, n, w, true, M));
+  else
+    TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, nh, w, true, M, hs->rows));
+  hipLaunchKernelGGL(add_identity_kernel, dim3((unsigned)ceil_div(kp, 256)), dim3(256), 0, s, M,
+                     (int64_t)kp);
+  if (ev_bf16) TSVM_HIP(c, hipEventRecord(ev_bf16[0], s));
+  if (nh < n) {
+    const int ntile = kpt / 128;
+    const int nlow = ntile * (ntile + 1) / 2;
+    const int64_t nst = hs->n_pad / SY_BK;
+    int nsplit = (int)ceil_div((int64_t)c->cu_count * 2, nlow);
+    nsplit = (int)round_up(nsplit, 8);
+    if (nsplit > nst) nsplit = (int)round_up(nst, 8);
+    const int64_t sps = ceil_div(nst, nsplit);
+    void* part;
+    TSVM_HIP(c, c->ws.get("hs_part", (size_t)nsplit * kpt * kpt * 4, &part));
+    hipLaunchKernelGGL(syrk_bf16_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
+                       (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile, nsplit,
+                       sps, (float*)part);
+    hipLaunchKernelGGL(syrk_accum_kernel, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)), dim3(256),
+                       0, s, (const float*)part, nsplit, kpt, kp, M);
+  }
+  if (ev_bf16) TSVM_HIP(c, hipEventRecord(ev_bf16[1], s));
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+double hybrid_syrk_flops(const HybridSyrk* hs) {
+  // rows streamed by the BF16 kernel times the SYRK count k (k + 1)
+  return (double)hs->n_pad * hs->kp * (hs->kp + 1.0);
+}
+
+tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, const double* w,
+                              int precision, double budget, double* M, tsvm_syrk_info* info) {
+  if (n < 0 || k <= 0 || k > 4096 || !Z || !w || !M)
+    return fail(c, TSVM_ERR_ARG, "weighted_syrk: bad arguments (n=%ld k=%ld)", (long)n, (long)k);
+  if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16)
+    return fail(c, TSVM_ERR_ARG, "weighted_syrk: unknown precision %d", precision);
+  hipStream_t s = c->stream;
+  tsvm_syrk_info inf;
+  inf.n_fp64_rows = n;
+  inf.bf16_bound = 0.0;
+  inf.t_bf16 = 0.0;
+  inf.bf16_tflops = 0.0;
+  if (precision == TSVM_PREC_FP32 || n == 0) {
+    TSVM_CHECK(gemm_tn_f64(c, Z, k, k, Z, k, k, n, w, true, M));
+    hipLaunchKernelGGL(add_identity_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, s, M, k);
+    TSVM_HIP(c, hipGetLastError());
+  } else {
+    HybridSyrk hs;
+    TSVM_CHECK(hybrid_syrk_setup(c, Z, n, (int)k, &hs));
+    void* p;
+    TSVM_HIP(c, c->ws.get("hs_slots", 16, &p));
+    double* slots = (double*)p;
+    TSVM_CHECK(hybrid_syrk_split(c, &hs, w, budget > 0 ? budget : 4.0, slots));
+    double hsl[2];
+    TSVM_HIP(c, hipMemcpyAsync(hsl, slots, 16, hipMemcpyDeviceToHost, s));
+    TSVM_HIP(c, hipStreamSynchronize(s));
+    hipEvent_t ev[2];
+    TSVM_HIP(c, hipEventCreate(&ev[0]));
+    TSVM_HIP(c, hipEventCreate(&ev[1]));
+    const int64_t nh = (int64_t)hsl[1];
+    tsvm_status st = hybrid_syrk_form(c, &hs, Z, w, nh, M, ev);
+    if (st == TSVM_OK) {
+      TSVM_HIP(c, hipStreamSynchronize(s));
+      float ms = 0.f;
+      TSVM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+      inf.n_fp64_rows = nh;
+      inf.bf16_bound = hsl[0];
+      if (nh < n) {
+        inf.t_bf16 = ms * 1e-3;
+        if (ms > 0) inf.bf16_tflops = hybrid_syrk_flops(&hs) / (ms * 1e-3) * 1e-12;
+      }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (st != TSVM_OK) return st;
+  }
+  TSVM_HIP(c, hipStreamSynchronize(s));
+  if (info) *info = inf;
+  return TSVM_OK;
+}
+
+}  // namespace tsvm

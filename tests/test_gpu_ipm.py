@@ -55,6 +55,8 @@ def test_ipm_bf16_refined_matches_oracle(ctx, n, k, C):
                               tol=1e-7)
     Xo, Xio, So, io = O.mpc(Z.astype(np.float64), y.astype(np.float64), C, stop="gap", tol=1e-7)
     assert info.reason == 0, info.as_dict()
+    # the hybrid capacitance put the bulk rows on BF16 in the later iterations
+    assert info.syrk_bf16_iters > 0 and info.bf16_row_frac > 0, info.as_dict()
     assert abs(info.pobj - io.pobj) <= 1e-6 * (1 + abs(io.pobj))
     assert np.abs(al - Xo).max() <= 1e-4 * C
 
