@@ -120,6 +120,54 @@ tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int
                                    const double *w, int precision, double budget, double *M,
                                    tsvm_syrk_info *info);
 
+/* Gradients of the trained model with respect to theta = (log gamma, s_1..s_d), the
+ * kernel being exp(-gamma sum_f e^{2 s_f} (x_if - x_jf)^2) (per-feature input log-scales
+ * s), with alpha and b held fixed (envelope theorem at the solver's optimum).  No
+ * counterpart in the reference (its dual objective: tensorsvm-train.cu:1636-1643).
+ *   vals[0] = D = sum alpha - 1/2 alpha^T P K_Y P alpha, P the projector onto
+ *             range(K_Y Omega): the trained dual through the factor construction (the
+ *             same Omega as training; U U^T = P K_Y P when no eigenvalue is clipped --
+ *             TSVM_ERR_NUMERIC otherwise)
+ *   vals[1] = L = mean over held-out rows of the smoothed hinge h(y_t f(x_t)), f the
+ *             exact-kernel decision function with bias b; h(z) = 1/2 - z (z <= 0),
+ *             (1 - z)^2 / 2 (0 < z < 1), 0 (z >= 1)
+ * g_dual, g_loss (d + 1 each, host; component 0 = d/dlog gamma) may be NULL.  The
+ * backward pass recomputes kernel tiles in the fused FP32 product over derivative
+ * right-hand sides; nothing of size n^2 is stored.  Device: X (n x d), y, alpha (FP64),
+ * omega (n x k, not modified), Xt (nt x d), yt; host: log_scales (d), vals (2). */
+typedef struct {
+  double t_forward;       /* s: scaled features, factor and dual value */
+  double t_backward;      /* s in the backward products and reductions (HIP events) */
+  double backward_flops;  /* 2 n^2 (d + 3 (2d+1)) + 2 nt n (d + 2d+1) */
+  double backward_tflops;
+  double t_total;
+  int64_t realk;
+  int orth_passes;
+} tsvm_grad_info;
+tsvm_status tsvm_grad_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
+                          const double *alpha, double b, int64_t k, double log_gamma,
+                          const double *log_scales, const float *omega, const float *Xt,
+                          const float *yt, int64_t nt, double *vals, double *g_dual,
+                          double *g_loss, tsvm_grad_info *info);
+
+/* ---- model files and prediction ------------------------------------------------------
+ * The model files the trainer writes (RBF: LIBSVM text, writemodel 353-406; linear:
+ * LIBLINEAR text, 340-350), read back and scored on the GPU -- what the reference leaves
+ * to an external ThunderSVM/LIBSVM predictor (README.md:27-37).  Decision values:
+ * RBF f = sum_j coef_j exp(-gamma |x - sv_j|^2) - rho; linear f = w . x + w_bias * bias;
+ * the prediction is label[0] if f > 0, else label[1]. */
+typedef struct tsvm_model tsvm_model;
+/* TSVM_ERR_IO when the file cannot be read, TSVM_ERR_FORMAT on a malformed model. */
+tsvm_status tsvm_read_model(const char *path, tsvm_model **model);
+void tsvm_model_free(tsvm_model *model);
+/* Any output pointer may be NULL; rho of a linear model is -w_bias * bias. */
+tsvm_status tsvm_model_get(const tsvm_model *model, int *kernel_type, double *gamma,
+                           double *rho, int *labels, int64_t *nsv, int64_t *d);
+/* T: nt x d host rows (features past either side's width count as 0); f (decision
+ * values) and pred (predicted labels) are host arrays of nt, either may be NULL. */
+tsvm_status tsvm_predict(tsvm_ctx *ctx, const tsvm_model *model, const float *T, int64_t nt,
+                         int64_t d, double *f, float *pred);
+
 /* Interior-point solver parameters (the reference's constants are the defaults). */
 #define TSVM_STOP_REF 0 /* mu, |r_x|, |r_y| < 1e-7 (tensorsvm-train.cu:1647) */
 #define TSVM_STOP_GAP 1 /* (pobj-dobj)/(1+|pobj|) <= tol, |r_x|/(1+sqrt n) <= tol,

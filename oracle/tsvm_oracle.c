@@ -359,6 +359,229 @@ int64_t orc_lra(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
 }
 
 /* ------------------------------------------------------------------------- */
+/* Gradients of the trained dual and of the held-out smoothed hinge loss      */
+/* (no counterpart in the reference; the dual and the factor follow LRA      */
+/* 1941-2249 and mpc 1636-1643).                                              */
+/* ------------------------------------------------------------------------- */
+static double kval(const double *xs, int64_t i, const double *xt, int64_t t, int64_t d,
+                   double gamma) {
+  double s = 0.0;
+  for (int64_t f = 0; f < d; f++) {
+    double df = xs[i * d + f] - xt[t * d + f];
+    s += df * df;
+  }
+  return exp(-gamma * s);
+}
+
+/* out (n x r) = K_Y A over the scaled rows (labels on both sides) */
+static void kprod64(const double *xs, const float *y, int64_t n, int64_t d, double gamma,
+                    const double *A, int64_t r, double *out) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < n; i++) {
+    double *o = out + i * r;
+    for (int64_t c = 0; c < r; c++) o[c] = 0.0;
+    for (int64_t j = 0; j < n; j++) {
+      double kij = (double)y[i] * (double)y[j] * kval(xs, i, xs, j, d, gamma);
+      for (int64_t c = 0; c < r; c++) o[c] += kij * A[j * r + c];
+    }
+  }
+}
+
+/* S_f = sum_ij p_i q_j K(a_i, c_j) (a_if - c_jf)^2 for rows a (m) and c (n) */
+static void bilinear_dk(const double *xa, int64_t m, const double *xc, int64_t n, int64_t d,
+                        double gamma, const double *p, const double *q, double *S) {
+  for (int64_t f = 0; f < d; f++) S[f] = 0.0;
+#pragma omp parallel
+  {
+    double *loc = (double *)calloc((size_t)d, sizeof(double));
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t i = 0; i < m; i++) {
+      if (p[i] == 0.0) continue;
+      for (int64_t j = 0; j < n; j++) {
+        double w = p[i] * q[j] * kval(xa, i, xc, j, d, gamma);
+        for (int64_t f = 0; f < d; f++) {
+          double df = xa[i * d + f] - xc[j * d + f];
+          loc[f] += w * df * df;
+        }
+      }
+    }
+#pragma omp critical
+    for (int64_t f = 0; f < d; f++) S[f] += loc[f];
+    free(loc);
+  }
+}
+
+/* x <- A^{-1} x for a k x k matrix (Gaussian elimination, partial pivoting) */
+static int solve_dense(const double *A, int64_t k, double *x) {
+  double *M = (double *)malloc(sizeof(double) * k * k);
+  memcpy(M, A, sizeof(double) * k * k);
+  for (int64_t c = 0; c < k; c++) {
+    int64_t piv = c;
+    for (int64_t r = c + 1; r < k; r++)
+      if (fabs(M[r * k + c]) > fabs(M[piv * k + c])) piv = r;
+    if (M[piv * k + c] == 0.0) { free(M); return -1; }
+    if (piv != c) {
+      for (int64_t j = 0; j < k; j++) { double t = M[c * k + j]; M[c * k + j] = M[piv * k + j]; M[piv * k + j] = t; }
+      double t = x[c]; x[c] = x[piv]; x[piv] = t;
+    }
+    for (int64_t r = c + 1; r < k; r++) {
+      double l = M[r * k + c] / M[c * k + c];
+      if (l == 0.0) continue;
+      for (int64_t j = c; j < k; j++) M[r * k + j] -= l * M[c * k + j];
+      x[r] -= l * x[c];
+    }
+  }
+  for (int64_t c = k - 1; c >= 0; c--) {
+    double s = x[c];
+    for (int64_t j = c + 1; j < k; j++) s -= M[c * k + j] * x[j];
+    x[c] = s / M[c * k + c];
+  }
+  free(M);
+  return 0;
+}
+
+int orc_grad(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
+             const float *omega, const double *alpha, double b, double log_gamma,
+             const double *log_scales, const float *Xt, const float *yt, int64_t nt,
+             double *vals, double *gD, double *gL) {
+  const double gamma = exp(log_gamma);
+  double *xs = (double *)malloc(sizeof(double) * n * d);
+  double *xt = (double *)malloc(sizeof(double) * (nt > 0 ? nt : 1) * d);
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t f = 0; f < d; f++) xs[i * d + f] = (double)X[i * d + f] * exp(log_scales[f]);
+  for (int64_t t = 0; t < nt; t++)
+    for (int64_t f = 0; f < d; f++) xt[t * d + f] = (double)Xt[t * d + f] * exp(log_scales[f]);
+  double *Om = (double *)malloc(sizeof(double) * n * k);
+  double *Y = (double *)malloc(sizeof(double) * n * k);
+  double *W = (double *)malloc(sizeof(double) * n * k);
+  double *Q = (double *)malloc(sizeof(double) * n * k);
+  for (int64_t i = 0; i < n * k; i++) Om[i] = (double)omega[i];
+  kprod64(xs, y, n, d, gamma, Om, k, Y); /* Y = K_Y Omega */
+  memcpy(W, Y, sizeof(double) * n * k);
+  orc_householder_q(W, n, k);            /* range(W) = range(Y) */
+  kprod64(xs, y, n, d, gamma, W, k, Q);  /* Q = K_Y W */
+  double *C = (double *)calloc((size_t)(k * k), sizeof(double));
+  double *R = (double *)calloc((size_t)(k * k), sizeof(double));
+  double *beta = (double *)calloc((size_t)k, sizeof(double));
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t a = 0; a < k; a++) {
+      double wa = W[i * k + a];
+      beta[a] += wa * alpha[i];
+      for (int64_t c = 0; c < k; c++) {
+        C[a * k + c] += wa * Q[i * k + c];
+        R[a * k + c] += wa * Y[i * k + c];
+      }
+    }
+  for (int64_t a = 0; a < k; a++)
+    for (int64_t c = 0; c < a; c++) {
+      double v = 0.5 * (C[a * k + c] + C[c * k + a]);
+      C[a * k + c] = v;
+      C[c * k + a] = v;
+    }
+  int ret = 0;
+  { /* positive definiteness of C (no eigenvalue clipping in the factor) */
+    double *L = (double *)malloc(sizeof(double) * k * k);
+    memcpy(L, C, sizeof(double) * k * k);
+    for (int64_t j = 0; j < k && ret == 0; j++) {
+      double s = L[j * k + j];
+      for (int64_t t = 0; t < j; t++) s -= L[j * k + t] * L[j * k + t];
+      if (!(s > 0.0)) { ret = -1; break; }
+      L[j * k + j] = sqrt(s);
+      for (int64_t i = j + 1; i < k; i++) {
+        double v = L[i * k + j];
+        for (int64_t t = 0; t < j; t++) v -= L[i * k + t] * L[j * k + t];
+        L[i * k + j] = v / L[j * k + j];
+      }
+    }
+    free(L);
+  }
+  double sa = 0.0, bcb = 0.0;
+  for (int64_t i = 0; i < n; i++) sa += alpha[i];
+  for (int64_t a = 0; a < k; a++)
+    for (int64_t c = 0; c < k; c++) bcb += beta[a] * C[a * k + c] * beta[c];
+  vals[0] = sa - 0.5 * bcb;
+  if (gD && ret == 0) {
+    double *Pa = (double *)malloc(sizeof(double) * n), *g = (double *)malloc(sizeof(double) * n);
+    double *wg = (double *)calloc((size_t)k, sizeof(double));
+    for (int64_t i = 0; i < n; i++) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int64_t a = 0; a < k; a++) {
+        s1 += W[i * k + a] * beta[a];
+        s2 += Q[i * k + a] * beta[a];
+      }
+      Pa[i] = s1; /* P alpha */
+      g[i] = s2;  /* K_Y P alpha */
+    }
+    for (int64_t i = 0; i < n; i++)
+      for (int64_t a = 0; a < k; a++) wg[a] += W[i * k + a] * g[i];
+    double *c1 = (double *)malloc(sizeof(double) * k), *c2 = (double *)malloc(sizeof(double) * k);
+    memcpy(c1, wg, sizeof(double) * k);
+    memcpy(c2, beta, sizeof(double) * k);
+    solve_dense(R, k, c1); /* Y^+ g = R^{-1} W^T g */
+    solve_dense(R, k, c2); /* Y^+ alpha */
+    double *pp = (double *)malloc(sizeof(double) * n * 3), *qq = (double *)malloc(sizeof(double) * n * 3);
+    for (int64_t i = 0; i < n; i++) {
+      double wwg = 0.0, o1 = 0.0, o2 = 0.0;
+      for (int64_t a = 0; a < k; a++) {
+        wwg += W[i * k + a] * wg[a];
+        o1 += Om[i * k + a] * c1[a];
+        o2 += Om[i * k + a] * c2[a];
+      }
+      double yi = y[i];
+      pp[i] = yi * Pa[i];              qq[i] = yi * Pa[i];       /* (P a, P a) */
+      pp[n + i] = yi * (alpha[i] - Pa[i]); qq[n + i] = yi * o1;  /* ((I-P) a, Omega Y^+ g) */
+      pp[2 * n + i] = yi * (g[i] - wwg);   qq[2 * n + i] = yi * o2; /* ((I-P) g, Omega Y^+ a) */
+    }
+    double *S = (double *)malloc(sizeof(double) * d), *T = (double *)calloc((size_t)d, sizeof(double));
+    for (int m = 0; m < 3; m++) {
+      bilinear_dk(xs, n, xs, n, d, gamma, pp + m * n, qq + m * n, S);
+      for (int64_t f = 0; f < d; f++) T[f] += (m == 0 ? 1.0 : 2.0) * S[f];
+    }
+    double tot = 0.0;
+    for (int64_t f = 0; f < d; f++) {
+      gD[1 + f] = gamma * T[f];
+      tot += T[f];
+    }
+    gD[0] = 0.5 * gamma * tot;
+    free(Pa); free(g); free(wg); free(c1); free(c2); free(pp); free(qq); free(S); free(T);
+  }
+  /* held-out smoothed hinge */
+  vals[1] = 0.0;
+  if (nt > 0) {
+    double *q = (double *)malloc(sizeof(double) * n), *p = (double *)malloc(sizeof(double) * nt);
+    for (int64_t i = 0; i < n; i++) q[i] = alpha[i] * (double)y[i];
+    double loss = 0.0;
+    for (int64_t t = 0; t < nt; t++) {
+      double f = b;
+      for (int64_t i = 0; i < n; i++) f += q[i] * kval(xt, t, xs, i, d, gamma);
+      double z = (double)yt[t] * f, h, dh;
+      if (z <= 0.0) { h = 0.5 - z; dh = -1.0; }
+      else if (z < 1.0) { h = 0.5 * (1.0 - z) * (1.0 - z); dh = -(1.0 - z); }
+      else { h = 0.0; dh = 0.0; }
+      loss += h;
+      p[t] = dh * (double)yt[t] / (double)nt;
+    }
+    vals[1] = loss / (double)nt;
+    if (gL) {
+      double *S = (double *)malloc(sizeof(double) * d);
+      bilinear_dk(xt, nt, xs, n, d, gamma, p, q, S);
+      double tot = 0.0;
+      for (int64_t f = 0; f < d; f++) {
+        gL[1 + f] = -2.0 * gamma * S[f];
+        tot += S[f];
+      }
+      gL[0] = -gamma * tot;
+      free(S);
+    }
+    free(q); free(p);
+  } else if (gL) {
+    for (int64_t f = 0; f <= d; f++) gL[f] = 0.0;
+  }
+  free(xs); free(xt); free(Om); free(Y); free(W); free(Q); free(C); free(R); free(beta);
+  return ret;
+}
+
+/* ------------------------------------------------------------------------- */
 /* IPM (mpc 1507-1785, NewtonStep 1794-1822, SMWSolve 1824-1840).              */
 /* ------------------------------------------------------------------------- */
 static void gemv_t(const double *Z, int64_t n, int64_t k, const double *v, double *out) {

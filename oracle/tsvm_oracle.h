@@ -103,6 +103,23 @@ void orc_decision(const double *alpha, const float *labels, const float *inst, i
 void orc_fnorm_res(const float *X, const float *y, int64_t n, int64_t d, double gamma,
                    const double *U, int64_t k, double *fnorm, double *fnorm_res);
 
+/* Gradients of the trained model with respect to theta = (log gamma, s_1..s_d), the
+ * kernel being exp(-gamma sum_f e^{2 s_f} (x_if - x_jf)^2) (per-feature input scales),
+ * with alpha and b held fixed (envelope theorem at the solver's optimum):
+ *   D(theta) = sum alpha - 1/2 alpha^T P K_Y P alpha, P the orthogonal projector onto
+ *              range(K_Y Omega) -- the trained dual objective through the factor
+ *              construction (U U^T = P K_Y P when W^T K_Y W is positive definite);
+ *   L(theta) = mean over held-out rows of the smoothed hinge h(y_t f(x_t)), f the exact-
+ *              kernel decision function with bias b; h(z) = 1/2 - z (z <= 0),
+ *              (1 - z)^2 / 2 (0 < z < 1), 0 (z >= 1).
+ * FP64 throughout, brute force over pairs (no expansion of (x_i - x_j)^2).  vals[0] = D,
+ * vals[1] = L; gD, gL (d + 1 each, component 0 = d/dlog gamma) may be NULL (values only).
+ * Returns 0, or -1 when W^T K_Y W is not positive definite. */
+int orc_grad(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
+             const float *omega, const double *alpha, double b, double log_gamma,
+             const double *log_scales, const float *Xt, const float *yt, int64_t nt,
+             double *vals, double *gD, double *gL);
+
 /* Threads the oracle's OpenMP regions use (the CPU baseline reports it). */
 int orc_num_threads(void);
 void orc_set_num_threads(int t);

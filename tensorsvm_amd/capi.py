@@ -73,7 +73,9 @@ def _bind_optional(L):
         fn = getattr(L, name, None)
         if fn is not None:
             fn.argtypes = args
-            if res is not None:
+            if res == 0:
+                fn.restype = None
+            elif res is not None:
                 fn.restype = res
 
 
@@ -105,6 +107,14 @@ class IpmInfo(ctypes.Structure):
 
 class SyrkInfo(ctypes.Structure):
     _fields_ = [("n_fp64_rows", _I64), ("bf16_bound", _D), ("t_bf16", _D), ("bf16_tflops", _D)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class GradInfo(ctypes.Structure):
+    _fields_ = [("t_forward", _D), ("t_backward", _D), ("backward_flops", _D),
+                ("backward_tflops", _D), ("t_total", _D), ("realk", _I64), ("orth_passes", _INT)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -156,7 +166,48 @@ _EXTRA = [
     ("tsvm_lra", [_P, _P, _P, _I64, _I64, _I64, _D, _INT, _P, _P, ctypes.POINTER(LraInfo)], None),
     ("tsvm_lra_dev", [_P, _P, _P, _I64, _I64, _I64, _D, _INT, _P, _P, ctypes.POINTER(LraInfo)], None),
     ("tsvm_weighted_syrk_dev", [_P, _P, _I64, _I64, _P, _INT, _D, _P, ctypes.POINTER(SyrkInfo)], None),
+    ("tsvm_read_model", [ctypes.c_char_p, ctypes.POINTER(_P)], None),
+    ("tsvm_model_free", [_P], 0),
+    ("tsvm_model_get", [_P, ctypes.POINTER(_INT), ctypes.POINTER(_D), ctypes.POINTER(_D),
+                        ctypes.POINTER(_INT), ctypes.POINTER(_I64), ctypes.POINTER(_I64)], None),
+    ("tsvm_predict", [_P, _P, _P, _I64, _I64, _P, _P], None),
+    ("tsvm_grad_dev", [_P, _P, _P, _I64, _I64, _P, _D, _I64, _D, _P, _P, _P, _P, _I64, _P, _P, _P,
+                       ctypes.POINTER(GradInfo)], None),
 ]
+
+
+
+class Model:
+    """A model file written by the trainer (LIBSVM RBF or LIBLINEAR text), parsed by
+    tsvm_read_model; scored with Context.predict."""
+
+    def __init__(self, path):
+        L = load()
+        self._lib = L
+        h = _P()
+        st = L.tsvm_read_model(os.fsencode(path), ctypes.byref(h))
+        if st != 0:
+            raise TsvmError(f"tsvm_read_model({path}) failed with status {st}")
+        self.h = h
+        kt, g, rho = _INT(), _D(), _D()
+        lab = (_INT * 2)()
+        nsv, d = _I64(), _I64()
+        L.tsvm_model_get(h, ctypes.byref(kt), ctypes.byref(g), ctypes.byref(rho), lab,
+                         ctypes.byref(nsv), ctypes.byref(d))
+        self.kernel_type, self.gamma, self.rho = kt.value, g.value, rho.value
+        self.labels = (lab[0], lab[1])
+        self.nsv, self.d = nsv.value, d.value
+
+    def close(self):
+        if self.h:
+            self._lib.tsvm_model_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def read_libsvm(path, scale256=False):
@@ -331,6 +382,34 @@ class Context:
         self._check(self.lib.tsvm_decision_function(self.h, _ptr(X), _ptr(y), _ptr(alpha), n, d, gamma,
                                                     b, _ptr(T), m, _ptr(f)), "tsvm_decision_function")
         return f
+
+    def grad_dev(self, X, y, n, d, alpha, b, k, log_gamma, log_scales, omega, Xt=None, yt=None,
+                 nt=0, want_grad=True):
+        """(D, L, g_dual, g_loss, info): the trained dual through the factor and the held-out
+        mean smoothed hinge, and their gradients in (log gamma, log_scales); device X, y,
+        alpha (FP64), omega, Xt, yt."""
+        sc = np.ascontiguousarray(log_scales, dtype=np.float64)
+        vals = np.zeros(2)
+        gD = np.zeros(d + 1)
+        gL = np.zeros(d + 1)
+        info = GradInfo()
+        self._check(self.lib.tsvm_grad_dev(
+            self.h, _ptr(X), _ptr(y), n, d, _ptr(alpha), b, k, log_gamma,
+            sc.ctypes.data_as(_P), _ptr(omega), _ptr(Xt) if nt else None,
+            _ptr(yt) if nt else None, nt, vals.ctypes.data_as(_P),
+            gD.ctypes.data_as(_P) if want_grad else None,
+            gL.ctypes.data_as(_P) if want_grad else None, ctypes.byref(info)), "tsvm_grad_dev")
+        return vals[0], vals[1], gD, gL, info
+
+    def predict(self, model, T):
+        """(decision values, predicted labels) of the rows T under a Model."""
+        T = _f32(T)
+        nt, d = T.shape
+        f = np.empty(nt)
+        pred = np.empty(nt, dtype=np.float32)
+        self._check(self.lib.tsvm_predict(self.h, model.h, _ptr(T), nt, d, _ptr(f), _ptr(pred)),
+                    "tsvm_predict")
+        return f, pred
 
     def weighted_syrk_dev(self, Z, n, k, w, precision, M, budget=0.0):
         """M (device, k x k FP64) = I + Z^T diag(w) Z for device Z (n x k FP32), w (FP64)."""

@@ -114,6 +114,17 @@ tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx* ctx, const float* Z, int64_t n, int
   return tsvm::weighted_syrk_dev(&ctx->c, Z, n, k, w, precision, budget, M, info);
 }
 
+tsvm_status tsvm_grad_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t n, int64_t d,
+                          const double* alpha, double b, int64_t k, double log_gamma,
+                          const double* log_scales, const float* omega, const float* Xt,
+                          const float* yt, int64_t nt, double* vals, double* g_dual,
+                          double* g_loss, tsvm_grad_info* info) {
+  if (!ctx) return TSVM_ERR_ARG;
+  hipSetDevice(ctx->c.device);
+  return tsvm::grad_dev(&ctx->c, X, y, n, d, alpha, b, k, log_gamma, log_scales, omega, Xt, yt,
+                        nt, vals, g_dual, g_loss, info);
+}
+
 tsvm_status tsvm_kernel_product(tsvm_ctx* ctx, const float* X1, const float* y1, int64_t m,
                                 const float* X2, const float* y2, int64_t n, int64_t d,
                                 const float* A, int64_t r, double gamma, int precision,

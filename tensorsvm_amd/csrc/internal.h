@@ -5,6 +5,7 @@
 
 #include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/tsvm.h"
 
@@ -60,6 +61,12 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
                         int64_t ldb, int64_t kb, int64_t n, const double* w, bool lower_only,
                         double* C, const int* rows = nullptr);
 
+// ---- factor construction helpers (lra.cpp) ---------------------------------------
+// Shifted Cholesky-QR of the n x k FP32 block W in place (tmp: n x k scratch); G receives
+// the Gram of the basis as stored (k x k, host), passes the number of QR passes taken.
+tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
+                        std::vector<double>& G, int* passes);
+
 // ---- weighted Gram M = I + Z^T diag(w) Z of the capacitance (syrk.hip) -----------
 // Hybrid form: bulk rows (small w_i |z_i|^2) on BF16 MFMA, the rest on FP64 MFMA.
 struct HybridSyrk {
@@ -83,6 +90,12 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
                              int64_t nh, double* M, hipEvent_t* ev);
 // flops of one BF16 part, SYRK convention (rows streamed x k (k + 1))
 double hybrid_syrk_flops(const HybridSyrk* hs);
+// tsvm_grad_dev (grad.hip)
+tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d,
+                     const double* alpha, double b, int64_t k, double log_gamma,
+                     const double* log_scales, const float* Om, const float* Xt,
+                     const float* yt, int64_t nt, double* vals, double* gD, double* gL,
+                     tsvm_grad_info* info);
 // tsvm_weighted_syrk_dev (synchronises before returning)
 tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, const double* w,
                               int precision, double budget, double* M, tsvm_syrk_info* info);

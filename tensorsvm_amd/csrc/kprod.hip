@@ -278,7 +278,8 @@ __global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict_
                                                   const float* __restrict__ y1,
                                                   float* __restrict__ W, int64_t ldw, int64_t m,
                                                   int64_t r, int64_t n_stages, int64_t r_pad,
-                                                  const float* __restrict__ sc) {
+                                                  const float* __restrict__ sc, int64_t nxb,
+                                                  int64_t ntiles) {
   using ET = typename std::conditional<EM == EM_BF16, __bf16, _Float16>::type;
   using ET8 = typename std::conditional<EM == EM_BF16, bf16x8_t, f16x8_t>::type;
   const ET* __restrict__ AT = (const ET*)ATv;
@@ -298,8 +299,12 @@ __global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict_
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int l32 = lane & 31, h = lane >> 5;
-  const int64_t rowbase = (int64_t)blockIdx.x * 256 + wave * 64;
-  const int64_t c0 = (int64_t)blockIdx.y * BN;
+  // Persistent grid: the block walks output tiles blockIdx.x, +gridDim.x, ...; a round of
+  // resident blocks covers consecutive row blocks of one column block, so they stream
+  // the same Qop and AT chunks at nearly the same time and share them through L2.
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const int64_t rowbase = (tile % nxb) * 256 + wave * 64;
+  const int64_t c0 = (tile / nxb) * BN;
 
   // S^T B operand: this wave's P rows, resident in registers for the whole j loop.
   bf16x8_t pb[2][KS];
@@ -444,6 +449,7 @@ __global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict_
       }
     }
   }
+  }  // tile
 }
 
 // -------------------------------------------------------------------------------------
@@ -563,26 +569,28 @@ template <int NT, int KD>
 static void launch_bf16(int em, const __bf16* P, const __bf16* Q, const void* AT,
                         const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
                         int64_t n_chunks, int64_t r_pad, int64_t m_pad, const float* sc,
-                        hipStream_t s) {
+                        int cu_count, hipStream_t s) {
   constexpr int CH = KBF16_CH;
   constexpr int OCC = NT <= 4 ? 2 : 1;
-  dim3 grid((unsigned)(m_pad / 256), (unsigned)(r_pad / (NT * 32)));
+  const int64_t nxb = m_pad / 256, ntiles = nxb * (r_pad / (NT * 32));
+  const int64_t resident = (int64_t)cu_count * OCC;
+  dim3 grid((unsigned)(ntiles < resident ? ntiles : resident));
   switch (em) {
     case EM_BF16:
       hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_BF16, OCC>), grid, dim3(256), 0, s, P, Q, AT,
-                         y1, W, ldw, m, r, n_chunks / CH, r_pad, sc);
+                         y1, W, ldw, m, r, n_chunks / CH, r_pad, sc, nxb, ntiles);
       break;
     case EM_F16:
       hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16, OCC>), grid, dim3(256), 0, s, P, Q, AT,
-                         y1, W, ldw, m, r, n_chunks / CH, r_pad, sc);
+                         y1, W, ldw, m, r, n_chunks / CH, r_pad, sc, nxb, ntiles);
       break;
     case EM_F16X2:
       hipLaunchKernelGGL((kprod_bf16<NT, KD, 1, EM_F16X2, 1>), grid, dim3(256), 0, s, P, Q, AT,
-                         y1, W, ldw, m, r, n_chunks, r_pad, sc);
+                         y1, W, ldw, m, r, n_chunks, r_pad, sc, nxb, ntiles);
       break;
     default:  // EM_F16R2 (its AT rows are twice as wide)
       hipLaunchKernelGGL((kprod_bf16<NT, KD, 1, EM_F16R2, 1>), grid, dim3(256), 0, s, P, Q, AT,
-                         y1, W, ldw, m, r, n_chunks, r_pad, sc);
+                         y1, W, ldw, m, r, n_chunks, r_pad, sc, nxb, ntiles);
       break;
   }
 }
@@ -659,13 +667,13 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
 #define KP_BF16_CASE(kd)                                                                        \
   case kd:                                                                                      \
     if (NT == 1)                                                                                \
-      launch_bf16<1, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, s);   \
+      launch_bf16<1, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, c->cu_count, s);   \
     else if (NT == 2)                                                                           \
-      launch_bf16<2, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, s);   \
+      launch_bf16<2, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, c->cu_count, s);   \
     else if (NT == 4)                                                                           \
-      launch_bf16<4, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, s);   \
+      launch_bf16<4, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, c->cu_count, s);   \
     else                                                                                        \
-      launch_bf16<8, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, s);   \
+      launch_bf16<8, kd>(em, Pb, Qb, Ab, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, sc, c->cu_count, s);   \
     break;
     switch (KD) {
       KP_BF16_CASE(16)

@@ -40,6 +40,64 @@ double now() {
 }
 }  // namespace
 
+// Shifted Cholesky-QR of the n x k FP32 block W (in place; tmp is an n x k scratch):
+// G = W^T W on FP64 MFMA, host Cholesky of G + s I, W <- W L^{-T}, repeated until G = I
+// to 1e-6 (at most 3 passes).  On return G holds the Gram of the basis as stored.
+tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
+                        std::vector<double>& G, int* passes) {
+  hipStream_t s = c->stream;
+  void *pG, *pB;
+  TSVM_HIP(c, c->ws.get("qr_G", (size_t)(k * k * 8), &pG));
+  TSVM_HIP(c, c->ws.get("qr_B", (size_t)(k * k * 8), &pB));
+  double* dG = (double*)pG;
+  double* dB = (double*)pB;
+  G.assign(k * k, 0.0);
+  std::vector<double> L(k * k), Li(k * k), T(k * k);
+  float* cur = W;
+  float* oth = tmp;
+  bool have_gram = false;
+  *passes = 0;
+  for (int pass = 0; pass < 3; pass++) {
+    TSVM_CHECK(gemm_tn_f64(c, cur, k, k, cur, k, k, n, nullptr, true, dG));
+    TSVM_HIP(c, hipMemcpyAsync(G.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
+    TSVM_HIP(c, hipStreamSynchronize(s));
+    double dev = 0.0, tr = 0.0;
+    for (int64_t i = 0; i < k; i++) {
+      tr += G[i * k + i];
+      for (int64_t j = 0; j < k; j++) dev = fmax(dev, fabs(G[i * k + j] - (i == j ? 1.0 : 0.0)));
+    }
+    if (!(tr > 0.0) || !std::isfinite(tr))
+      return fail(c, TSVM_ERR_NUMERIC, "cholqr: block has norm %g", tr);
+    have_gram = true;
+    if (dev < 1e-6) break;  // orthonormal to FP32 storage accuracy
+    *passes = pass + 1;
+    // shift (pass 0 only) keeps the Cholesky of an ill-conditioned Gram defined
+    double shift = pass == 0 ? 11.0 * ((double)n * k + (double)k * (k + 1)) * 1.11e-16 * tr : 0.0;
+    int fails = 0;
+    for (;;) {
+      memcpy(L.data(), G.data(), k * k * 8);
+      for (int64_t i = 0; i < k; i++) L[i * k + i] += shift;
+      if (host_chol(L.data(), k) == 0) break;
+      shift = shift > 0 ? shift * 10.0 : 1e-14 * tr;
+      if (++fails > 30) return fail(c, TSVM_ERR_NUMERIC, "cholqr: Cholesky failed");
+    }
+    host_tri_inv_lower(L.data(), k, Li.data());
+    for (int64_t i = 0; i < k; i++)  // B = L^{-T}
+      for (int64_t j = 0; j < k; j++) T[i * k + j] = Li[j * k + i];
+    TSVM_HIP(c, hipMemcpyAsync(dB, T.data(), k * k * 8, hipMemcpyHostToDevice, s));
+    TSVM_CHECK(gemm_nn_tall(c, cur, k, k, dB, k, n, oth, k));
+    std::swap(cur, oth);
+    have_gram = false;
+  }
+  if (cur != W) TSVM_HIP(c, hipMemcpyAsync(W, cur, n * k * 4, hipMemcpyDeviceToDevice, s));
+  if (!have_gram) {
+    TSVM_CHECK(gemm_tn_f64(c, W, k, k, W, k, k, n, nullptr, true, dG));
+    TSVM_HIP(c, hipMemcpyAsync(G.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
+  }
+  TSVM_HIP(c, hipStreamSynchronize(s));
+  return TSVM_OK;
+}
+
 tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d, int64_t k,
                     double gamma, int precision, float* Om, float* U, tsvm_lra_info* info) {
   if (n <= 0 || d <= 0 || k <= 0 || k > n)
@@ -73,51 +131,8 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   double t1 = now();
   inf.t_product1 = ms * 1e-3;
 
-  // 2. shifted Cholesky-QR; the basis ping-pongs between W and U
-  float* cur = W;
-  float* oth = U;
-  bool have_gram = false;
-  for (int pass = 0; pass < 3; pass++) {
-    TSVM_CHECK(gemm_tn_f64(c, cur, k, k, cur, k, k, n, nullptr, true, dG));
-    TSVM_HIP(c, hipMemcpyAsync(G.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
-    TSVM_HIP(c, hipStreamSynchronize(s));
-    double dev = 0.0, tr = 0.0;
-    for (int64_t i = 0; i < k; i++) {
-      tr += G[i * k + i];
-      for (int64_t j = 0; j < k; j++) dev = fmax(dev, fabs(G[i * k + j] - (i == j ? 1.0 : 0.0)));
-    }
-    if (!(tr > 0.0) || !std::isfinite(tr))
-      return fail(c, TSVM_ERR_NUMERIC, "lra: range-finder block has norm %g", tr);
-    have_gram = true;
-    if (dev < 1e-6) break;  // orthonormal to FP32 storage accuracy
-    inf.orth_passes = pass + 1;
-    // shift (pass 0 only) keeps the Cholesky of an ill-conditioned Gram defined
-    double shift = pass == 0 ? 11.0 * ((double)n * k + (double)k * (k + 1)) * 1.11e-16 * tr : 0.0;
-    int fails = 0;
-    for (;;) {
-      memcpy(L.data(), G.data(), k * k * 8);
-      for (int64_t i = 0; i < k; i++) L[i * k + i] += shift;
-      if (host_chol(L.data(), k) == 0) break;
-      shift = shift > 0 ? shift * 10.0 : 1e-14 * tr;
-      if (++fails > 30) return fail(c, TSVM_ERR_NUMERIC, "lra: Cholesky-QR failed");
-    }
-    host_tri_inv_lower(L.data(), k, Li.data());
-    for (int64_t i = 0; i < k; i++)  // B = L^{-T}
-      for (int64_t j = 0; j < k; j++) T[i * k + j] = Li[j * k + i];
-    TSVM_HIP(c, hipMemcpyAsync(dB, T.data(), k * k * 8, hipMemcpyHostToDevice, s));
-    TSVM_CHECK(gemm_nn_tall(c, cur, k, k, dB, k, n, oth, k));
-    std::swap(cur, oth);
-    have_gram = false;
-  }
-  if (cur != W) {
-    TSVM_HIP(c, hipMemcpyAsync(W, cur, n * k * 4, hipMemcpyDeviceToDevice, s));
-    cur = W;
-  }
-  // 3. Gram of the basis as stored
-  if (!have_gram) {
-    TSVM_CHECK(gemm_tn_f64(c, W, k, k, W, k, k, n, nullptr, true, dG));
-    TSVM_HIP(c, hipMemcpyAsync(G.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
-  }
+  // 2-3. shifted Cholesky-QR (U is the scratch basis); G = Gram of the basis as stored
+  TSVM_CHECK(cholqr_orth(c, W, U, n, k, G, &inf.orth_passes));
   TSVM_HIP(c, hipStreamSynchronize(s));
   double t2 = now();
   inf.t_orth = t2 - t1;

@@ -53,6 +53,8 @@ def lib():
         L.orc_write_model_linear.argtypes = [ctypes.c_char_p, P, I64, D, ctypes.c_int, ctypes.c_int]
         L.orc_decision.argtypes = [P, P, P, I64, I64, D, D, P, I64, P]
         L.orc_fnorm_res.argtypes = [P, P, I64, I64, D, P, I64, ctypes.POINTER(D), ctypes.POINTER(D)]
+        L.orc_grad.argtypes = [P, P, I64, I64, I64, P, P, D, D, P, P, P, I64, P, P, P]
+        L.orc_grad.restype = ctypes.c_int
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_num_threads.argtypes = [ctypes.c_int]
         _LIB = L
@@ -214,3 +216,26 @@ def fnorm_res(X, y, gamma, U):
     a, b = D(), D()
     lib().orc_fnorm_res(_p(X), _p(y), n, d, gamma, _p(U), U.shape[1], ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+def grad(X, y, k, omega, alpha, b, log_gamma, log_scales, Xt=None, yt=None, want_grad=True):
+    """FP64 gradients of the trained dual D and the held-out mean smoothed hinge L with
+    respect to (log gamma, log_scales); returns (D, L, gD, gL, status)."""
+    X, y = f32(X), f32(y)
+    n, d = X.shape
+    om = f32(omega).reshape(-1)
+    alpha = f64(alpha)
+    sc = f64(log_scales)
+    if Xt is None:
+        Xt = np.zeros((0, d), np.float32)
+        yt = np.zeros(0, np.float32)
+    Xt, yt = f32(Xt), f32(yt)
+    nt = Xt.shape[0]
+    vals = np.zeros(2)
+    gD = np.zeros(d + 1)
+    gL = np.zeros(d + 1)
+    st = lib().orc_grad(_p(X), _p(y), n, d, k, _p(om), _p(alpha), b, log_gamma, _p(sc),
+                        _p(Xt) if nt else None, _p(yt) if nt else None, nt, _p(vals),
+                        _p(gD) if want_grad else None, _p(gL) if want_grad else None)
+    return vals[0], vals[1], gD, gL, st
+

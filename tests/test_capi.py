@@ -104,3 +104,25 @@ def test_context_without_device_is_an_error():
         pytest.skip("a GPU is present")
     with pytest.raises(T.TsvmError):
         T.Context(0)
+
+def test_model_reader_on_golden_models(tmp_path):
+    """The golden model files parse back to the values that were written (no GPU)."""
+    import tensorsvm_amd as T
+    m = np.load(os.path.join(G, "model_inputs.npz"))
+    md = T.Model(os.path.join(G, "model_rbf.txt"))
+    assert md.kernel_type == 2
+    assert md.gamma == float("%.7f" % float(m["gamma"]))
+    assert md.rho == float("%f" % -float(m["b"]))
+    assert md.labels == (int(m["pos"]), int(m["neg"]))
+    assert md.nsv == int((m["alpha"] > 1e-6).sum())
+    assert md.d <= m["inst"].shape[1]
+    ml = T.Model(os.path.join(G, "model_linear.txt"))
+    assert ml.kernel_type == 0 and ml.d == m["w"].shape[0]
+    assert abs(ml.rho + float(m["b"])) <= 1e-15 * (1 + abs(float(m["b"])))
+    bad = tmp_path / "bad.model"
+    bad.write_text("svm_type c_svc\nkernel_type poly\nSV\n")
+    with pytest.raises(T.TsvmError):
+        T.Model(str(bad))
+    with pytest.raises(T.TsvmError):
+        T.Model(str(tmp_path / "missing.model"))
+

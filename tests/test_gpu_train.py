@@ -131,3 +131,54 @@ def test_cli_linear_model_file(tmp_path):
     bo, wo = O.bias_linear(al, lab, xs, 1.0)
     assert np.allclose(w, wo, atol=1e-4 * (1 + np.abs(wo).max()))
     assert abs(float(lines[6 + d]) - bo) <= 1e-3 * (1 + abs(bo))
+
+
+def test_predict_from_written_rbf_model(ctx, tmp_path):
+    """The predictor on the written model file reproduces the in-memory decision function
+    (up to the file's 6-decimal coefficients and features) and the oracle's FP64 one; the
+    command line prints the matching accuracy."""
+    import tensorsvm_amd as T
+    gamma, C, k = 0.05, 32.0, 32
+    X, y, Xt, yt = _split(3000, 18, 41)
+    al, _, info = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=T.PREC_FP32,
+                            omega=O.gaussrand(X.shape[0] * k))
+    path = str(tmp_path / "m.model")
+    T.capi.write_model_rbf(path, al, y, X, C, gamma, info.b, 1, -1)
+    md = T.Model(path)
+    assert md.nsv == int((al > 1e-6).sum())
+    f_file, pred = ctx.predict(md, Xt)
+    f_mem = ctx.decision_function(X, y, al, gamma, info.b, Xt)
+    scale = np.abs(f_mem).max()
+    assert np.abs(f_file - f_mem).max() <= 1e-4 * scale + 1e-5
+    # FP64 oracle decision on the model's own (rounded) coefficients is the same function
+    f_orc = O.decision(al, y, X, gamma, info.b, Xt)
+    assert np.abs(f_file - f_orc).max() <= 1e-4 * scale + 1e-5
+    sure = np.abs(f_mem) > 1e-3 * scale
+    assert np.all(pred[sure] == np.where(f_mem[sure] > 0, 1.0, -1.0))
+    te = str(tmp_path / "test.libsvm")
+    _write_libsvm(te, Xt, yt)
+    out = str(tmp_path / "pred.txt")
+    exe = os.path.join(ROOT, "tensorsvm_amd", "tensorsvm-predict")
+    r = subprocess.run([exe, te, path, out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    p2 = np.array([float(v) for v in open(out).read().split()])
+    acc = np.mean(p2 == yt)
+    assert f"Accuracy = {100 * acc:g}%" in r.stdout, r.stdout
+    assert np.mean(p2 == pred) >= 0.999
+
+
+def test_predict_from_written_linear_model(ctx, tmp_path):
+    import tensorsvm_amd as T
+    rng = np.random.default_rng(9)
+    n, d = 2000, 7
+    y = np.where(rng.random(n) < 0.5, 1.0, -1.0).astype(np.float32)
+    X = (rng.normal(size=(n, d)) + 0.6 * y[:, None]).astype(np.float32)
+    al, w, info = ctx.train(X, y, k=d, C=1.0, kernel_type=0)
+    path = str(tmp_path / "l.model")
+    T.capi.write_model_linear(path, w, info.b, 1, -1)
+    md = T.Model(path)
+    f, pred = ctx.predict(md, X)
+    ref = X.astype(np.float64) @ w + info.b
+    assert np.abs(f - ref).max() <= 1e-12 * (1 + np.abs(ref).max()) + 1e-9
+    assert np.all(pred == np.where(ref > 0, 1.0, -1.0))
+

@@ -169,3 +169,30 @@ def test_bias_rbf_matches_kkt():
     assert abs(b - float(f["b"])) <= 3 * sd + 1e-4
     assert abs(b + info.y) <= 3 * sd + 1e-4
     assert abs(info.y + float(f["b"])) < 1e-4
+
+
+def test_grad_matches_central_differences():
+    """The FP64 gradient of the dual through the factor (all three projector terms) and of
+    the held-out smoothed hinge, against central differences of the same objectives."""
+    from datagen import scaled_mixture
+    X, y, g0 = scaled_mixture(360, 5, 21)
+    X, Xt, y, yt = X[:280], X[280:], y[:280], y[280:]
+    n, d, k = X.shape[0], X.shape[1], 14
+    rng = np.random.default_rng(4)
+    om = O.gaussrand(n * k)
+    alpha = rng.uniform(0.0, 4.0, n)
+    alpha[rng.random(n) < 0.4] = 0.0
+    lg, s = np.log(g0), rng.normal(0.0, 0.3, d)
+    D, L, gD, gL, st = O.grad(X, y, k, om, alpha, 0.15, lg, s, Xt, yt)
+    assert st == 0
+    h = 1e-5
+    for m in range(d + 1):
+        e = np.zeros(d + 1)
+        e[m] = h
+        Dp, Lp, _, _, _ = O.grad(X, y, k, om, alpha, 0.15, lg + e[0], s + e[1:], Xt, yt, False)
+        Dm, Lm, _, _, _ = O.grad(X, y, k, om, alpha, 0.15, lg - e[0], s - e[1:], Xt, yt, False)
+        assert abs((Dp - Dm) / (2 * h) - gD[m]) <= 1e-6 * np.abs(gD).max(), (m, gD[m])
+        assert abs((Lp - Lm) / (2 * h) - gL[m]) <= 1e-6 * np.abs(gL).max(), (m, gL[m])
+    # log gamma acts like a common shift of the log scales (K depends on gamma e^{2 s_f})
+    assert abs(gD[0] - 0.5 * gD[1:].sum()) <= 1e-9 * np.abs(gD).max()
+
