@@ -13,9 +13,17 @@ form is run once after the timed steps for the comparison the metric names, and 
 models are scored on the held-out rows.
 
 Reported beside the timing:
-  roofline      the fused kernel-product kernel (>98% of training flops): algorithmic
-                flops 2 m n (d + k) per launch / its launch time (HIP events on the
-                library's stream), against the BF16 dense MFMA peak
+  roofline      the dominant kernel, the factor's sketch product K_Y Omega (split-fp16
+                fused kernel product): algorithmic flops 2 m n (d + k) per launch / its
+                launch time (HIP events on the library's stream), against the dense
+                BF16/FP16 MFMA peak; traffic = HBM bytes per launch from rocprofv3
+                FETCH_SIZE + WRITE_SIZE (calibrated on a copy of known size)
+  zdz           Z^T D^-1 Z: the BF16 MFMA part of the hybrid capacitance (n k (k+1) flops
+                per call / its average time, HIP events) against the BF16 peak
+  grad          the differentiation form (200,000 + 50,000 rows, d = 32, k = 256, FP32):
+                gradient of the trained dual and of the held-out smoothed hinge in
+                (log gamma, 32 log-scales), backward flops / time against the FP32 peak,
+                with a central-difference check of the log-gamma component (h = 1e-3)
   cpu_baseline  the CPU oracle's kernel product (FP64, all host threads) on a bounded row
                 sample, extrapolated to the two full products of the factor construction
 """
@@ -39,6 +47,63 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# HBM bytes per launch of the sketch kernel at this workload: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch) scaled by the factor that the same counters give for the
+# bench's 3.28 GB Omega copy (copyBuffer) -- see the profile named in "source"
+KPROD_TRAFFIC = {"fetch_bytes": 416_600_207_377, "write_bytes": 3_279_159_296,
+                 "source": "profiles/kprod_sketch_c2_pmc.md"}
+
+
+def grad_form(ctx, torch, n=200_000, nt=50_000, d=32, k=256, C=1.0, seed=1003):
+    """Differentiation form: train (FP32), then the gradient of the trained dual and of
+    the held-out smoothed hinge; returns the measurement object."""
+    import numpy as np
+    import tensorsvm_amd as T
+    from datagen import scaled_mixture
+    X, y, g0 = scaled_mixture(n + nt, d, seed)
+    Xt, yt = np.ascontiguousarray(X[n:]), np.ascontiguousarray(y[n:])
+    X, y = np.ascontiguousarray(X[:n]), np.ascontiguousarray(y[:n])
+    dev = torch.device("cuda:0")
+    dX, dy = torch.from_numpy(X).to(dev), torch.from_numpy(y).to(dev)
+    dXt, dyt = torch.from_numpy(Xt).to(dev), torch.from_numpy(yt).to(dev)
+    om = torch.from_numpy(T.gaussrand_fill(n * k)).to(dev)
+    dom = om.clone()
+    dal = torch.empty(n, dtype=torch.float64, device=dev)
+    prm = T.capi.train_params(k=k, C=C, gamma=g0, kernel_type=2, precision=T.PREC_FP32,
+                              stop=T.capi.STOP_GAP, tol=1e-5, maxit=300, refine=1)
+    t0 = time.perf_counter()
+    info = ctx.train_dev(dX, dy, n, d, prm, dom, dal)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    lg, s0 = float(np.log(g0)), np.zeros(d)
+    D, L, gD, gL, gi = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg, s0, om, dXt, dyt, nt)
+    h = 1e-3
+    Dp, Lp, _, _, _ = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg + h, s0, om, dXt, dyt, nt, False)
+    Dm, Lm, _, _, _ = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg - h, s0, om, dXt, dyt, nt, False)
+    fdD, fdL = (Dp - Dm) / (2 * h), (Lp - Lm) / (2 * h)
+    h2 = 1e-2  # a wider step: FP32 noise of the objectives / h shrinks tenfold
+    Dp2, Lp2, _, _, _ = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg + h2, s0, om, dXt, dyt, nt, False)
+    Dm2, Lm2, _, _, _ = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg - h2, s0, om, dXt, dyt, nt, False)
+    fdD2, fdL2 = (Dp2 - Dm2) / (2 * h2), (Lp2 - Lm2) / (2 * h2)
+    tf = gi.backward_tflops
+    log(f"grad: train {t_train:.2f}s, gradient {gi.t_total:.2f}s (backward {gi.t_backward:.3f}s, "
+        f"{tf:.1f} TF/s); dD/dlog(gamma) {gD[0]:.6e} vs FD {fdD:.6e}")
+    return {"workload": f"scaled-variance mixture n={n} train + {nt} held-out, d={d}, k={k}, FP32",
+            "t_train_s": round(t_train, 3), "t_grad_s": round(gi.t_total, 3),
+            "t_backward_s": round(gi.t_backward, 4), "backward_tflops": round(tf, 2),
+            "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": PEAK_FP32,
+                         "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32, 4)},
+            "dual": D, "heldout_loss": L,
+            "dD_dloggamma": gD[0], "dD_dloggamma_fd": fdD,
+            "dD_rel_err_vs_fd": abs(gD[0] - fdD) / abs(fdD),
+            "dL_dloggamma": gL[0], "dL_dloggamma_fd": fdL,
+            "dL_rel_err_vs_fd": abs(gL[0] - fdL) / max(abs(fdL), 1e-300),
+            "fd_h1e-2": {"dD_rel_err": abs(gD[0] - fdD2) / abs(fdD2),
+                         "dL_rel_err": abs(gL[0] - fdL2) / max(abs(fdL2), 1e-300)},
+            "ipm": {"iters": info.ipm.iters, "reason": info.ipm.reason,
+                    "gap_rel": info.ipm.gap_rel}, "nsv": info.nsv}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +114,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-rows", type=int, default=384)
+    ap.add_argument("--no-grad", action="store_true", help="skip the differentiation form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trainer is "
@@ -123,14 +189,25 @@ def main():
                 "alpha_max_abs_diff_vs_bf16": float(np.abs(alpha32 - alpha_bf16).max())}
         log(f"fp32 train {dt32:.2f}s (lra {info32.lra.t_total:.2f}s), accuracy {acc32:.4f}")
 
-    # roofline of the fused kernel-product kernel (second product of the last BF16 step)
+    # roofline of the dominant kernel: the sketch product (first product of the last BF16
+    # step); the second product (plain fp16) is listed beside it
     flops = 2.0 * ntr * ntr * (d + k)
-    t_kp = bf16_info["lra"]["t_product2"]
+    t_kp = bf16_info["lra"]["t_product1"]
     achieved = flops / t_kp / 1e12
+    tr = KPROD_TRAFFIC
     roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_BF16, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_BF16, 4), "traffic": None,
-                "kernel": "kprod_bf16 (fused RBF kernel product)",
-                "flops_per_launch": flops}
+                "frac": round(achieved / PEAK_BF16, 4),
+                "traffic": (tr["fetch_bytes"] + tr["write_bytes"]
+                            if tr["fetch_bytes"] is not None else None),
+                "kernel": "kprod_bf16<8,64,1,EM_F16X2> (fused RBF product, sketch K_Y Omega)",
+                "flops_per_launch": flops, "launch_s": round(t_kp, 4),
+                "traffic_source": tr["source"],
+                "second_product": {
+                    "kernel": "kprod_bf16<4,64,2,EM_F16> (K_Y W)",
+                    "achieved": round(flops / bf16_info["lra"]["t_product2"] / 1e12, 1),
+                    "frac": round(flops / bf16_info["lra"]["t_product2"] / 1e12 / PEAK_BF16, 4)}}
+
+    grad = None if args.no_grad else grad_form(ctx, torch)
 
     cpu = None
     if not args.no_cpu:
@@ -160,9 +237,16 @@ def main():
         "fp32": fp32,
         "speedup_bf16_vs_fp32": (round(fp32["train_s"] / value, 2) if fp32 else None),
         "omega_gen_s": round(t_omega, 2),
-        "zdz": {"form": "FP64 MFMA capacitance (both forms)",
-                "tflops": bf16_info["ipm"]["syrk_tflops"],
-                "frac_of_fp64_peak": round(bf16_info["ipm"]["syrk_tflops"] / PEAK_FP64, 4)},
+        "zdz": {"form": "hybrid capacitance: bulk rows on BF16 MFMA (error-budgeted split), "
+                        "high-weight rows on FP64 MFMA",
+                "bf16_tflops": round(bf16_info["ipm"]["syrk_bf16_tflops"], 1),
+                "bf16_frac_of_peak": round(bf16_info["ipm"]["syrk_bf16_tflops"] / PEAK_BF16, 4),
+                "bf16_row_frac": round(bf16_info["ipm"]["bf16_row_frac"], 4),
+                "bf16_iters": bf16_info["ipm"]["syrk_bf16_iters"],
+                "t_capacitance_s": round(bf16_info["ipm"]["t_syrk"], 3),
+                "t_bf16_part_s": round(bf16_info["ipm"]["t_syrk_bf16"], 3),
+                "fp32_form_fp64_tflops": (round(fp32["info"]["ipm"]["syrk_tflops"], 1) if fp32 else None)},
+        "grad": grad,
     }
     print(json.dumps(out))
 

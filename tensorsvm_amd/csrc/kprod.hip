@@ -463,7 +463,8 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
                                                  const float* __restrict__ Af,
                                                  const float* __restrict__ y1,
                                                  float* __restrict__ W, int64_t ldw, int64_t m,
-                                                 int64_t r, int64_t n_chunks, int64_t r_pad) {
+                                                 int64_t r, int64_t n_chunks, int64_t r_pad,
+                                                 int64_t seg_chunks, int64_t seg_stride) {
   constexpr int BN = NT * 32;
   constexpr int KSF = KF / 2;
   constexpr int AS = BN + 8;          // LDS row stride (floats) of the A' chunk
@@ -518,11 +519,15 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
     }
   };
 
-  gload(0);
-  for (int64_t t = 0; t < n_chunks; t++) {
+  // j segment of this block (blockIdx.z): its FP32 sum goes to its own output slice
+  const int64_t t0 = (int64_t)blockIdx.z * seg_chunks;
+  const int64_t t1 = t0 + seg_chunks < n_chunks ? t0 + seg_chunks : n_chunks;
+  W += (int64_t)blockIdx.z * seg_stride;
+  gload(t0);
+  for (int64_t t = t0; t < t1; t++) {
     sstore();
     __syncthreads();
-    gload(t + 1 < n_chunks ? t + 1 : t);
+    gload(t + 1 < t1 ? t + 1 : t);
     __builtin_amdgcn_sched_barrier(0);
     f32x16_t s;
 #pragma unroll
@@ -597,10 +602,24 @@ static void launch_bf16(int em, const __bf16* P, const __bf16* Q, const void* AT
 template <int NT, int KF>
 static void launch_f32(const float* P, const float* Q, const float* Af, const float* y1,
                        float* W, int64_t ldw, int64_t m, int64_t r, int64_t n_chunks,
-                       int64_t r_pad, int64_t m_pad, hipStream_t s) {
-  dim3 grid((unsigned)(m_pad / 128), (unsigned)(r_pad / (NT * 32)));
+                       int64_t r_pad, int64_t m_pad, int64_t nseg, int64_t seg_chunks,
+                       int64_t seg_stride, hipStream_t s) {
+  dim3 grid((unsigned)(m_pad / 128), (unsigned)(r_pad / (NT * 32)), (unsigned)nseg);
   hipLaunchKernelGGL((kprod_f32<NT, KF>), grid, dim3(256), 0, s, P, Q, Af, y1, W, ldw, m, r,
-                     n_chunks, r_pad);
+                     n_chunks, r_pad, seg_chunks, seg_stride);
+}
+
+// W[i, c] = FP32(sum over segments of part[s][i][c], in FP64 and segment order)
+__global__ void seg_reduce_kernel(const float* __restrict__ part, int64_t nseg, int64_t m,
+                                  int64_t r, float* __restrict__ W, int64_t ldw) {
+  const int64_t tot = m * r;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int64_t t = 0; t < nseg; t++) acc += (double)part[t * tot + e];
+    const int64_t i = e / r, c = e - i * r;
+    W[i * ldw + c] = (float)acc;
+  }
 }
 
 tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t m,
@@ -711,13 +730,32 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
     hipLaunchKernelGGL(prep_A_f32, dim3((unsigned)ceil_div(n_pad * r_pad, 256)), dim3(256), 0, s,
                        A, lda, y2, n, n_pad, r, r_pad, (float*)Af);
     TSVM_HIP(c, hipGetLastError());
+    // FP32 sums over segments of 8192 columns (the reference's tile, whose products it
+    // accumulates with beta = 1, 2537-2555), added in FP64: the rounding of an output
+    // grows like sqrt(8192) instead of sqrt(n).  Bounded by 8 GiB of partial sums.
+    const int64_t seg_len = 256;  // chunks of 32 columns
+    int64_t nseg = ceil_div(n_chunks, seg_len);
+    const int64_t max_seg = (int64_t)(8.0 * (1ll << 30) / ((double)m * r * 4.0));
+    if (nseg > max_seg) nseg = max_seg > 1 ? max_seg : 1;
+    const int64_t seg_chunks = ceil_div(n_chunks, nseg);
+    nseg = ceil_div(n_chunks, seg_chunks);
+    float* Wout = W;
+    int64_t ldo = ldw, seg_stride = 0;
+    if (nseg > 1) {
+      void* pp;
+      TSVM_HIP(c, c->ws.get("kp_seg", (size_t)(nseg * m * r * 4), &pp));
+      Wout = (float*)pp;
+      ldo = r;
+      seg_stride = m * r;
+    }
     const float *Pf = (const float*)P, *Qf = (const float*)Q, *Aff = (const float*)Af;
-#define KP_F32_CASE(kf)                                                                        \
-  case kf:                                                                                     \
-    if (NT == 1) launch_f32<1, kf>(Pf, Qf, Aff, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s);  \
-    else if (NT == 2) launch_f32<2, kf>(Pf, Qf, Aff, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s); \
-    else if (NT == 4) launch_f32<4, kf>(Pf, Qf, Aff, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s); \
-    else launch_f32<8, kf>(Pf, Qf, Aff, y1, W, ldw, m, r, n_chunks, r_pad, m_pad, s);          \
+#define KP_F32_ARGS Pf, Qf, Aff, y1, Wout, ldo, m, r, n_chunks, r_pad, m_pad, nseg, seg_chunks, seg_stride, s
+#define KP_F32_CASE(kf)                                   \
+  case kf:                                                \
+    if (NT == 1) launch_f32<1, kf>(KP_F32_ARGS);          \
+    else if (NT == 2) launch_f32<2, kf>(KP_F32_ARGS);     \
+    else if (NT == 4) launch_f32<4, kf>(KP_F32_ARGS);     \
+    else launch_f32<8, kf>(KP_F32_ARGS);                  \
     break;
     switch (KF) {
       KP_F32_CASE(4)
@@ -735,6 +773,10 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
         return fail(c, TSVM_ERR_ARG, "kernel_product(fp32): KF=%d not compiled", KF);
     }
 #undef KP_F32_CASE
+#undef KP_F32_ARGS
+    if (nseg > 1)
+      hipLaunchKernelGGL(seg_reduce_kernel, dim3(4096), dim3(256), 0, s, (const float*)Wout, nseg, m,
+                         r, W, ldw);
   } else {
     return fail(c, TSVM_ERR_ARG, "kernel_product: unknown precision %d", precision);
   }
