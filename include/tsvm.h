@@ -168,6 +168,13 @@ tsvm_status tsvm_model_get(const tsvm_model *model, int *kernel_type, double *ga
 tsvm_status tsvm_predict(tsvm_ctx *ctx, const tsvm_model *model, const float *T, int64_t nt,
                          int64_t d, double *f, float *pred);
 
+/* -analysis (tensorsvm-train.cu:2328-2360, rbf_fnorm_res 2388-2467): |K_Y|_F and
+ * |K_Y - U U^T|_F over all n^2 pairs, on the GPU without storing any n x n block.
+ * Device X (n x d, d <= 64), y, U (n x k FP32 row-major); results on the host. */
+tsvm_status tsvm_fnorm_res_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t n,
+                               int64_t d, double gamma, const float *U, int64_t k,
+                               double *fnorm, double *fnorm_res);
+
 /* Interior-point solver parameters (the reference's constants are the defaults). */
 #define TSVM_STOP_REF 0 /* mu, |r_x|, |r_y| < 1e-7 (tensorsvm-train.cu:1647) */
 #define TSVM_STOP_GAP 1 /* (pobj-dobj)/(1+|pobj|) <= tol, |r_x|/(1+sqrt n) <= tol,
@@ -233,6 +240,7 @@ typedef struct {
   int refine;         /* refinement sweeps of each Newton step */
   int verbose;        /* reference-style progress lines on stdout */
   int init;           /* IPM initial point, as tsvm_ipm_params.init */
+  int analysis;       /* -analysis: |K_Y|_F and |K_Y - U U^T|_F after the factor (RBF) */
 } tsvm_train_params;
 typedef struct {
   tsvm_lra_info lra;
@@ -242,6 +250,8 @@ typedef struct {
   int b_samples;      /* free SVs averaged (<= 50 RBF, <= 5000 linear) */
   int64_t nsv, nfree; /* alpha > 1e-6; and alpha < C - 1e-6 */
   double t_bias, t_total;  /* s */
+  double fnorm, fnorm_res; /* -analysis results (0 unless requested) */
+  double t_analysis;       /* s (not part of t_total) */
 } tsvm_train_info;
 void tsvm_train_default_params(tsvm_train_params *p);
 /* Train on X (n x d row-major FP32) with labels y in {+1,-1}: the RBF branch of main

@@ -128,12 +128,13 @@ def ipm_params(C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5, precision=P
 class TrainParams(ctypes.Structure):
     _fields_ = [("k", _I64), ("C", _D), ("gamma", _D), ("kernel_type", _INT), ("precision", _INT),
                 ("stop_mode", _INT), ("tol", _D), ("maxit", _INT), ("refine", _INT),
-                ("verbose", _INT), ("init", _INT)]
+                ("verbose", _INT), ("init", _INT), ("analysis", _INT)]
 
 
 class TrainInfo(ctypes.Structure):
     _fields_ = [("lra", LraInfo), ("ipm", IpmInfo), ("b", _D), ("b_std", _D), ("b_samples", _INT),
-                ("nsv", _I64), ("nfree", _I64), ("t_bias", _D), ("t_total", _D)]
+                ("nsv", _I64), ("nfree", _I64), ("t_bias", _D), ("t_total", _D),
+                ("fnorm", _D), ("fnorm_res", _D), ("t_analysis", _D)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("lra", "ipm")}
@@ -143,8 +144,9 @@ class TrainInfo(ctypes.Structure):
 
 
 def train_params(k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32, stop=STOP_REF,
-                 tol=1e-5, maxit=100, refine=1, verbose=0, init=0):
-    return TrainParams(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init)
+                 tol=1e-5, maxit=100, refine=1, verbose=0, init=0, analysis=0):
+    return TrainParams(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init,
+                       analysis)
 
 
 _EXTRA = [
@@ -171,6 +173,8 @@ _EXTRA = [
     ("tsvm_model_get", [_P, ctypes.POINTER(_INT), ctypes.POINTER(_D), ctypes.POINTER(_D),
                         ctypes.POINTER(_INT), ctypes.POINTER(_I64), ctypes.POINTER(_I64)], None),
     ("tsvm_predict", [_P, _P, _P, _I64, _I64, _P, _P], None),
+    ("tsvm_fnorm_res_dev", [_P, _P, _P, _I64, _I64, _D, _P, _I64, ctypes.POINTER(_D),
+                            ctypes.POINTER(_D)], None),
     ("tsvm_grad_dev", [_P, _P, _P, _I64, _I64, _P, _D, _I64, _D, _P, _P, _P, _P, _I64, _P, _P, _P,
                        ctypes.POINTER(GradInfo)], None),
 ]
@@ -353,14 +357,16 @@ class Context:
         return al, xi, s, info
 
     def train(self, X, y, k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32,
-              stop=STOP_REF, tol=1e-5, maxit=100, refine=1, verbose=0, omega=None, init=0):
+              stop=STOP_REF, tol=1e-5, maxit=100, refine=1, verbose=0, omega=None, init=0,
+              analysis=0):
         """Train on X (n x d) with labels +-1; returns (alpha, w (linear) or None, info)."""
         X, y = _f32(X), _f32(y)
         n, d = X.shape
         om = None if omega is None else _f32(omega).reshape(n, k)
         al = np.empty(n)
         w = np.empty(d)
-        prm = train_params(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init)
+        prm = train_params(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init,
+                           analysis)
         info = TrainInfo()
         self._check(self.lib.tsvm_train(self.h, _ptr(X), _ptr(y), n, d, ctypes.byref(prm), _ptr(om),
                                         _ptr(al), _ptr(w), ctypes.byref(info)), "tsvm_train")
@@ -400,6 +406,14 @@ class Context:
             gD.ctypes.data_as(_P) if want_grad else None,
             gL.ctypes.data_as(_P) if want_grad else None, ctypes.byref(info)), "tsvm_grad_dev")
         return vals[0], vals[1], gD, gL, info
+
+    def fnorm_res_dev(self, X, y, n, d, gamma, U, k):
+        """(|K_Y|_F, |K_Y - U U^T|_F) for device X (n x d), y and U (n x k FP32)."""
+        a, b = _D(), _D()
+        self._check(self.lib.tsvm_fnorm_res_dev(self.h, _ptr(X), _ptr(y), n, d, gamma, _ptr(U), k,
+                                                ctypes.byref(a), ctypes.byref(b)),
+                    "tsvm_fnorm_res_dev")
+        return a.value, b.value
 
     def predict(self, model, T):
         """(decision values, predicted labels) of the rows T under a Model."""

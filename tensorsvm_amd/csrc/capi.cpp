@@ -125,6 +125,14 @@ tsvm_status tsvm_grad_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t
                         nt, vals, g_dual, g_loss, info);
 }
 
+tsvm_status tsvm_fnorm_res_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t n,
+                               int64_t d, double gamma, const float* U, int64_t k,
+                               double* fnorm, double* fnorm_res) {
+  if (!ctx || !X || !y || !U || !fnorm || !fnorm_res) return TSVM_ERR_ARG;
+  hipSetDevice(ctx->c.device);
+  return tsvm::fnorm_res_dev(&ctx->c, X, y, n, d, gamma, U, k, k, fnorm, fnorm_res);
+}
+
 tsvm_status tsvm_kernel_product(tsvm_ctx* ctx, const float* X1, const float* y1, int64_t m,
                                 const float* X2, const float* y2, int64_t n, int64_t d,
                                 const float* A, int64_t r, double gamma, int precision,

@@ -90,6 +90,10 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
                              int64_t nh, double* M, hipEvent_t* ev);
 // flops of one BF16 part, SYRK convention (rows streamed x k (k + 1))
 double hybrid_syrk_flops(const HybridSyrk* hs);
+// tsvm_fnorm_res_dev (fnorm.hip): |K_Y|_F, |K_Y - U U^T|_F; U has row stride ldu
+tsvm_status fnorm_res_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d,
+                          double gamma, const float* U, int64_t ldu, int64_t k, double* fnorm,
+                          double* fnorm_res);
 // tsvm_grad_dev (grad.hip)
 tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d,
                      const double* alpha, double b, int64_t k, double log_gamma,

@@ -197,24 +197,31 @@ tsvm_status ipm_gemv_t(Ctx* c, const float* Z, int64_t n, int kp, const double* 
 }
 
 // ------------------------------------------------------------------------------------
-// Z T: L = min(64, kp/4) lanes per row, each lane F = kp/(4L) float4 groups; the lane
-// segment is reduced with xor shuffles.  T columns of a lane stay in registers.
+// Z T: G = kp/4 float4 groups per row, L = the largest power of two <= min(64, G) lanes
+// per row (the lane segment is reduced with xor shuffles), each lane F = ceil(G/L) groups
+// (groups past G contribute zero).  T columns of a lane stay in registers.
 // ------------------------------------------------------------------------------------
 template <int P, int F>
 __global__ __launch_bounds__(256) void gemv_n_kernel(const float* __restrict__ Z, int64_t n, int kp,
                                                      const double* __restrict__ T,
                                                      double* __restrict__ E, int64_t lde) {
-  const int L = (kp / 4 < 64) ? kp / 4 : 64;
+  const int G = kp / 4;
+  int L = 64;
+  while (L > G) L >>= 1;
   const int RW = 64 / L;  // rows per wave per step
   const int lane = threadIdx.x & 63;
   const int seg = lane / L, li = lane % L;
   double t[P][F][4];
+  int grp[F];  // float4 group of each f, clamped to a valid one (its t is zero)
 #pragma unroll
-  for (int q = 0; q < P; q++)
+  for (int f = 0; f < F; f++) {
+    const int g = li + f * L;
+    grp[f] = g < G ? g : 0;
 #pragma unroll
-    for (int f = 0; f < F; f++)
+    for (int q = 0; q < P; q++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) t[q][f][j] = T[q * kp + 4 * (li + f * L) + j];
+      for (int j = 0; j < 4; j++) t[q][f][j] = g < G ? T[q * kp + 4 * g + j] : 0.0;
+  }
   const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 64;
   const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / 64;
   for (int64_t base = wave * RW; base < n; base += nwaves * RW * 2) {
@@ -225,7 +232,7 @@ __global__ __launch_bounds__(256) void gemv_n_kernel(const float* __restrict__ Z
       rows[u] = base + u * nwaves * RW + seg;
       int64_t rr = rows[u] < n ? rows[u] : n - 1;
 #pragma unroll
-      for (int f = 0; f < F; f++) z[u][f] = *(const f32x4_t*)(Z + rr * kp + 4 * (li + f * L));
+      for (int f = 0; f < F; f++) z[u][f] = *(const f32x4_t*)(Z + rr * kp + 4 * grp[f]);
     }
 #pragma unroll
     for (int u = 0; u < 2; u++) {
@@ -253,8 +260,9 @@ tsvm_status ipm_gemv_n(Ctx* c, const float* Z, int64_t n, int kp, const double* 
                        double* E, int64_t lde) {
   if (kp % 32 != 0 || kp > 1024 || p < 1 || p > 2)
     return fail(c, TSVM_ERR_ARG, "gemv_n: kp=%d p=%d unsupported", kp, p);
-  const int L = kp / 4 < 64 ? kp / 4 : 64;
-  const int F = kp / (4 * L);
+  int L = 64;
+  while (L > kp / 4) L >>= 1;
+  const int F = (int)ceil_div(kp / 4, L);  // <= 4 for kp <= 1024
   int64_t blocks = (int64_t)c->cu_count * 8;
   int64_t need = ceil_div(n, 4 * (64 / L) * 2);
   if (blocks > need) blocks = need < 1 ? 1 : need;

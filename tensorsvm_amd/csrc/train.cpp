@@ -58,6 +58,15 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
     printf("real rank is %ld\n", (long)inf.lra.realk);
     printf("\x1b[95mLRA elapsed time = %.3f seconds\x1b[39m\n", inf.lra.t_total);
   }
+  if (prm->analysis) {  // 2328-2360
+    const double ta = now();
+    TSVM_CHECK(fnorm_res_dev(c, X, y, n, d, prm->gamma, U, k, k, &inf.fnorm, &inf.fnorm_res));
+    inf.t_analysis = now() - ta;
+    if (prm->verbose) {
+      printf("fnorm(K)=%.3e fnorm(K-U*U=%.3e')\n", inf.fnorm, inf.fnorm_res);
+      printf("rbf_fnorm_res took %.3f seconds\n", inf.t_analysis);
+    }
+  }
   tsvm_ipm_params ip;
   to_ipm_params(prm, &ip);
   TSVM_CHECK(ipm_dev(c, U, y, n, k, &ip, alpha, (double*)pxi, (double*)ps, &inf.ipm));
@@ -103,7 +112,7 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
   inf.b_std = sqrt(sq / nb);
   inf.b_samples = (int)bs.size();
   inf.t_bias = now() - tb;
-  inf.t_total = now() - t0;
+  inf.t_total = now() - t0 - inf.t_analysis;
   if (prm->verbose) {
     printf("#BSV %ld, #SV %ld\n", (long)inf.nfree, (long)inf.nsv);
     printf("approx mean b=%.6e std b=%.6e, #samples=%d\n ", inf.b, inf.b_std, inf.b_samples);
@@ -131,6 +140,7 @@ void tsvm_train_default_params(tsvm_train_params* p) {
   p->refine = 1;
   p->verbose = 0;
   p->init = 0;
+  p->analysis = 0;
 }
 
 tsvm_status tsvm_train_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t n, int64_t d,

@@ -35,7 +35,7 @@ void help_msg() {
       "-N n_samples : only consider the first n_samples in training\n"
       "-test file : held-out LIBSVM file; prints the prediction accuracy\n"
       "-scale256 : divide every feature value by 256\n"
-      "-analysis : compute fnorm(K-U*U'), the error of the low rank approximation\n"
+      "-analysis : compute fnorm(K-U*U'), the error of the low rank approximation (RBF)\n"
       "-precision fp32|bf16 : arithmetic of the kernel products (default fp32)\n"
       "-tensorcore : same as -precision bf16\n"
       "-stop ref|gap : IPM stop rule (ref: mu,|r_x|,|r_y| < 1e-7; gap: relative gap <= -tol)\n"
@@ -151,6 +151,7 @@ int main(int argc, char* argv[]) {
     return 0;
   }
   std::vector<double> alpha(n), w(d);
+  prm.analysis = analysis && prm.kernel_type == 2;
   tsvm_train_info info;
   const double t0 = now();
   st = tsvm_train(ctx, inst, labels, n, d, &prm, nullptr, alpha.data(), w.data(), &info);
@@ -172,7 +173,6 @@ int main(int argc, char* argv[]) {
     return 1;
   }
   if (v) printf("done\n");
-  if (analysis && v) printf("-analysis: see tsvm_fnorm_res (not run by this command line)\n");
   double acc = -1;
   if (test_path) {
     float *tl = nullptr, *tx = nullptr;

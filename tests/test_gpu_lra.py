@@ -67,3 +67,22 @@ def test_lra_bf16_close_to_fp64(ctx):
     e_same = np.linalg.norm(G - Go) / np.linalg.norm(Go)
     print(f"bf16 form: |K-UU'|/|K| {e_gpu:.3e} (fp64 {e_orc:.3e}), |UU' - UoUo'|/|UoUo'| {e_same:.3e}")
     assert e_same < 1e-3 * e_orc, (e_same, e_orc)
+
+
+@pytest.mark.parametrize("n,d,k,gamma", [(1500, 18, 64, 0.05), (700, 54, 40, 1.0)])
+def test_fnorm_res_matches_oracle(ctx, n, d, k, gamma):
+    """-analysis on the GPU: |K_Y|_F and |K_Y - U U^T|_F against the FP64 oracle on the same U."""
+    import torch
+    import tensorsvm_amd as T
+    X, y = gaussian_mixture(n, d, 13 + n)
+    om = O.gaussrand(n * k).reshape(n, k)
+    U, info = ctx.lra(X, y, k, gamma, T.PREC_FP32, omega=om)
+    fo, ro = O.fnorm_res(X, y, gamma, U.astype(np.float64))
+    dX, dy, dU = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (X, y, U))
+    f, r = ctx.fnorm_res_dev(dX, dy, n, d, gamma, dU, k)
+    assert abs(f - fo) <= 1e-6 * fo
+    assert abs(r - ro) <= 1e-5 * ro + 1e-6 * fo
+    # through the trainer's -analysis flag
+    al, _, ti = ctx.train(X, y, k=k, C=4.0, gamma=gamma, precision=T.PREC_FP32, omega=om, analysis=1)
+    assert abs(ti.fnorm - fo) <= 1e-6 * fo and abs(ti.fnorm_res - ro) <= 1e-5 * ro + 1e-6 * fo
+

@@ -92,10 +92,11 @@ def test_cli_rbf_model_file(tmp_path):
     _write_libsvm(tr, X, y)
     model = str(tmp_path / "out.model")
     exe = os.path.join(ROOT, "tensorsvm_amd", "tensorsvm-train")
-    r = subprocess.run([exe, "-k", "32", "-c", "32", "-g", "0.05", "-t", "2", tr, model],
+    r = subprocess.run([exe, "-k", "32", "-c", "32", "-g", "0.05", "-t", "2", "-analysis", tr, model],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "found labels:" in r.stdout and "#BSV" in r.stdout
+    assert "fnorm(K)=" in r.stdout and "fnorm(K-U*U=" in r.stdout
     head, coef = _read_model(model)
     assert head["svm_type"] == "c_svc" and head["kernel_type"] == "rbf"
     assert head["gamma"] == "0.0500000" and head["nr_class"] == "2"
