@@ -8,12 +8,15 @@
 //                     x^T Q x, sums, max/min D) -- one 256-byte device->host copy
 //   capacitance       M = I + Z^T D^-1 Z: FP64 MFMA from FP32 Z (plain form) or BF16
 //                     MFMA with FP32 accumulation (BF16 form); FP64 Cholesky on the GPU
-//   predictor         SMW solves for [r5, a] batched into one pass pair over Z
-//                     (the reference's third solve is a linear combination: SMW(a dy - r5)
-//                     = dy SMW(a) - SMW(r5)); step bound and mu_aff as device reductions
-//   corrector         one SMW solve (the a-solve is reused), combined step, update
-// In the BF16 form each SMW solve is followed by iterative-refinement sweeps whose
-// residual rhs - (D + Z Z^T) v is formed with the FP32 factor and FP64 accumulation.
+//   predictor         SMW solves for [r5, a] batched into one pass pair over Z; the
+//                     reference's third solve is a linear combination (S^-1 is linear:
+//                     S^-1 (a dy - r5) = dy S^-1 a - S^-1 r5); then a sweep of bordered
+//                     refinement: the residual of the bordered Newton system against the
+//                     exact operator D + Z Z^T (FP32 factor, FP64 accumulation), one SMW
+//                     solve, the correction again combined with S^-1 a; step bound and
+//                     mu_aff as device reductions
+//   corrector         the same with one SMW solve (S^-1 a is reused), combined step, update
+// Per iteration that is 7 pass pairs over Z (Z^T v, then Z t).
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -391,14 +394,17 @@ __global__ __launch_bounds__(256) void k_dots(int64_t n, int p, const double* __
   block_reduce_store<2>(v, op, red);
 }
 
-// right-hand side of the third SMW solve (1815-1816): rhs3 = a dy - r5, with
-// dy = (r2 + a^T r7) / (a^T b), r2 = -a^T x (predictor) or 0 (corrector)
-__global__ void k_rhs3(int64_t n, int corr, const double* __restrict__ a, const double* __restrict__ r5,
-                       double* __restrict__ sc, double* __restrict__ rhs3) {
+// third solve of the Newton step (1815-1816): dx = S^-1 (a dy - r5) with
+// dy = (r2 + a^T r7) / (a^T b), r2 = -a^T x (predictor) or 0 (corrector).  The SMW
+// operator S^-1 = (D + Z Z^T)^-1 (with the capacitance as factored) is linear and
+// r7 = S^-1 r5, b = S^-1 a are already at hand, so dx = dy b - r7 without another solve.
+__global__ void k_sol3_lin(int64_t n, int corr, const double* __restrict__ b,
+                           const double* __restrict__ r7, double* __restrict__ sc,
+                           double* __restrict__ sol3) {
   const double r2 = corr ? 0.0 : -sc[SL_AX];
   const double dy = (r2 + sc[corr ? SL_AR7C : SL_AR7]) / sc[SL_AB];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[corr ? SL_DYC : SL_DY] = dy;
-  GRID_LOOP(i, n) rhs3[i] = a[i] * dy - r5[i];
+  GRID_LOOP(i, n) sol3[i] = dy * b[i] - r7[i];
 }
 
 // Residual of the bordered Newton system (A dx - a dy = -r5, a^T dx = r2) for dx = sol,
@@ -427,28 +433,24 @@ __global__ __launch_bounds__(256) void k_border_res(int64_t n, int corr, const d
   block_reduce_store<3>(v, op, red);
 }
 
-// correction right-hand side: rhs = a ddy - r5', ddy = (r2 - a^T dx + a^T S r5') / a^T b
-__global__ void k_border_rhs(int64_t n, int corr, const double* __restrict__ a,
-                             const double* __restrict__ r5p, double* __restrict__ sc,
-                             double* __restrict__ rhs) {
+// bordered correction: (ddx, ddy) solve the same bordered system for the residual
+// (r5', rho2): ddy = (rho2 + a^T S^-1 r5') / a^T b, ddx = S^-1 (a ddy - r5') = ddy b - s5
+// with s5 = S^-1 r5' and b = S^-1 a (S^-1 linear); dx += ddx, dy += ddy
+__global__ void k_border_lin(int64_t n, int corr, const double* __restrict__ b,
+                             const double* __restrict__ s5, double* __restrict__ sc,
+                             double* __restrict__ sol3) {
   const double r2 = corr ? 0.0 : -sc[SL_AX];
   const double rho2 = r2 - sc[SL_R2B];
   const double ddy = (rho2 + sc[SL_AR7B]) / sc[SL_AB];
-  GRID_LOOP(i, n) rhs[i] = a[i] * ddy - r5p[i];
-  __syncthreads();
+  GRID_LOOP(i, n) sol3[i] += ddy * b[i] - s5[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc[SL_DYB] = ddy;
     sc[corr ? SL_DYC : SL_DY] += ddy;
   }
 }
 
-// dx += ddx
-__global__ void k_axpy1(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
-  GRID_LOOP(i, n) y[i] += x[i];
-}
-
 // Newton step completion (1817-1821) and step bound (1733-1740 / 1762-1768):
-//   ds = (r3 - s dx)/x;  dxi = (r4 + xi dx)/(C-x), dx = (D + Z Z^T)^-1 rhs3
+//   ds = (r3 - s dx)/x;  dxi = (r4 + xi dx)/(C-x), dx from k_sol3_lin
 // corr = 0: predictor, results to (dx, ds, dxi) in place.
 // corr = 1: corrector; the total step aff + corr goes to (dx, ds, dxi), the affine
 //           step is read from (adx, ads, adxi).
@@ -624,7 +626,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     return TSVM_OK;
   };
   double *a, *e, *dx, *ds, *dxi, *adx, *ads, *adxi, *D, *winv, *rhs, *V, *E, *solp, *solc, *T, *M,
-      *sc, *red, *s, *rhs3, *sol3;
+      *sc, *red, *s, *sol3;
   int *dinfo, *tables;
 #define WS(name, bytes, var)              \
   TSVM_CHECK(ws(name, (bytes), &p));      \
@@ -644,7 +646,6 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   WS("ipm_E", 2 * n * 8, E);
   WS("ipm_solp", 2 * n * 8, solp);
   WS("ipm_solc", n * 8, solc);
-  WS("ipm_rhs3", n * 8, rhs3);
   WS("ipm_sol3", n * 8, sol3);
   WS("ipm_T", 4 * kp * 8, T);
   WS("ipm_M", (int64_t)kp * kp * 8, M);
@@ -719,14 +720,12 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
                          (const double*)sol3, (const double*)E, (const double*)D, (const double*)sc,
                          V + n, red);
       TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_R2B, SL_BRES, SL_BRHS}));
-      // r7' = S r5'
+      // s5 = S^-1 r5'; the correction is a combination of s5 and b = S^-1 a
       TSVM_CHECK(smw(V + n, 1, solc + 0));
       hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 1, (const double*)a, (const double*)solc, n, red);
       TSVM_CHECK(reduce({RED_SUM}, {SL_AR7B}));
-      hipLaunchKernelGGL(k_border_rhs, eg, eb, 0, st, n, corr, (const double*)a, (const double*)(V + n),
-                         sc, E + n);
-      TSVM_CHECK(smw(E + n, 1, solc));
-      hipLaunchKernelGGL(k_axpy1, eg, eb, 0, st, n, (const double*)solc, sol3);
+      hipLaunchKernelGGL(k_border_lin, eg, eb, 0, st, n, corr, (const double*)(solp + n),
+                         (const double*)solc, sc, sol3);
     }
     return TSVM_OK;
   };
@@ -862,8 +861,8 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_CHECK(smw(rhs, 2, solp));
     hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 2, (const double*)a, (const double*)solp, n, red);
     TSVM_CHECK(reduce({RED_SUM, RED_SUM}, {SL_AR7, SL_AB}));
-    hipLaunchKernelGGL(k_rhs3, eg, eb, 0, st, n, 0, (const double*)a, (const double*)rhs, sc, rhs3);
-    TSVM_CHECK(smw(rhs3, 1, sol3));
+    hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 0, (const double*)(solp + n), (const double*)solp,
+                       sc, sol3);
     TSVM_CHECK(border_refine(0));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 0, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
@@ -879,8 +878,8 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_CHECK(smw(rhs, 1, solc));
     hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 1, (const double*)a, (const double*)solc, n, red);
     TSVM_CHECK(reduce({RED_SUM}, {SL_AR7C}));
-    hipLaunchKernelGGL(k_rhs3, eg, eb, 0, st, n, 1, (const double*)a, (const double*)rhs, sc, rhs3);
-    TSVM_CHECK(smw(rhs3, 1, sol3));
+    hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 1, (const double*)(solp + n), (const double*)solc,
+                       sc, sol3);
     TSVM_CHECK(border_refine(1));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 1, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,

@@ -8,17 +8,21 @@
 // (small t) goes through a BF16 MFMA SYRK with FP32 accumulation; the rows that carry
 // the large weights (the free support vectors late in the solve) go through the FP64
 // GEMM, gathered by an index list.  The split keeps the BF16 part's error below a
-// budget: each bulk term w_i z_i z_i^T is computed with three bf16 roundings (z twice,
-// w z once), a relative error of about 3 * 2^-9 that is zero-mean and independent
-// across rows, so |E|_2 ~ c 2^-9 sqrt(sum_B t_i^2).  Because M = I + ... has
-// lambda_min(M) >= 1, |M^-1 E| <= |E|, which is the contraction factor of the
-// refinement sweeps that restore the Newton step; sqrt(sum_B t_i^2) <= budget keeps it
-// well below one.  sum_B t_i^2 is bounded from integer counts of t_i per binary
+// budget: each bulk term w_i z_i z_i^T is computed with three bf16 roundings (z once
+// per solve on both sides, w z once), a relative error of about 3 * 2^-9 that is
+// zero-mean and independent across rows, so |E|_2 ~ c 2^-9 sqrt(sum_B t_i^2).  Because
+// M = I + ... has lambda_min(M) >= 1, |M^-1 E| <= |E|, which is the contraction factor
+// of the refinement sweeps that restore the Newton step; sqrt(sum_B t_i^2) <= budget
+// keeps it well below one.  sum_B t_i^2 is bounded from integer counts of t_i per binary
 // exponent (t in [2^e, 2^(e+1)) counts as 4^(e+1)), so the split is deterministic.
 //
-// Data: ZbT (kpt x n_pad bf16) = Z^T rounded once per solve (Z does not change during
-// the interior-point iterations), nz = |z_i|^2 (FP64), per iteration wb (FP32 weights
-// of the bulk rows, 0 for the FP64 rows) and rows (indices of the FP64 rows, in order).
+// Data: ZbT (n_pad x kpt bf16) = Z rounded once per solve (Z does not change during the
+// interior-point iterations), stored in chunks of SY_BK rows, each chunk column-major:
+// element (i, c) sits at ((i / SY_BK) kpt + c) SY_BK + i % SY_BK, so that one stage of a
+// SYRK tile (SY_BK rows x 128 or 256 columns) is one contiguous block of memory and not
+// hundreds of rows a full column length apart (a TLB miss per load); nz = |z_i|^2
+// (FP64); per iteration wb (FP32 weights of the bulk rows, 0 for the FP64 rows) and rows
+// (indices of the FP64 rows, in order).
 #include "common.h"
 #include "internal.h"
 
@@ -35,7 +39,7 @@ __device__ inline int hs_bin(double t) {
   return e < 0 ? 0 : (e > HS_BINS - 1 ? HS_BINS - 1 : e);
 }
 
-// ZbT[c][i] = bf16(Z[i][c]) for c < kp, 0 for kp <= c < kpt or i >= n (32 x 32 tiles)
+// ZbT(i, c) = bf16(Z[i][c]) for c < kp, 0 for kp <= c < kpt or i >= n (32 x 32 tiles)
 __global__ void make_zbt_kernel(const float* __restrict__ Z, int64_t n, int kp, int kpt,
                                 int64_t n_pad, __bf16* __restrict__ ZbT) {
   __shared__ float tile[32][33];
@@ -50,7 +54,7 @@ __global__ void make_zbt_kernel(const float* __restrict__ Z, int64_t n, int kp, 
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
     int c = c0 + r;
-    if (c < kpt) ZbT[(int64_t)c * n_pad + i0 + tx] = (__bf16)tile[tx][r];
+    if (c < kpt) ZbT[((i0 / SY_BK) * kpt + c) * SY_BK + i0 % SY_BK + tx] = (__bf16)tile[tx][r];
   }
 }
 
@@ -181,11 +185,19 @@ __global__ __launch_bounds__(256) void hs_scatter_kernel(const double* __restric
   }
 }
 
-// BF16 SYRK: part[split] (kpt x kpt, lower 128 x 128 tiles) = sum over the split's rows
-// of wb_i z_i z_i^T.  Block = 4 waves, 128 x 128 output tile (wave 64 x 64 = 2 x 2 MFMA
-// 32x32x16 tiles); SY_BK rows per stage, double-buffered LDS with the next stage's
-// global loads in flight; the weights scale the A panel while it is staged.  The tiles
-// of one split sit 8 blocks apart (on one XCD) so that its rows are read once into L2.
+// scales 8 bf16 by 8 FP32 factors, one rounding each
+__device__ inline bf16x8_t scale8(u32x4_t v, const f32x4_t* w) {
+  bf16x8_t a8 = __builtin_bit_cast(bf16x8_t, v);
+#pragma unroll
+  for (int j = 0; j < 8; j++) a8[j] = (__bf16)((float)a8[j] * w[j >> 2][j & 3]);
+  return a8;
+}
+
+// BF16 SYRK for kpt = 128: part[split] (128 x 128) = sum over the split's rows of
+// wb_i z_i z_i^T.  Block = 4 waves (wave 64 x 64 = 2 x 2 MFMA 32x32x16 tiles); SY_BK
+// rows per stage, double-buffered LDS with the next stage's global loads in flight; the
+// weights scale the A panel while it is staged.  The tiles of one split
+// sit 8 blocks apart (on one XCD) so that its rows are read once into L2.
 __global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict__ ZbT,
                                                         int64_t n_pad, int kpt,
                                                         const float* __restrict__ wb, int ntile,
@@ -216,8 +228,8 @@ __global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
-  const __bf16* Apan = ZbT + (int64_t)ta * 128 * n_pad;
-  const __bf16* Bpan = ZbT + (int64_t)tb * 128 * n_pad;
+  const __bf16* Apan = ZbT + (int64_t)ta * 128 * SY_BK;
+  const __bf16* Bpan = ZbT + (int64_t)tb * 128 * SY_BK;
   // staging map: thread -> (panel row = idx >> 3, 8 consecutive K at (idx & 7) * 8) for
   // idx = tid + 256 v; the K offset depends on tid only, so 8 weights cover all four
   const int scol = (tid & 7) * 8;
@@ -228,8 +240,8 @@ __global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int row = (tid + v * 256) >> 3;
-      va[v] = *(const u32x4_t*)(Apan + (int64_t)row * n_pad + n0 + scol);
-      vb[v] = *(const u32x4_t*)(Bpan + (int64_t)row * n_pad + n0 + scol);
+      va[v] = *(const u32x4_t*)(Apan + ((s0 + st) * kpt + row) * SY_BK + scol);
+      vb[v] = *(const u32x4_t*)(Bpan + ((s0 + st) * kpt + row) * SY_BK + scol);
     }
     wv[0] = *(const f32x4_t*)(wb + n0 + scol);
     wv[1] = *(const f32x4_t*)(wb + n0 + scol + 4);
@@ -238,10 +250,7 @@ __global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int row = (tid + v * 256) >> 3;
-      bf16x8_t a8 = __builtin_bit_cast(bf16x8_t, va[v]);
-#pragma unroll
-      for (int j = 0; j < 8; j++) a8[j] = (__bf16)((float)a8[j] * wv[j >> 2][j & 3]);
-      *(bf16x8_t*)&sA[buf][row * SY_LS + scol] = a8;
+      *(bf16x8_t*)&sA[buf][row * SY_LS + scol] = scale8(va[v], wv);
       *(u32x4_t*)&sB[buf][row * SY_LS + scol] = vb[v];
     }
   };
@@ -287,21 +296,177 @@ __global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict
       }
 }
 
-// M (kp x kp FP64) += sum over splits of the BF16 partials, in split order; every upper
-// entry reads its mirrored lower entry (M stays exactly symmetric)
+// BF16 SYRK for kpt a multiple of 256: part[split] (kpt x kpt, lower 256 x 256 tiles) =
+// sum over the split's rows of wb_i z_i z_i^T.  One block per CU; 4 waves, each
+// a 128 x 128 quarter of the tile (4 x 4 MFMA 32x32x16 tiles, 256 accumulators), so that
+// each 16-byte fragment read from LDS feeds 4 MFMAs.  Software pipeline per stage of SB_K
+// rows (4 steps of 16): the fragments of step s + 1 are read while the MFMAs of step s
+// run (one wave per SIMD leaves nothing else to hide LDS latency), the next stage's
+// global loads are issued at the top of the stage and staged into the other LDS buffer
+// (the A panel scaled by the weights) during the last two steps, one barrier per stage.
+// On a diagonal tile one panel is loaded (staged scaled and as is) and the result of the
+// wave above the diagonal is not stored.
+constexpr int SB_K = SY_BK;      // the chunk of the ZbT layout
+constexpr int SB_LS = SB_K + 8;  // LDS row stride (bf16): conflict-free 16-byte reads
+
+template <bool DIAG>
+__device__ __forceinline__ void syrk256_stages(const __bf16* __restrict__ Apan,
+                                               const __bf16* __restrict__ Bpan,
+                                               const float* __restrict__ wb, int kpt, int64_t s0,
+                                               int64_t nst, __bf16 (*sm)[256 * SB_LS],
+                                               f32x16_t (&acc)[4][4], int tid, int wa, int wbo,
+                                               int l32, int h) {
+  constexpr int NP = DIAG ? 1 : 2;  // panels loaded
+  const int scol = (tid & 7) * 8;   // staging map: panel rows (tid >> 3) + 32 q, 8 K at scol
+  const int srow = tid >> 3;
+  u32x4_t v[NP][8];
+  f32x4_t wv[2];
+  auto gload = [&](int64_t st) {
+    const int64_t base = ((s0 + st) * kpt + srow) * SB_K + scol;
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        v[p][q] = *(const u32x4_t*)((p ? Bpan : Apan) + base + (int64_t)32 * q * SB_K);
+    wv[0] = *(const f32x4_t*)(wb + (s0 + st) * SB_K + scol);
+    wv[1] = *(const f32x4_t*)(wb + (s0 + st) * SB_K + scol + 4);
+  };
+  // half hf of the staging: panel rows (tid >> 3) + 32 q for q in [4 hf, 4 hf + 4);
+  // A scaled, B as loaded
+  auto sstore = [&](int buf, int hf) {
+#pragma unroll
+    for (int q = 4 * hf; q < 4 * hf + 4; q++) {
+      *(bf16x8_t*)&sm[2 * buf][(srow + 32 * q) * SB_LS + scol] = scale8(v[0][q], wv);
+      *(u32x4_t*)&sm[2 * buf + 1][(srow + 32 * q) * SB_LS + scol] = v[NP - 1][q];
+    }
+  };
+  auto fread = [&](bf16x8_t(&fa)[4], bf16x8_t(&fb)[4], int buf, int ks) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      fa[i] = *(const bf16x8_t*)&sm[2 * buf][(wa + 32 * i + l32) * SB_LS + ks * 16 + 8 * h];
+      fb[i] = *(const bf16x8_t*)&sm[2 * buf + 1][(wbo + 32 * i + l32) * SB_LS + ks * 16 + 8 * h];
+    }
+  };
+  auto mm = [&](const bf16x8_t(&fa)[4], const bf16x8_t(&fb)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+  };
+  if (nst <= 0) return;
+  gload(0);
+  sstore(0, 0);
+  sstore(0, 1);
+  __syncthreads();
+  bf16x8_t fa0[4], fb0[4], fa1[4], fb1[4];
+  // Phases are fenced with sched_barrier.  Inside one: the next step's 8 fragment reads
+  // first, then the 16 MFMAs; in the last two, the staging VALU work is spread between
+  // the MFMAs (5 per MFMA) and its LDS writes come last.
+  for (int64_t st = 0; st < nst; st++) {
+    const int buf = (int)(st & 1);
+    fread(fa0, fb0, buf, 0);
+    // the last stage reloads itself into the unused buffer: no branch in the body
+    gload(st + 1 < nst ? st + 1 : st);
+    __builtin_amdgcn_sched_barrier(0);
+    fread(fa1, fb1, buf, 1);
+    mm(fa0, fb0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    fread(fa0, fb0, buf, 2);
+    mm(fa1, fb1);
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    fread(fa1, fb1, buf, 3);
+    sstore(buf ^ 1, 0);  // buf ^ 1 was last read before the previous barrier
+    mm(fa0, fb0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    sstore(buf ^ 1, 1);
+    mm(fa1, fb1);
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256, 1) void syrk_bf16_256_kernel(const __bf16* __restrict__ ZbT,
+                                                               int64_t n_pad, int kpt,
+                                                               const float* __restrict__ wb,
+                                                               int ntile, int nsplit,
+                                                               int64_t stages_per_split,
+                                                               float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 sm[4][256 * SB_LS];  // [buf * 2 + panel]
+  const int nlow = ntile * (ntile + 1) / 2;
+  const int bid = blockIdx.x;
+  const int xcd = bid % 8, grp = bid / 8;
+  const int tile = grp % nlow;
+  const int split = (grp / nlow) * 8 + xcd;
+  if (split >= nsplit) return;
+  int ta = 0;
+  while ((ta + 1) * (ta + 2) / 2 <= tile) ta++;
+  const int tb = tile - ta * (ta + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, h = lane >> 5;
+  const int wa = (wave >> 1) * 128, wbo = (wave & 1) * 128;
+  const int64_t s0 = (int64_t)split * stages_per_split;
+  const int64_t nst_all = n_pad / SB_K;
+  const int64_t s1 = s0 + stages_per_split < nst_all ? s0 + stages_per_split : nst_all;
+  const int64_t nst = s1 > s0 ? s1 - s0 : 0;
+  f32x16_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
+  const __bf16* Apan = ZbT + (int64_t)ta * 256 * SB_K;
+  const __bf16* Bpan = ZbT + (int64_t)tb * 256 * SB_K;
+  if (ta == tb)
+    syrk256_stages<true>(Apan, Bpan, wb, kpt, s0, nst, sm, acc, tid, wa, wbo, l32, h);
+  else
+    syrk256_stages<false>(Apan, Bpan, wb, kpt, s0, nst, sm, acc, tid, wa, wbo, l32, h);
+  if (ta == tb && wbo > wa) return;  // strictly above the diagonal
+  // C layout of the 32x32 MFMA: register q holds row 8(q/4) + 4h + q%4, column l32
+  float* P = part + (int64_t)split * kpt * kpt;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int a = ta * 256 + wa + 32 * i + 8 * (q / 4) + 4 * h + (q % 4);
+        const int bcol = tb * 256 + wbo + 32 * j + l32;
+        P[(int64_t)a * kpt + bcol] = acc[i][j][q];
+      }
+}
+
+// M (kp x kp FP64) += sum over splits of the BF16 partials, in split order.  Threads
+// read lower entries (a >= b, coalesced along b) and add each sum to both mirrored
+// entries, so M stays exactly symmetric.
 __global__ void syrk_accum_kernel(const float* __restrict__ part, int nsplit, int kpt, int kp,
                                   double* __restrict__ M) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= (int64_t)kp * kp) return;
-  const int a = (int)(idx / kp), bb = (int)(idx % kp);
-  int ra = a, rb = bb;
-  if (rb > ra) {
-    ra = bb;
-    rb = a;
-  }
+  const int a = (int)(idx / kp), b = (int)(idx % kp);
+  if (b > a) return;
   double s = 0.0;
-  for (int t = 0; t < nsplit; t++) s += (double)part[(int64_t)t * kpt * kpt + (int64_t)ra * kpt + rb];
+  for (int t = 0; t < nsplit; t++) s += (double)part[(int64_t)t * kpt * kpt + (int64_t)a * kpt + b];
   M[idx] += s;
+  if (a != b) M[(int64_t)b * kp + a] += s;
 }
 
 __global__ void add_identity_kernel(double* __restrict__ M, int64_t k) {
@@ -314,7 +479,7 @@ __global__ void add_identity_kernel(double* __restrict__ M, int64_t k) {
 tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridSyrk* hs) {
   hs->n = n;
   hs->kp = kp;
-  hs->kpt = (int)round_up(kp, 128);
+  hs->kpt = kp <= 128 ? 128 : (int)round_up(kp, 256);
   hs->n_pad = round_up(n > 0 ? n : 1, SY_BK);
   void* p;
   TSVM_HIP(c, c->ws.get("hs_ZbT", (size_t)hs->kpt * hs->n_pad * 2, &p));
@@ -373,18 +538,26 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
                      (int64_t)kp);
   if (ev_bf16) TSVM_HIP(c, hipEventRecord(ev_bf16[0], s));
   if (nh < n) {
-    const int ntile = kpt / 128;
+    // splits: as many per XCD as fit its CUs with every tile of a split resident at once
+    // (the 128 kernel runs two blocks per CU, the 256 kernel one)
+    const bool big = kpt >= 256;
+    const int ntile = kpt / (big ? 256 : 128);
     const int nlow = ntile * (ntile + 1) / 2;
     const int64_t nst = hs->n_pad / SY_BK;
-    int nsplit = (int)ceil_div((int64_t)c->cu_count * 2, nlow);
-    nsplit = (int)round_up(nsplit, 8);
-    if (nsplit > nst) nsplit = (int)round_up(nst, 8);
+    const int slots_xcd = (c->cu_count / 8) * (big ? 1 : 2);
+    int nsplit = 8 * (slots_xcd / nlow > 1 ? slots_xcd / nlow : 1);
+    if (nsplit > round_up(nst, 8)) nsplit = (int)round_up(nst, 8);
     const int64_t sps = ceil_div(nst, nsplit);
     void* part;
     TSVM_HIP(c, c->ws.get("hs_part", (size_t)nsplit * kpt * kpt * 4, &part));
-    hipLaunchKernelGGL(syrk_bf16_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
-                       (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile, nsplit,
-                       sps, (float*)part);
+    if (big)
+      hipLaunchKernelGGL(syrk_bf16_256_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
+                         (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile,
+                         nsplit, sps, (float*)part);
+    else
+      hipLaunchKernelGGL(syrk_bf16_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
+                         (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile,
+                         nsplit, sps, (float*)part);
     hipLaunchKernelGGL(syrk_accum_kernel, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)), dim3(256),
                        0, s, (const float*)part, nsplit, kpt, kp, M);
   }
