@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-rows", type=int, default=384)
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiation form")
+    ap.add_argument("--precision", choices=("bf16", "fp32"), default="bf16",
+                    help="form of the timed steps (the comparison run, if any, is FP32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trainer is "
@@ -168,44 +170,50 @@ def main():
         f = ctx.decision_function(X, y, alpha, gamma, info.b, Xte)
         return float(np.mean(np.sign(f) == yte)), alpha
 
+    prec = T.PREC_BF16 if args.precision == "bf16" else T.PREC_FP32
     for w in range(args.warmup):
-        dt, _ = run(T.PREC_BF16)
+        dt, _ = run(prec)
         log(f"warmup {w}: {dt:.2f}s")
     times, info = [], None
     for s in range(args.steps):
-        dt, info = run(T.PREC_BF16)
+        dt, info = run(prec)
         times.append(dt)
-        log(f"step {s}: bf16 train {dt:.2f}s (lra {info.lra.t_total:.2f}s, ipm {info.ipm.t_total:.2f}s, "
+        log(f"step {s}: {args.precision} train {dt:.2f}s (lra {info.lra.t_total:.2f}s, ipm {info.ipm.t_total:.2f}s, "
             f"{info.ipm.iters} it, reason {info.ipm.reason}, gap {info.ipm.gap_rel:.2e})")
     acc_bf16, alpha_bf16 = accuracy(info)
     bf16_info = info.as_dict()
     value = float(np.mean(times))
 
     fp32 = None
-    if not args.no_fp32:
+    if not args.no_fp32 and args.precision == "bf16":
         dt32, info32 = run(T.PREC_FP32)
         acc32, alpha32 = accuracy(info32)
         fp32 = {"train_s": dt32, "accuracy": acc32, "info": info32.as_dict(),
                 "alpha_max_abs_diff_vs_bf16": float(np.abs(alpha32 - alpha_bf16).max())}
         log(f"fp32 train {dt32:.2f}s (lra {info32.lra.t_total:.2f}s), accuracy {acc32:.4f}")
 
-    # roofline of the dominant kernel: the sketch product (first product of the last BF16
-    # step); the second product (plain fp16) is listed beside it
+    # roofline of the dominant kernel: the sketch product (first product of the last
+    # step); the second product is listed beside it
     flops = 2.0 * ntr * ntr * (d + k)
     t_kp = bf16_info["lra"]["t_product1"]
     achieved = flops / t_kp / 1e12
     tr = KPROD_TRAFFIC
-    roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_BF16, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_BF16, 4),
+    if args.precision == "bf16":
+        peak, k1, k2 = PEAK_BF16, "kprod_bf16<8,64,1,EM_F16X2> (fused RBF product, sketch K_Y Omega)", \
+            "kprod_bf16<4,64,2,EM_F16> (K_Y W)"
+    else:
+        peak, k1, k2 = PEAK_FP32, "kprod_f32 (fused RBF product, K_Y Omega)", "kprod_f32 (K_Y W)"
+    same_config = args.n == 2_000_000 and k == 512 and args.precision == "bf16"
+    roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4),
                 "traffic": (tr["fetch_bytes"] + tr["write_bytes"]
-                            if tr["fetch_bytes"] is not None else None),
-                "kernel": "kprod_bf16<8,64,1,EM_F16X2> (fused RBF product, sketch K_Y Omega)",
-                "flops_per_launch": flops, "launch_s": round(t_kp, 4),
-                "traffic_source": tr["source"],
+                            if tr["fetch_bytes"] is not None and same_config else None),
+                "kernel": k1, "flops_per_launch": flops, "launch_s": round(t_kp, 4),
+                "traffic_source": tr["source"] if same_config else None,
                 "second_product": {
-                    "kernel": "kprod_bf16<4,64,2,EM_F16> (K_Y W)",
+                    "kernel": k2,
                     "achieved": round(flops / bf16_info["lra"]["t_product2"] / 1e12, 1),
-                    "frac": round(flops / bf16_info["lra"]["t_product2"] / 1e12 / PEAK_BF16, 4)}}
+                    "frac": round(flops / bf16_info["lra"]["t_product2"] / 1e12 / peak, 4)}}
 
     grad = None if args.no_grad else grad_form(ctx, torch)
 
@@ -227,13 +235,14 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(value * 1000, 1), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
-        "config": {"workload": "gaussian-mixture n=2M (1.6M train / 0.4M held-out), d=18, k=512, "
+        "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": f"gaussian-mixture n={args.n / 1e6:g}M ({ntr / 1e6:g}M train / "
+                               f"{(args.n - ntr) / 1e6:g}M held-out), d=18, k={k}, "
                                "gamma=0.05, C=32, stop: relative gap 1e-5",
                    "n_train": ntr, "d": d, "k": k, "gamma": gamma, "C": C, "seed": seed},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "bf16": {"train_s": value, "accuracy": acc_bf16, "info": bf16_info},
+        args.precision: {"train_s": value, "accuracy": acc_bf16, "info": bf16_info},
         "fp32": fp32,
         "speedup_bf16_vs_fp32": (round(fp32["train_s"] / value, 2) if fp32 else None),
         "omega_gen_s": round(t_omega, 2),

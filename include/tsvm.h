@@ -283,6 +283,12 @@ tsvm_status tsvm_decision_function(tsvm_ctx *ctx, const float *X, const float *y
 tsvm_status tsvm_read_libsvm(const char *path, int scale256, float **labels, float **inst,
                              int64_t *n, int64_t *d);
 void tsvm_free(void *p);
+/* LIBSVM text output of a dense data set (no counterpart in the reference: the writer of
+ * generated data sets for the command-line path).  One line per row: the label, then
+ * "j:v" for every non-zero feature (1-based), values as "%.9g", which tsvm_read_libsvm
+ * reads back to the same floats.  Rows are formatted on all host threads. */
+tsvm_status tsvm_write_libsvm(const char *path, const float *labels, const float *inst,
+                              int64_t n, int64_t d);
 /* Label mapping of main (879-896): first label -> +1 (POS), first different -> -1 (NEG).
  * Returns the count of labels outside {POS, NEG} (left unchanged). */
 int64_t tsvm_map_labels(float *labels, int64_t n, int *pos, int *neg);

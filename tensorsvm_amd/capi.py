@@ -157,6 +157,7 @@ _EXTRA = [
     ("tsvm_decision_function", [_P, _P, _P, _P, _I64, _I64, _D, _D, _P, _I64, _P], None),
     ("tsvm_read_libsvm", [ctypes.c_char_p, _INT, ctypes.POINTER(_P), ctypes.POINTER(_P),
                           ctypes.POINTER(_I64), ctypes.POINTER(_I64)], None),
+    ("tsvm_write_libsvm", [ctypes.c_char_p, _P, _P, _I64, _I64], None),
     ("tsvm_free", [_P], ctypes.c_int),
     ("tsvm_map_labels", [_P, _I64, ctypes.POINTER(_INT), ctypes.POINTER(_INT)], _I64),
     ("tsvm_write_model_rbf", [ctypes.c_char_p, _P, _P, _P, _I64, _I64, _D, _D, _D, _INT, _INT], None),
@@ -212,6 +213,18 @@ class Model:
             self.close()
         except Exception:
             pass
+
+
+def write_libsvm(path, labels, x):
+    """Dense (labels (n,), features (n, d)) -> LIBSVM text (non-zero features, "%.9g")."""
+    lab = np.ascontiguousarray(labels, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    n, d = xx.shape
+    if lab.shape != (n,):
+        raise ValueError("labels must have one entry per row")
+    st = load().tsvm_write_libsvm(str(path).encode(), _ptr(lab), _ptr(xx), n, d)
+    if st != 0:
+        raise TsvmError(f"tsvm_write_libsvm({path}) failed with status {st}")
 
 
 def read_libsvm(path, scale256=False):

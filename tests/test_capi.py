@@ -64,6 +64,40 @@ def test_reader_matches_oracle_and_golden(tmp_path):
     assert np.array_equal(x3, x4)
 
 
+def test_reader_many_chunks_and_writer_round_trip(tmp_path):
+    """A file of several MiB is parsed by several threads (chunks cut at line ends); the
+    rows, labels and index range match the oracle's single-pass reader.  The writer's text
+    reads back to the same floats."""
+    import tensorsvm_amd as T
+    rng = np.random.default_rng(5)
+    p = tmp_path / "big.libsvm"
+    seps = [" ", "\t", "  ", " \t"]
+    with open(p, "w") as f:
+        for i in range(60000):
+            idx = np.sort(rng.choice(60, rng.integers(1, 16), replace=False)) + 1
+            vals = rng.normal(size=idx.size) * 10.0 ** rng.integers(-6, 6, idx.size)
+            sp = seps[i % 4]
+            end = "\r\n" if i % 7 == 0 else (" \n" if i % 5 == 0 else "\n")
+            f.write(("+1" if i % 3 else "-1") + sp +
+                     sp.join(f"{a}:{b:.8g}" for a, b in zip(idx, vals)) + end)
+    assert p.stat().st_size > 4 << 20
+    l1, x1 = T.capi.read_libsvm(str(p))
+    l2, x2 = O.libsvm_read(str(p))
+    assert x1.shape == x2.shape == (60000, 60)
+    assert np.array_equal(l1, l2) and np.array_equal(x1.view(np.uint32), x2.view(np.uint32))
+    # writer -> reader: exact floats, zeros dropped, the width set by the last non-zero column
+    x = (rng.standard_normal((3001, 17)) * 10.0 ** rng.integers(-30, 30, (3001, 17))).astype(np.float32)
+    x[rng.random(x.shape) < 0.3] = 0.0
+    x[:, -1] = 1.5  # keeps d = 17
+    lab = np.where(rng.random(3001) < 0.5, 1.0, -1.0).astype(np.float32)
+    q = tmp_path / "w.libsvm"
+    T.capi.write_libsvm(q, lab, x)
+    l3, x3 = T.capi.read_libsvm(str(q))
+    assert np.array_equal(l3, lab) and np.array_equal(x3.view(np.uint32), x.view(np.uint32))
+    l4, x4 = O.libsvm_read(str(q))
+    assert np.array_equal(x4, x3)
+
+
 def test_reader_rejects_bad_input(tmp_path):
     import tensorsvm_amd as T
     for bad in ("+1 2:1 1:3\n", "+1 a:1\n", "x 1:2\n", "+1 1:2\n\n"):
