@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-rows", type=int, default=384)
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiation form")
+    ap.add_argument("--progress", action="store_true",
+                    help="progress lines on stderr from the library (row slabs of each product)")
     ap.add_argument("--precision", choices=("bf16", "fp32"), default="bf16",
                     help="form of the timed steps (the comparison run, if any, is FP32)")
     args = ap.parse_args()
@@ -159,7 +161,8 @@ def main():
         dOm.copy_(dOm0)  # the factor construction consumes Omega
         torch.cuda.synchronize()
         prm = T.capi.train_params(k=k, C=C, gamma=gamma, kernel_type=2, precision=precision,
-                                  stop=T.capi.STOP_GAP, tol=1e-5, maxit=300, refine=1)
+                                  stop=T.capi.STOP_GAP, tol=1e-5, maxit=300, refine=1,
+                                  progress=int(args.progress))
         t0 = time.perf_counter()
         info = ctx.train_dev(dX, dy, ntr, d, prm, dOm, dal)
         torch.cuda.synchronize()

@@ -241,6 +241,9 @@ typedef struct {
   int verbose;        /* reference-style progress lines on stdout */
   int init;           /* IPM initial point, as tsvm_ipm_params.init */
   int analysis;       /* -analysis: |K_Y|_F and |K_Y - U U^T|_F after the factor (RBF) */
+  int progress;       /* progress lines on stderr: the kernel products run in row slabs,
+                         one line per slab, and one line per 10 IPM iterations (results
+                         are the same; meant for runs of many minutes) */
 } tsvm_train_params;
 typedef struct {
   tsvm_lra_info lra;

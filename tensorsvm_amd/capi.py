@@ -128,7 +128,7 @@ def ipm_params(C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5, precision=P
 class TrainParams(ctypes.Structure):
     _fields_ = [("k", _I64), ("C", _D), ("gamma", _D), ("kernel_type", _INT), ("precision", _INT),
                 ("stop_mode", _INT), ("tol", _D), ("maxit", _INT), ("refine", _INT),
-                ("verbose", _INT), ("init", _INT), ("analysis", _INT)]
+                ("verbose", _INT), ("init", _INT), ("analysis", _INT), ("progress", _INT)]
 
 
 class TrainInfo(ctypes.Structure):
@@ -144,9 +144,9 @@ class TrainInfo(ctypes.Structure):
 
 
 def train_params(k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32, stop=STOP_REF,
-                 tol=1e-5, maxit=100, refine=1, verbose=0, init=0, analysis=0):
+                 tol=1e-5, maxit=100, refine=1, verbose=0, init=0, analysis=0, progress=0):
     return TrainParams(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init,
-                       analysis)
+                       analysis, progress)
 
 
 _EXTRA = [

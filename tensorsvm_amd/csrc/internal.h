@@ -27,6 +27,7 @@ struct Ctx {
   void* rocblas = nullptr;  // rocblas_handle for the k x k eigen-solver, created on first use
   // phase timers of the last tsvm_lra / tsvm_ipm call (seconds)
   double t_phase[16] = {0};
+  int progress = 0;  // progress lines on stderr during a training call
 };
 
 // Record a failure on the context and return its status.

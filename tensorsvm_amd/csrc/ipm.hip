@@ -816,6 +816,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
       printf("iter %d, mu=%.3e, normdx=%.3e, normdy=%.3e max/min(D)=%.3e pobj=%.9e dobj=%.9e%s\n", iter,
              mu, normdx, normdy, cond, pobj, dobj,
              prm->verbose > 1 ? (std::string(" newton-res=") + std::to_string(inf.refine_res) + " ay-res=" + std::to_string(hsc[SL_R2B])).c_str() : "");
+    if (c->progress && iter % 10 == 0) {
+      fprintf(stderr, "ipm: iteration %d, relative gap %.3e (%.1f s)\n", iter, gap, now() - t_start);
+      fflush(stderr);
+    }
     bool conv;
     if (prm->stop_mode == TSVM_STOP_REF) {
       conv = mu < 1e-7 && normdx < 1e-7 && normdy < 1e-7;

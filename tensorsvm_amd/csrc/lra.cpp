@@ -25,6 +25,7 @@
 //   7. U = W X                            FP64 MFMA, FP32 output (the reference's U = W X)
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <chrono>
@@ -98,6 +99,32 @@ tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
   return TSVM_OK;
 }
 
+// W = K_Y A over all n rows.  With progress lines on, the rows go in slabs of about
+// 2e15 flops (multiples of 256 rows), one stderr line per slab; each row's result does
+// not depend on the slabbing.
+static tsvm_status product_rows(Ctx* c, const char* name, const float* X, const float* y,
+                                int64_t n, int64_t d, const float* A, int64_t k, double gamma,
+                                int prec, float* W) {
+  int64_t slabs = 1;
+  if (c->progress) {
+    slabs = (int64_t)ceil(2.0 * (double)n * (double)n * (double)(d + k) / 2e15);
+    if (slabs < 1) slabs = 1;
+  }
+  const int64_t rows = ((n + slabs - 1) / slabs + 255) / 256 * 256;
+  const double t0 = now();
+  for (int64_t r0 = 0; r0 < n; r0 += rows) {
+    const int64_t m = rows < n - r0 ? rows : n - r0;
+    TSVM_CHECK(kernel_product_dev(c, X + r0 * d, y + r0, m, X, y, n, d, A, k, k, gamma, prec,
+                                  W + r0 * k, k));
+    if (c->progress) {
+      TSVM_HIP(c, hipStreamSynchronize(c->stream));
+      fprintf(stderr, "%s: rows %ld / %ld (%.1f s)\n", name, (long)(r0 + m), (long)n, now() - t0);
+      fflush(stderr);
+    }
+  }
+  return TSVM_OK;
+}
+
 tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d, int64_t k,
                     double gamma, int precision, float* Om, float* U, tsvm_lra_info* info) {
   if (n <= 0 || d <= 0 || k <= 0 || k > n)
@@ -123,7 +150,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   TSVM_HIP(c, hipEventCreate(&e1));
   TSVM_HIP(c, hipEventRecord(e0, s));
   const int sketch_prec = precision == TSVM_PREC_BF16 ? TSVM_PREC_FP16X2 : precision;
-  TSVM_CHECK(kernel_product_dev(c, X, y, n, X, y, n, d, Om, k, k, gamma, sketch_prec, W, k));
+  TSVM_CHECK(product_rows(c, "factor: K_Y Omega", X, y, n, d, Om, k, gamma, sketch_prec, W));
   TSVM_HIP(c, hipEventRecord(e1, s));
   TSVM_HIP(c, hipStreamSynchronize(s));
   float ms = 0.f;
@@ -141,7 +168,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   float* Q = Om;
   TSVM_HIP(c, hipEventRecord(e0, s));
   const int q_prec = precision == TSVM_PREC_BF16 ? TSVM_PREC_FP16 : precision;
-  TSVM_CHECK(kernel_product_dev(c, X, y, n, X, y, n, d, W, k, k, gamma, q_prec, Q, k));
+  TSVM_CHECK(product_rows(c, "factor: K_Y W", X, y, n, d, W, k, gamma, q_prec, Q));
   TSVM_HIP(c, hipEventRecord(e1, s));
   TSVM_HIP(c, hipStreamSynchronize(s));
   TSVM_HIP(c, hipEventElapsedTime(&ms, e0, e1));

@@ -43,6 +43,11 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
   const double t0 = now();
   const int64_t k = prm->k;
   if (k <= 0 || k > n) return fail(c, TSVM_ERR_ARG, "train: need 0 < k <= n");
+  struct ProgressScope {  // progress lines for the duration of this call
+    Ctx* c;
+    ProgressScope(Ctx* cc, int p) : c(cc) { c->progress = p; }
+    ~ProgressScope() { c->progress = 0; }
+  } progress_scope(c, prm->progress);
   tsvm_train_info inf;
   memset(&inf, 0, sizeof(inf));
   void *pU, *pxi, *ps;
@@ -141,6 +146,7 @@ void tsvm_train_default_params(tsvm_train_params* p) {
   p->verbose = 0;
   p->init = 0;
   p->analysis = 0;
+  p->progress = 0;
 }
 
 tsvm_status tsvm_train_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t n, int64_t d,
