@@ -10,6 +10,8 @@
 //
 // gemm_tn_f64 splits the n rows over a third grid dimension; each split writes its
 // partial tile and a second kernel sums the splits in a fixed order (deterministic).
+#include <type_traits>
+
 #include "common.h"
 #include "internal.h"
 
@@ -17,14 +19,21 @@ namespace tsvm {
 
 // 256 threads; the block owns a 64 x 64 output tile, wave w a 32 x 32 quarter
 // (2 x 2 MFMA tiles of 16 x 16).  Rows are staged 32 at a time through LDS as FP64
-// (the A panel already multiplied by the row weight), double-buffered: the next stage's
-// global loads are in flight while the MFMAs consume the current one.
-constexpr int TN_R = 32;  // rows per stage
+// (the A panel already multiplied by the row weight), double-buffered in LDS, with the
+// global loads two stages ahead in a register ring (a stage is 2048 cycles of FP64 MFMA
+// per wave, less than a loaded memory latency).  LDS rows are 64 doubles (a multiple of
+// the bank count), so odd rows store their columns XOR 16: the four row groups of a
+// fragment read (lanes 16 q .. 16 q + 15 read row 4 s + q) then fall on disjoint banks
+// within each half-wave instead of all on the same 32.
+constexpr int TN_R = 16;  // rows per stage
+constexpr int TN_H = TN_R / 16;  // staging passes of 16 rows
+__device__ inline int tn_sw(int row, int col) { return col ^ ((row & 1) << 4); }
 
+template <bool VEC, bool HAS_W>
 __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     const float* __restrict__ A, int64_t lda, int64_t ka, const float* __restrict__ B,
     int64_t ldb, int64_t kb, int64_t n, const double* __restrict__ w, int lower_only,
-    int64_t rows_per_split, const int* __restrict__ rows, double* __restrict__ part) {
+    int64_t rows_per_split, double* __restrict__ part) {
   const int ta = blockIdx.x, tb = blockIdx.y;
   if (lower_only && tb > ta) return;
   __shared__ double sA[2][TN_R][64];
@@ -40,72 +49,92 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
   const int64_t nst = (r1 - r0 + TN_R - 1) / TN_R;
   // staging map: thread -> (row, 4 columns) for two passes of 16 rows
   const int sr = tid >> 4, sc4 = (tid & 15) * 4;
-  const bool avec = (lda % 4 == 0) && (a0 + 64 <= ka);
-  const bool bvec = (ldb % 4 == 0) && (b0 + 64 <= kb);
-  f32x4_t ra[2], rb[2];
-  double rw[2];
-  auto gload = [&](int64_t st) {
+  f32x4_t ra[2][TN_H], rb[2][TN_H];  // [ring slot][h]
+  double rw[2][TN_H];
+  bool rok[2][TN_H];
+  // rows past the split load a valid row and are zeroed when staged: no branches, and
+  // nothing waits on the loads before the data is needed
+  auto gload = [&](int64_t st, int slot) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      int64_t row = r0 + st * TN_R + sr + 16 * h;
-      bool ok = row < r1;
-      int64_t rr = ok ? row : r0;
-      if (rows) rr = rows[rr];
-      rw[h] = ok ? (w ? w[rr] : 1.0) : 0.0;
+    for (int h = 0; h < TN_H; h++) {
+      const int64_t row = r0 + st * TN_R + sr + 16 * h;
+      const bool ok = row < r1;
+      const int64_t rr = ok ? row : r0;
+      rok[slot][h] = ok;
+      rw[slot][h] = HAS_W ? w[rr] : 1.0;
       const float* pa = A + rr * lda + a0 + sc4;
       const float* pb = B + rr * ldb + b0 + sc4;
-      if (avec) {
-        ra[h] = *(const f32x4_t*)pa;
+      f32x4_t va, vb;
+      if (VEC) {
+        va = *(const f32x4_t*)pa;
+        vb = *(const f32x4_t*)pb;
       } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) ra[h][j] = (a0 + sc4 + j < ka) ? pa[j] : 0.0f;
+        for (int j = 0; j < 4; j++) {
+          const bool ca = a0 + sc4 + j < ka, cb = b0 + sc4 + j < kb;
+          const float xa = A[rr * lda + (ca ? a0 + sc4 + j : 0)];
+          const float xb = B[rr * ldb + (cb ? b0 + sc4 + j : 0)];
+          va[j] = ca ? xa : 0.0f;
+          vb[j] = cb ? xb : 0.0f;
+        }
       }
-      if (bvec) {
-        rb[h] = *(const f32x4_t*)pb;
-      } else {
-#pragma unroll
-        for (int j = 0; j < 4; j++) rb[h][j] = (b0 + sc4 + j < kb) ? pb[j] : 0.0f;
-      }
-      if (!ok) rb[h] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+      ra[slot][h] = va;
+      rb[slot][h] = vb;
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, int slot) {
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+    for (int h = 0; h < TN_H; h++) {
+      const int r = sr + 16 * h;
+      const double wv = rok[slot][h] ? rw[slot][h] : 0.0;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        sA[buf][sr + 16 * h][sc4 + j] = rw[h] * (double)ra[h][j];
-        sB[buf][sr + 16 * h][sc4 + j] = (double)rb[h][j];
+        sA[buf][r][tn_sw(r, sc4 + j)] = wv * (double)ra[slot][h][j];
+        sB[buf][r][tn_sw(r, sc4 + j)] = rok[slot][h] ? (double)rb[slot][h][j] : 0.0;
       }
+    }
   };
   f64x4_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = (f64x4_t){0, 0, 0, 0};
-  if (nst > 0) {
-    gload(0);
-    sstore(0);
-    __syncthreads();
-  }
-  for (int64_t st = 0; st < nst; st++) {
-    const int buf = (int)(st & 1);
-    gload(st + 1 < nst ? st + 1 : st);
-    __builtin_amdgcn_sched_barrier(0);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int s4 = 0; s4 < TN_R / 4; s4++) {
       const int row = 4 * s4 + kq;
-      double av0 = sA[buf][row][wa + l16], av1 = sA[buf][row][wa + 16 + l16];
-      double bv0 = sB[buf][row][wb + l16], bv1 = sB[buf][row][wb + 16 + l16];
+      double av0 = sA[buf][row][tn_sw(row, wa + l16)], av1 = sA[buf][row][tn_sw(row, wa + 16 + l16)];
+      double bv0 = sB[buf][row][tn_sw(row, wb + l16)], bv1 = sB[buf][row][tn_sw(row, wb + 16 + l16)];
       acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv0, acc[0][0], 0, 0, 0);
       acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv1, acc[0][1], 0, 0, 0);
       acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv0, acc[1][0], 0, 0, 0);
       acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv1, acc[1][1], 0, 0, 0);
     }
+  };
+  // One stage with its ring slot and LDS buffer fixed at compile time (b): register slot
+  // b held stage st, already in LDS buffer b, and is refilled with stage st + 2; after
+  // the MFMAs stage st + 1 (slot 1 - b) goes into LDS buffer 1 - b, last read before the
+  // previous barrier.  Stages past the split stage zeros, so stages run in unconditional
+  // pairs (a conditional stage would move the accumulators between register files).
+  auto stage = [&](int64_t st, auto bc) {
+    constexpr int b = decltype(bc)::value;
+    gload(st + 2, b);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(b);
     __builtin_amdgcn_sched_barrier(0);
     asm volatile("" ::: "memory");
-    sstore(buf ^ 1);
+    sstore(1 - b, 1 - b);
     __syncthreads();
+  };
+  if (nst > 0) {
+    gload(0, 0);
+    gload(1, 1);
+    sstore(0, 0);
+    __syncthreads();
+  }
+  for (int64_t st = 0; st < nst; st += 2) {
+    stage(st, std::integral_constant<int, 0>());
+    stage(st + 1, std::integral_constant<int, 1>());
   }
   // D layout of v_mfma_f64_16x16x4f64: item q of a lane holds row lane/16 + 4q,
   // column lane%16
@@ -119,6 +148,22 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
         int64_t a = a0 + wa + 16 * i + kq + 4 * q, b = b0 + wb + 16 * j + l16;
         if (a < ka && b < kb) P[a * kb + b] = acc[i][j][q];
       }
+}
+
+// rows[0..nh) of A (lda) into the contiguous G (nh x ka) and their weights into wg
+__global__ __launch_bounds__(256) void gather_rows_kernel(const float* __restrict__ A, int64_t lda,
+                                                          int64_t ka, const int* __restrict__ rows,
+                                                          int64_t nh, const double* __restrict__ w,
+                                                          float* __restrict__ G,
+                                                          double* __restrict__ wg) {
+  const int64_t tot = nh * ka;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / ka, c = e - i * ka;
+    const int64_t r = rows[i];
+    G[e] = A[r * lda + c];
+    if (c == 0) wg[i] = w ? w[r] : 1.0;
+  }
 }
 
 __global__ void reduce_splits_kernel(const double* __restrict__ part, int64_t nsplit,
@@ -141,6 +186,21 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
                         double* C, const int* rows) {
   if (ka <= 0 || kb <= 0) return TSVM_OK;
   if (lower_only && ka != kb) return fail(c, TSVM_ERR_ARG, "gemm_tn_f64: lower_only needs ka==kb");
+  if (rows) {
+    // a row list: gather the rows (and weights) into a contiguous block first, so that
+    // the GEMM's loads do not wait on the index loads (the gather moves 8 n ka bytes, a
+    // few percent of the GEMM's time)
+    if (A != B || lda != ldb || ka != kb)
+      return fail(c, TSVM_ERR_ARG, "gemm_tn_f64: a row list needs A == B");
+    void *pg, *pw;
+    TSVM_HIP(c, c->ws.get("gemm_gather", (size_t)((n > 0 ? n : 1) * ka * 4), &pg));
+    TSVM_HIP(c, c->ws.get("gemm_gather_w", (size_t)((n > 0 ? n : 1) * 8), &pw));
+    if (n > 0)
+      hipLaunchKernelGGL(gather_rows_kernel, dim3(4096), dim3(256), 0, c->stream, A, lda, ka, rows, n,
+                         w, (float*)pg, (double*)pw);
+    return gemm_tn_f64(c, (const float*)pg, ka, ka, (const float*)pg, ka, kb, n, (const double*)pw,
+                       lower_only, C, nullptr);
+  }
   const int64_t ta = ceil_div(ka, 64), tb = ceil_div(kb, 64);
   const int64_t tiles = lower_only ? ta * (ta + 1) / 2 : ta * tb;
   // enough blocks to fill the chip several times; at least 256 rows per split
@@ -153,8 +213,17 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   void* part;
   TSVM_HIP(c, c->ws.get("gemm_part", (size_t)(nsplit * ka * kb * 8), &part));
   dim3 grid((unsigned)ta, (unsigned)tb, (unsigned)nsplit);
-  hipLaunchKernelGGL(gemm_tn_f64_kernel, grid, dim3(256), 0, c->stream, A, lda, ka, B, ldb, kb,
-                     n, w, lower_only ? 1 : 0, rps, rows, (double*)part);
+  // full 64-column tiles with 16-byte rows: vector loads without column masks
+  const bool vec = ka % 64 == 0 && kb % 64 == 0 && lda % 4 == 0 && ldb % 4 == 0;
+#define GTN(V, W)                                                                              \
+  hipLaunchKernelGGL((gemm_tn_f64_kernel<V, W>), grid, dim3(256), 0, c->stream, A, lda, ka, B, \
+                     ldb, kb, n, w, lower_only ? 1 : 0, rps, (double*)part)
+  if (vec) {
+    if (w) GTN(true, true); else GTN(true, false);
+  } else {
+    if (w) GTN(false, true); else GTN(false, false);
+  }
+#undef GTN
   hipLaunchKernelGGL(reduce_splits_kernel, dim3((unsigned)ceil_div(ka * kb, 256)), dim3(256), 0,
                      c->stream, (const double*)part, nsplit, ka, kb, lower_only ? 1 : 0, C);
   TSVM_HIP(c, hipGetLastError());

@@ -492,6 +492,10 @@ tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridS
   hs->rows = (int*)p;
   TSVM_HIP(c, c->ws.get("hs_ints", (size_t)(HS_BLOCKS * (HS_BINS + 2) + 8) * 4, &p));
   hs->ints = (int*)p;
+  // the FP64 rows are gathered into these by gemm_tn_f64 each iteration: sized once for
+  // every row, so that a growing row count never reallocates (and drains the device)
+  TSVM_HIP(c, c->ws.get("gemm_gather", (size_t)(n > 0 ? n : 1) * kp * 4, &p));
+  TSVM_HIP(c, c->ws.get("gemm_gather_w", (size_t)(n > 0 ? n : 1) * 8, &p));
   hipStream_t s = c->stream;
   dim3 g((unsigned)(hs->n_pad / 32), (unsigned)(hs->kpt / 32));
   hipLaunchKernelGGL(make_zbt_kernel, g, dim3(256), 0, s, Z, n, kp, hs->kpt, hs->n_pad,
