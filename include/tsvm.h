@@ -30,11 +30,15 @@ typedef int tsvm_status;
 /* Arithmetic of the kernel-product and Z^T D^-1 Z GEMMs. */
 #define TSVM_PREC_FP32 0       /* plain form: FP32 MFMA products, FP64 capacitance */
 #define TSVM_PREC_BF16 1       /* low-precision MFMA form: kernel products with the distance
-                                  in split bf16 and E.A on the fp16 MFMA (the factor's sketch
-                                  in TSVM_PREC_FP16X2, its second product in TSVM_PREC_FP16),
-                                  FP32 accumulate; FP64 Cholesky + refinement in the IPM */
-/* Kernel-product-only forms (tsvm_kernel_product[_dev]); the distance GEMM is the BF16
- * form's, the product E.A runs on the fp16 MFMA with A under one power-of-two scale: */
+                                  in split bf16 and E.A on the fp16 MFMA (both products of
+                                  the factor in TSVM_PREC_FP16), FP32 accumulate; hybrid
+                                  BF16/FP64 capacitance, FP64 Cholesky + refinement in the
+                                  IPM */
+/* Kernel-product forms (tsvm_kernel_product[_dev]); the distance GEMM is the BF16 form's,
+ * the product E.A runs on the fp16 MFMA with A under one power-of-two scale.
+ * TSVM_PREC_FP16X2 is also a training / factor form: the BF16 form with the factor's
+ * sketch in the two-term product (decision values closer to the FP32 form's, one more
+ * MFMA per tile in the sketch). */
 #define TSVM_PREC_FP16X2 2     /* E as two fp16 terms (~2^-22), A fp16 (2^-11) */
 #define TSVM_PREC_FP16 3       /* E and A fp16 (2^-11 each), one MFMA per tile */
 #define TSVM_PREC_FP16R2 4     /* E fp16 (2^-11), A as two fp16 terms (~2^-22) */
@@ -189,9 +193,9 @@ typedef struct {
   int maxit;         /* MAX_MPC_ITER = 100 (62) */
   int stop_mode;     /* TSVM_STOP_REF or TSVM_STOP_GAP */
   double tol;        /* for TSVM_STOP_GAP */
-  int precision;     /* TSVM_PREC_FP32: FP64 capacitance; TSVM_PREC_BF16: hybrid capacitance
-                        (tsvm_weighted_syrk_dev, bulk rows on BF16 MFMA) + refinement of
-                        the Newton step against the FP32 factor */
+  int precision;     /* TSVM_PREC_FP32: FP64 capacitance; TSVM_PREC_BF16 (or FP16X2): hybrid
+                        capacitance (tsvm_weighted_syrk_dev, bulk rows on BF16 MFMA) +
+                        refinement of the Newton step against the FP32 factor */
   int refine;        /* refinement sweeps of each Newton step's bordered system, default 1 */
   int verbose;       /* print the reference's iteration line every 5 iterations */
   int init;          /* 0: the reference's initial point for kernel_type; 1: the same x with
@@ -233,7 +237,7 @@ typedef struct {
   double C;           /* -c (default 1) */
   double gamma;       /* -g (default 0) */
   int kernel_type;    /* -t: 2 RBF (LRA + IPM), 0 linear (IPM on diag(y) X) */
-  int precision;      /* TSVM_PREC_FP32 / TSVM_PREC_BF16 (kernel products; -tensorcore) */
+  int precision;      /* TSVM_PREC_FP32 / TSVM_PREC_BF16 (-tensorcore) / TSVM_PREC_FP16X2 */
   int stop_mode;      /* TSVM_STOP_REF (default) or TSVM_STOP_GAP */
   double tol;         /* gap tolerance for TSVM_STOP_GAP */
   int maxit;          /* IPM iterations (100) */

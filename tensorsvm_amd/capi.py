@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 PREC_FP32 = 0
 PREC_BF16 = 1
-PREC_FP16X2 = 2  # kernel product only: kernel values as two fp16 terms
+PREC_FP16X2 = 2  # kernel values as two fp16 terms; as a training form: BF16 with a two-term sketch
 PREC_FP16 = 3    # kernel product only: kernel values and right-hand side in fp16
 PREC_FP16R2 = 4  # kernel product only: right-hand side as two fp16 terms
 

@@ -615,7 +615,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   // on FP64 (split per iteration so that the BF16 error bound stays within
   // IPM_SYRK_BUDGET; lambda_min(M) >= 1 makes that bound the refinement's contraction
   // factor up to a constant).  The plain form builds it in FP64 throughout.
-  const bool bf16 = prm->precision == TSVM_PREC_BF16;
+  const bool bf16 = prm->precision == TSVM_PREC_BF16 || prm->precision == TSVM_PREC_FP16X2;
   // refinement sweeps of every SMW solve against the exact operator D + Z Z^T (FP64
   // accumulation of the FP32 factor); SMW alone loses accuracy like eps lambda_max(M)
   // cond(M), which is large at n in the millions

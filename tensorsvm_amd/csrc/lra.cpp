@@ -2,10 +2,11 @@
 //
 // Everything of size n stays resident in HBM; only k x k matrices cross to the host.
 //   1. W = K_Y Omega                      fused kernel-product kernel (FP32 MFMA; the BF16
-//                                         form runs this sketch in the split FP16X2 product:
-//                                         rounding noise in the sketch adds non-kernel
-//                                         directions to range(W) and costs captured energy
-//                                         in its trailing directions)
+//                                         form runs this sketch in the fp16 product, the
+//                                         FP16X2 form in the two-term product: rounding
+//                                         noise in the sketch adds non-kernel directions to
+//                                         range(W), which the two-term form keeps ~2^-11
+//                                         smaller)
 //   2. W <- orth(W)                       shifted Cholesky-QR (Fukaya et al., SIAM J. Sci.
 //                                         Comput. 2020): G = W^T W on FP64 MFMA, host
 //                                         Cholesky of G + s I, W <- W L^{-T}; repeated until
@@ -129,7 +130,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
                     double gamma, int precision, float* Om, float* U, tsvm_lra_info* info) {
   if (n <= 0 || d <= 0 || k <= 0 || k > n)
     return fail(c, TSVM_ERR_ARG, "lra: need 0 < k <= n (n=%ld k=%ld)", (long)n, (long)k);
-  if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16)
+  if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16 && precision != TSVM_PREC_FP16X2)
     return fail(c, TSVM_ERR_ARG, "lra: unknown precision %d", precision);
   hipStream_t s = c->stream;
   tsvm_lra_info inf;
@@ -149,7 +150,8 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   TSVM_HIP(c, hipEventCreate(&e0));
   TSVM_HIP(c, hipEventCreate(&e1));
   TSVM_HIP(c, hipEventRecord(e0, s));
-  const int sketch_prec = precision == TSVM_PREC_BF16 ? TSVM_PREC_FP16X2 : precision;
+  // low-precision forms: the sketch in fp16 (BF16 form) or in the two-term product
+  const int sketch_prec = precision == TSVM_PREC_BF16 ? TSVM_PREC_FP16 : precision;
   TSVM_CHECK(product_rows(c, "factor: K_Y Omega", X, y, n, d, Om, k, gamma, sketch_prec, W));
   TSVM_HIP(c, hipEventRecord(e1, s));
   TSVM_HIP(c, hipStreamSynchronize(s));
@@ -167,7 +169,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   // 4. Q = K_Y W (into the Omega buffer)
   float* Q = Om;
   TSVM_HIP(c, hipEventRecord(e0, s));
-  const int q_prec = precision == TSVM_PREC_BF16 ? TSVM_PREC_FP16 : precision;
+  const int q_prec = precision == TSVM_PREC_FP32 ? TSVM_PREC_FP32 : TSVM_PREC_FP16;
   TSVM_CHECK(product_rows(c, "factor: K_Y W", X, y, n, d, W, k, gamma, q_prec, Q));
   TSVM_HIP(c, hipEventRecord(e1, s));
   TSVM_HIP(c, hipStreamSynchronize(s));

@@ -49,14 +49,16 @@ def test_train_fp32_matches_oracle(ctx, n, k):
     assert np.allclose(fg, O.decision(al, y, X, gamma, info.b, Xt), atol=1e-4)
 
 
-def test_train_bf16_same_accuracy(ctx):
+@pytest.mark.parametrize("form", ["bf16", "split"])
+def test_train_bf16_same_accuracy(ctx, form):
     import tensorsvm_amd as T
+    prec = T.PREC_BF16 if form == "bf16" else T.PREC_FP16X2
     gamma, C, k = 0.05, 32.0, 64
     X, y, Xt, yt = _split(6000, 18, 77)
     om = O.gaussrand(X.shape[0] * k)
     al32, _, i32 = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=T.PREC_FP32, omega=om,
                              stop=T.capi.STOP_GAP, tol=1e-5)
-    al16, _, i16 = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=T.PREC_BF16, omega=om,
+    al16, _, i16 = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=prec, omega=om,
                              stop=T.capi.STOP_GAP, tol=1e-5)
     assert i16.ipm.reason == 0 and i32.ipm.reason == 0
     assert abs(i16.ipm.pobj - i32.ipm.pobj) <= 5e-3 * abs(i32.ipm.pobj)
