@@ -50,8 +50,10 @@ def log(*a):
 # HBM bytes per launch of the sketch kernel at this workload: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch) scaled by the factor that the same counters give for the
 # bench's 3.28 GB Omega copy (copyBuffer) -- see the profile named in "source"
-KPROD_TRAFFIC = {"fetch_bytes": 416_600_207_377, "write_bytes": 3_279_159_296,
-                 "source": "profiles/kprod_sketch_c2_pmc.md"}
+# HBM bytes per launch of the factor's product kernel at the default workload (mean of its
+# two launches; rocprof FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, calibrated on a copy)
+KPROD_TRAFFIC = {"fetch_bytes": 5_681_017_946_112, "write_bytes": 3_280_994_304,
+                 "source": "profiles/kprod_factor_c2_pmc.md"}
 
 
 def grad_form(ctx, torch, n=200_000, nt=50_000, d=32, k=256, C=1.0, seed=1003):
@@ -207,28 +209,26 @@ def main():
                 "alpha_max_abs_diff_vs_bf16": float(np.abs(alpha32 - alpha_bf16).max())}
         log(f"fp32 train {dt32:.2f}s (lra {info32.lra.t_total:.2f}s), accuracy {acc32:.4f}")
 
-    # roofline of the dominant kernel: the sketch product (first product of the last
-    # step); the second product is listed beside it
+    # roofline of the dominant kernel.  BF16 form: both products of the factor run the
+    # same kernel (fp16 product), so its average launch is the mean of the two; the
+    # FP32 form likewise runs kprod_f32 twice.
     flops = 2.0 * ntr * ntr * (d + k)
-    t_kp = bf16_info["lra"]["t_product1"]
-    achieved = flops / t_kp / 1e12
+    t1, t2 = bf16_info["lra"]["t_product1"], bf16_info["lra"]["t_product2"]
     tr = KPROD_TRAFFIC
     if args.precision == "bf16":
-        peak, k1, k2 = PEAK_BF16, "kprod_bf16<8,64,1,EM_F16X2> (fused RBF product, sketch K_Y Omega)", \
-            "kprod_bf16<4,64,2,EM_F16> (K_Y W)"
+        peak, kname = PEAK_BF16, "kprod_bf16<4,64,2,EM_F16,2> (fused RBF product, K_Y Omega and K_Y W)"
     else:
-        peak, k1, k2 = PEAK_FP32, "kprod_f32 (fused RBF product, K_Y Omega)", "kprod_f32 (K_Y W)"
-    same_config = args.n == 2_000_000 and k == 512 and args.precision == "bf16"
+        peak, kname = PEAK_FP32, "kprod_f32 (fused RBF product, K_Y Omega and K_Y W)"
+    t_kp = 0.5 * (t1 + t2)
+    achieved = flops / t_kp / 1e12
+    same_config = args.n == 2_000_000 and k == 512 and args.precision == "bf16" and args.data == "gaussian"
     roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4),
                 "traffic": (tr["fetch_bytes"] + tr["write_bytes"]
                             if tr["fetch_bytes"] is not None and same_config else None),
-                "kernel": k1, "flops_per_launch": flops, "launch_s": round(t_kp, 4),
-                "traffic_source": tr["source"] if same_config else None,
-                "second_product": {
-                    "kernel": k2,
-                    "achieved": round(flops / bf16_info["lra"]["t_product2"] / 1e12, 1),
-                    "frac": round(flops / bf16_info["lra"]["t_product2"] / 1e12 / peak, 4)}}
+                "kernel": kname, "flops_per_launch": flops, "launch_s": round(t_kp, 4),
+                "launches": {"K_Y Omega": round(t1, 4), "K_Y W": round(t2, 4)},
+                "traffic_source": tr["source"] if same_config else None}
 
     grad = None if args.no_grad else grad_form(ctx, torch)
 
