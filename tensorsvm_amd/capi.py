@@ -371,7 +371,7 @@ class Context:
 
     def train(self, X, y, k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32,
               stop=STOP_REF, tol=1e-5, maxit=100, refine=1, verbose=0, omega=None, init=0,
-              analysis=0):
+              analysis=0, progress=0):
         """Train on X (n x d) with labels +-1; returns (alpha, w (linear) or None, info)."""
         X, y = _f32(X), _f32(y)
         n, d = X.shape
@@ -379,7 +379,7 @@ class Context:
         al = np.empty(n)
         w = np.empty(d)
         prm = train_params(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init,
-                           analysis)
+                           analysis, progress)
         info = TrainInfo()
         self._check(self.lib.tsvm_train(self.h, _ptr(X), _ptr(y), n, d, ctypes.byref(prm), _ptr(om),
                                         _ptr(al), _ptr(w), ctypes.byref(info)), "tsvm_train")

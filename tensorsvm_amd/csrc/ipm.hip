@@ -358,30 +358,6 @@ __global__ void k_smw_fin(int64_t n, int p, int accumulate, const double* __rest
   }
 }
 
-// refinement residual: res = rhs - D sol - E (E = Z Z^T sol); V = res / D; reductions of
-// |res|^2 and |rhs|^2 for RHS 0
-__global__ __launch_bounds__(256) void k_refine_res(int64_t n, int p, const double* __restrict__ rhs,
-                                                    int64_t ldr, const double* __restrict__ sol,
-                                                    int64_t lds, const double* __restrict__ E, int64_t lde,
-                                                    const double* __restrict__ D,
-                                                    const double* __restrict__ winv,
-                                                    double* __restrict__ V, int64_t ldv,
-                                                    double* __restrict__ red) {
-  double v[3] = {0, 0, 0};
-  const int op[3] = {RED_SUM, RED_SUM, RED_SUM};
-  GRID_LOOP(i, n) {
-    double Dv = D[i], w = winv[i];
-    for (int q = 0; q < p; q++) {
-      double r = rhs[q * ldr + i] - Dv * sol[q * lds + i] - E[q * lde + i];
-      V[q * ldv + i] = r * w;
-      if (q < 2) v[q] += r * r;
-    }
-    double b = rhs[i];
-    v[2] += b * b;
-  }
-  block_reduce_store<3>(v, op, red);
-}
-
 // dots of the Newton step: a^T sol0 (and a^T sol1)
 __global__ __launch_bounds__(256) void k_dots(int64_t n, int p, const double* __restrict__ a,
                                               const double* __restrict__ sol, int64_t lds,
@@ -618,10 +594,6 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   // IPM_SYRK_BUDGET; lambda_min(M) >= 1 makes that bound the refinement's contraction
   // factor up to a constant).  The plain form builds it in FP64 throughout.
   const bool bf16 = prm->precision == TSVM_PREC_BF16 || prm->precision == TSVM_PREC_FP16X2;
-  // refinement sweeps of every SMW solve against the exact operator D + Z Z^T (FP64
-  // accumulation of the FP32 factor); SMW alone loses accuracy like eps lambda_max(M)
-  // cond(M), which is large at n in the millions
-  const int refine = 0;
   void* p;
   auto ws = [&](const char* name, int64_t bytes, void** out) -> tsvm_status {
     TSVM_HIP(c, c->ws.get(name, (size_t)bytes, out));
@@ -686,7 +658,9 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
                        (const int*)tables, (const int*)(tables + MAXR), sc);
     return TSVM_OK;
   };
-  // SMW solve with optional refinement: sol = (D + Z Z^T)^{-1} rhs, p right-hand sides
+  // SMW solve: sol = (D + Z Z^T)^{-1} rhs with the capacitance as factored, p right-hand
+  // sides (its error is corrected by the bordered refinement below, whose residual uses
+  // the exact operator)
   auto smw = [&](double* rh, int pp, double* sol) -> tsvm_status {
     hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, pp, (const double*)rh, n, (const double*)winv, V, n);
     TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, T));
@@ -694,18 +668,6 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, pp, E, n));
     hipLaunchKernelGGL(k_smw_fin, eg, eb, 0, st, n, pp, 0, (const double*)V, n, (const double*)E, n,
                        (const double*)winv, sol, n);
-    for (int sw = 0; sw < refine; sw++) {
-      TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, sol, n, pp, T));
-      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, pp, E, n));
-      hipLaunchKernelGGL(k_refine_res, eg, eb, 0, st, n, pp, (const double*)rh, n, (const double*)sol, n,
-                         (const double*)E, n, (const double*)D, (const double*)winv, V, n, red);
-      TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_RES0, SL_RES1, SL_RHS0}));
-      TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, T));
-      TSVM_CHECK(chol_solve_f64(c, M, kp, T, pp, dinfo));
-      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, pp, E, n));
-      hipLaunchKernelGGL(k_smw_fin, eg, eb, 0, st, n, pp, 1, (const double*)V, n, (const double*)E, n,
-                         (const double*)winv, sol, n);
-    }
     TSVM_HIP(c, hipGetLastError());
     return TSVM_OK;
   };

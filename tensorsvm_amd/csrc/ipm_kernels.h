@@ -26,9 +26,6 @@ enum IpmSlot {
   SL_DY,         // predictor dy
   SL_DYC,        // corrector dy
   SL_CHOLFAIL,   // 1 when the capacitance Cholesky failed (the step was not taken)
-  SL_RES0,       // |refinement residual|^2, RHS 0 (before / after)
-  SL_RES1,
-  SL_RHS0,       // |rhs|^2 of RHS 0
   SL_QQ,         // init: |Z Z^T 1|^2
   SL_QE,         // init: 1^T Z Z^T 1
   SL_R2B,        // bordered refinement: a^T dx of the step being refined

@@ -2,6 +2,7 @@
 plain method on the same seeded inputs and Omega."""
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -112,6 +113,57 @@ def test_cli_rbf_model_file(tmp_path):
     assert abs(int(head["total_sv"]) - nsv) <= max(2, nsv // 200)
     assert abs(float(head["rho"]) + bo) <= 5e-3 * (1 + abs(bo))
     assert np.all(np.abs(coef) <= 32.0 + 1e-6)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "split"])
+def test_cli_low_precision_forms(tmp_path, prec):
+    """-precision bf16 / split through the command line: a model file whose held-out
+    accuracy (through tensorsvm-predict) matches the FP32 form's."""
+    X, y = gaussian_mixture(3000, 18, 21)
+    tr, te = str(tmp_path / "tr.libsvm"), str(tmp_path / "te.libsvm")
+    _write_libsvm(tr, X[:2400], y[:2400])
+    _write_libsvm(te, X[2400:], y[2400:])
+    exe = os.path.join(ROOT, "tensorsvm_amd", "tensorsvm-train")
+    pred = os.path.join(ROOT, "tensorsvm_amd", "tensorsvm-predict")
+    acc = {}
+    for form in ("fp32", prec):
+        model = str(tmp_path / f"{form}.model")
+        r = subprocess.run([exe, "-k", "48", "-c", "32", "-g", "0.05", "-t", "2", "-precision", form,
+                            "-stop", "gap", tr, model], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        out = str(tmp_path / f"{form}.out")
+        r = subprocess.run([pred, te, model, out], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        acc[form] = float(r.stdout.split("Accuracy = ")[1].split("%")[0])
+    assert abs(acc[prec] - acc["fp32"]) <= 1.0, acc
+
+
+def test_train_progress_lines(tmp_path):
+    """progress = 1: one stderr line per kernel-product slab and per 10 IPM iterations, and
+    the same multipliers as without."""
+    script = tmp_path / "p.py"
+    script.write_text(f"""
+import sys
+sys.path.insert(0, {ROOT!r}); sys.path.insert(0, {os.path.join(ROOT, 'tests')!r})
+import numpy as np
+import tensorsvm_amd as T
+import oracle_lib as O
+from datagen import gaussian_mixture
+X, y = gaussian_mixture(3000, 18, 3)
+om = O.gaussrand(3000 * 32)
+c = T.Context(0)
+a0, _, i0 = c.train(X, y, k=32, C=32.0, gamma=0.05, precision=T.PREC_BF16, omega=om,
+                    stop=T.capi.STOP_GAP, tol=1e-5)
+a1, _, i1 = c.train(X, y, k=32, C=32.0, gamma=0.05, precision=T.PREC_BF16, omega=om,
+                    stop=T.capi.STOP_GAP, tol=1e-5, progress=1)
+print("same", bool(np.array_equal(a0, a1)), i0.ipm.iters, i1.ipm.iters)
+""")
+    r = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "same True" in r.stdout, r.stdout
+    assert "factor: K_Y Omega: rows 3000 / 3000" in r.stderr, r.stderr
+    assert "factor: K_Y W: rows 3000 / 3000" in r.stderr
+    assert "ipm: iteration 0, relative gap" in r.stderr
 
 
 def test_cli_linear_model_file(tmp_path):
