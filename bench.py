@@ -122,6 +122,9 @@ def main():
                          "covtype-like set (d=54, 10 uniform + 44 binary columns, RBF labels)")
     ap.add_argument("--gamma", type=float, default=None, help="RBF gamma (0.05 gaussian, 1.0 covtype)")
     ap.add_argument("--analysis", action="store_true", help="|K_Y - U U^T|_F / |K_Y|_F after the factor")
+    ap.add_argument("--grad-timed", action="store_true",
+                    help="gradient (log gamma and per-feature scales) of the timed model at its own "
+                         "size, forward products in the timed form's precision")
     ap.add_argument("--progress", action="store_true",
                     help="progress lines on stderr from the library (row slabs of each product)")
     ap.add_argument("--precision", choices=("bf16", "fp32"), default="bf16",
@@ -161,6 +164,8 @@ def main():
     log(f"omega: {ntr}x{k} gaussrand stream in {t_omega:.1f}s")
 
     ctx = T.Context(0)
+    if args.progress:
+        ctx.set_progress(True)
     dev = torch.device("cuda:0")
     dX = torch.from_numpy(X).to(dev)
     dy = torch.from_numpy(y).to(dev)
@@ -200,6 +205,31 @@ def main():
     acc_bf16, alpha_bf16 = accuracy(info)
     bf16_info = info.as_dict()
     value = float(np.mean(times))
+
+    grad_timed = None
+    if args.grad_timed:
+        # the multipliers of the timed run are still in dal (the comparison run comes later)
+        dXte, dyte = torch.from_numpy(Xte).to(dev), torch.from_numpy(yte).to(dev)
+        gprec = T.PREC_BF16 if args.precision == "bf16" else T.PREC_FP32
+        torch.cuda.synchronize()
+        tg0 = time.perf_counter()
+        Dg, Lg, gDg, gLg, gi = ctx.grad_dev(dX, dy, ntr, d, dal, info.b, k, float(np.log(gamma)),
+                                            np.zeros(d), dOm0, dXte, dyte, Xte.shape[0],
+                                            precision=gprec)
+        torch.cuda.synchronize()
+        tg = time.perf_counter() - tg0
+        grad_timed = {"workload": f"gradient of the timed model in (log gamma, {d} log-scales), "
+                                  f"{ntr} train + {Xte.shape[0]} held-out rows",
+                      "forward_products": "fp16" if args.precision == "bf16" else "fp32",
+                      "t_grad_s": round(tg, 3), "t_backward_s": round(gi.t_backward, 3),
+                      "backward_tflops": round(gi.backward_tflops, 1),
+                      "roofline": {"bound": "mfma", "achieved": round(gi.backward_tflops, 1),
+                                   "peak": PEAK_FP32, "unit": "TFLOP/s",
+                                   "frac": round(gi.backward_tflops / PEAK_FP32, 4)},
+                      "dual": Dg, "heldout_loss": Lg, "dD_dloggamma": gDg[0], "dL_dloggamma": gLg[0]}
+        log(f"grad of the timed model: {tg:.1f}s (backward {gi.t_backward:.1f}s, "
+            f"{gi.backward_tflops:.1f} TF/s); dD/dlog(gamma) {gDg[0]:.6e}")
+        del dXte, dyte
 
     fp32 = None
     if not args.no_fp32 and args.precision == "bf16":
@@ -273,6 +303,7 @@ def main():
                 "fp32_form_fp64_tflops": (round(fp32["info"]["ipm"]["syrk_tflops"], 1) if fp32 else None),
                 "timed_form_capacitance_tflops": round(bf16_info["ipm"]["syrk_tflops"], 1)},
         "grad": grad,
+        "grad_timed": grad_timed,
     }
     # the timed form under its own name; with BF16 timed, "fp32" holds the comparison run
     out[args.precision] = {"train_s": value, "accuracy": acc_bf16, "info": bf16_info}

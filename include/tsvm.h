@@ -51,6 +51,11 @@ typedef struct tsvm_ctx tsvm_ctx;
 tsvm_status tsvm_create(int device, tsvm_ctx **out);
 void tsvm_destroy(tsvm_ctx *ctx);
 const char *tsvm_last_error(const tsvm_ctx *ctx);
+/* Progress lines on stderr during long calls on this context (kernel-product slabs of
+ * the factor, IPM iterations, gradient phases).  No counterpart in the reference: for
+ * runs of many minutes under a supervisor that watches for output.  Off by default;
+ * tsvm_train_params.progress turns it on for one training call. */
+void tsvm_set_progress(tsvm_ctx *ctx, int on);
 /* Synchronise the context's stream. */
 tsvm_status tsvm_sync(tsvm_ctx *ctx);
 
@@ -137,8 +142,11 @@ tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int
  *             (1 - z)^2 / 2 (0 < z < 1), 0 (z >= 1)
  * g_dual, g_loss (d + 1 each, host; component 0 = d/dlog gamma) may be NULL.  The
  * backward pass recomputes kernel tiles in the fused FP32 product over derivative
- * right-hand sides; nothing of size n^2 is stored.  Device: X (n x d), y, alpha (FP64),
- * omega (n x k, not modified), Xt (nt x d), yt; host: log_scales (d), vals (2). */
+ * right-hand sides; nothing of size n^2 is stored.  precision: the forward pass's two
+ * products of the factor (K_Y Omega, K_Y W) in TSVM_PREC_FP32, or in the fp16 product
+ * with TSVM_PREC_BF16 (about 8x faster at large n; the backward stays FP32).  Device: X
+ * (n x d), y, alpha (FP64), omega (n x k, not modified), Xt (nt x d), yt; host:
+ * log_scales (d), vals (2). */
 typedef struct {
   double t_forward;       /* s: scaled features, factor and dual value */
   double t_backward;      /* s in the backward products and reductions (HIP events) */
@@ -151,8 +159,8 @@ typedef struct {
 tsvm_status tsvm_grad_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
                           const double *alpha, double b, int64_t k, double log_gamma,
                           const double *log_scales, const float *omega, const float *Xt,
-                          const float *yt, int64_t nt, double *vals, double *g_dual,
-                          double *g_loss, tsvm_grad_info *info);
+                          const float *yt, int64_t nt, int precision, double *vals,
+                          double *g_dual, double *g_loss, tsvm_grad_info *info);
 
 /* ---- model files and prediction ------------------------------------------------------
  * The model files the trainer writes (RBF: LIBSVM text, writemodel 353-406; linear:

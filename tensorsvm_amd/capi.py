@@ -159,6 +159,7 @@ _EXTRA = [
                           ctypes.POINTER(_I64), ctypes.POINTER(_I64)], None),
     ("tsvm_write_libsvm", [ctypes.c_char_p, _P, _P, _I64, _I64], None),
     ("tsvm_free", [_P], ctypes.c_int),
+    ("tsvm_set_progress", [_P, _INT], 0),
     ("tsvm_map_labels", [_P, _I64, ctypes.POINTER(_INT), ctypes.POINTER(_INT)], _I64),
     ("tsvm_write_model_rbf", [ctypes.c_char_p, _P, _P, _P, _I64, _I64, _D, _D, _D, _INT, _INT], None),
     ("tsvm_write_model_linear", [ctypes.c_char_p, _P, _I64, _D, _INT, _INT], None),
@@ -176,8 +177,8 @@ _EXTRA = [
     ("tsvm_predict", [_P, _P, _P, _I64, _I64, _P, _P], None),
     ("tsvm_fnorm_res_dev", [_P, _P, _P, _I64, _I64, _D, _P, _I64, ctypes.POINTER(_D),
                             ctypes.POINTER(_D)], None),
-    ("tsvm_grad_dev", [_P, _P, _P, _I64, _I64, _P, _D, _I64, _D, _P, _P, _P, _P, _I64, _P, _P, _P,
-                       ctypes.POINTER(GradInfo)], None),
+    ("tsvm_grad_dev", [_P, _P, _P, _I64, _I64, _P, _D, _I64, _D, _P, _P, _P, _P, _I64, _INT, _P, _P,
+                       _P, ctypes.POINTER(GradInfo)], None),
 ]
 
 
@@ -402,11 +403,15 @@ class Context:
                                                     b, _ptr(T), m, _ptr(f)), "tsvm_decision_function")
         return f
 
+    def set_progress(self, on=True):
+        """Progress lines on stderr during long calls on this context."""
+        self.lib.tsvm_set_progress(self.h, int(bool(on)))
+
     def grad_dev(self, X, y, n, d, alpha, b, k, log_gamma, log_scales, omega, Xt=None, yt=None,
-                 nt=0, want_grad=True):
+                 nt=0, want_grad=True, precision=PREC_FP32):
         """(D, L, g_dual, g_loss, info): the trained dual through the factor and the held-out
         mean smoothed hinge, and their gradients in (log gamma, log_scales); device X, y,
-        alpha (FP64), omega, Xt, yt."""
+        alpha (FP64), omega, Xt, yt.  precision: the forward products of the factor."""
         sc = np.ascontiguousarray(log_scales, dtype=np.float64)
         vals = np.zeros(2)
         gD = np.zeros(d + 1)
@@ -415,7 +420,7 @@ class Context:
         self._check(self.lib.tsvm_grad_dev(
             self.h, _ptr(X), _ptr(y), n, d, _ptr(alpha), b, k, log_gamma,
             sc.ctypes.data_as(_P), _ptr(omega), _ptr(Xt) if nt else None,
-            _ptr(yt) if nt else None, nt, vals.ctypes.data_as(_P),
+            _ptr(yt) if nt else None, nt, precision, vals.ctypes.data_as(_P),
             gD.ctypes.data_as(_P) if want_grad else None,
             gL.ctypes.data_as(_P) if want_grad else None, ctypes.byref(info)), "tsvm_grad_dev")
         return vals[0], vals[1], gD, gL, info

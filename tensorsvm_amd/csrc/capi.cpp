@@ -90,6 +90,10 @@ const char* tsvm_last_error(const tsvm_ctx* ctx) {
   return ctx ? ctx->c.last_error.c_str() : "no context";
 }
 
+void tsvm_set_progress(tsvm_ctx* ctx, int on) {
+  if (ctx) ctx->c.progress = on ? 1 : 0;
+}
+
 tsvm_status tsvm_sync(tsvm_ctx* ctx) {
   if (!ctx) return TSVM_ERR_ARG;
   TSVM_HIP(&ctx->c, hipStreamSynchronize(ctx->c.stream));
@@ -117,12 +121,12 @@ tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx* ctx, const float* Z, int64_t n, int
 tsvm_status tsvm_grad_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t n, int64_t d,
                           const double* alpha, double b, int64_t k, double log_gamma,
                           const double* log_scales, const float* omega, const float* Xt,
-                          const float* yt, int64_t nt, double* vals, double* g_dual,
-                          double* g_loss, tsvm_grad_info* info) {
+                          const float* yt, int64_t nt, int precision, double* vals,
+                          double* g_dual, double* g_loss, tsvm_grad_info* info) {
   if (!ctx) return TSVM_ERR_ARG;
   hipSetDevice(ctx->c.device);
   return tsvm::grad_dev(&ctx->c, X, y, n, d, alpha, b, k, log_gamma, log_scales, omega, Xt, yt,
-                        nt, vals, g_dual, g_loss, info);
+                        nt, precision, vals, g_dual, g_loss, info);
 }
 
 tsvm_status tsvm_fnorm_res_dev(tsvm_ctx* ctx, const float* X, const float* y, int64_t n,

@@ -28,6 +28,8 @@
 #include <chrono>
 #include <vector>
 
+#include <stdio.h>
+
 #include "common.h"
 #include "internal.h"
 
@@ -187,10 +189,11 @@ bool solve_dense(std::vector<double> A, int64_t k, double* x) {
 tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d,
                      const double* alpha, double b, int64_t k, double log_gamma,
                      const double* log_scales, const float* Om, const float* Xt,
-                     const float* yt, int64_t nt, double* vals, double* gD, double* gL,
-                     tsvm_grad_info* info) {
+                     const float* yt, int64_t nt, int precision, double* vals, double* gD,
+                     double* gL, tsvm_grad_info* info) {
   if (n <= 0 || d <= 0 || d > 128 || k <= 0 || k > n || nt < 0 || !X || !y || !alpha || !Om ||
-      !log_scales || !vals || (nt > 0 && (!Xt || !yt)))
+      !log_scales || !vals || (nt > 0 && (!Xt || !yt)) ||
+      (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16))
     return fail(c, TSVM_ERR_ARG, "grad: bad arguments (n=%ld d=%ld k=%ld nt=%ld)", (long)n,
                 (long)d, (long)k, (long)nt);
   const double t0 = now();
@@ -239,8 +242,19 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
     TSVM_HIP(c, hipMemcpyAsync(ones, h1.data(), nmax * 4, hipMemcpyHostToDevice, s));
     TSVM_HIP(c, hipStreamSynchronize(s));
   }
-  // ---- factor: Y = K_Y Omega, W = orth(Y), R = W^T Y, Q = K_Y W, C = W^T Q
-  TSVM_CHECK(kernel_product_dev(c, Xs, y, n, Xs, y, n, d, Om, k, k, gamma, TSVM_PREC_FP32, Yb, k));
+  // ---- factor: Y = K_Y Omega, W = orth(Y), R = W^T Y, Q = K_Y W, C = W^T Q (the two
+  // products in FP32, or in the fp16 product)
+  const int fprec = precision == TSVM_PREC_FP32 ? TSVM_PREC_FP32 : TSVM_PREC_FP16;
+  auto progress = [&](const char* what) -> tsvm_status {
+    if (c->progress) {
+      TSVM_HIP(c, hipStreamSynchronize(s));
+      fprintf(stderr, "grad: %s (%.1f s)\n", what, now() - t0);
+      fflush(stderr);
+    }
+    return TSVM_OK;
+  };
+  TSVM_CHECK(kernel_product_dev(c, Xs, y, n, Xs, y, n, d, Om, k, k, gamma, fprec, Yb, k));
+  TSVM_CHECK(progress("factor product K_Y Omega"));
   TSVM_HIP(c, hipMemcpyAsync(Wb, Yb, n * k * 4, hipMemcpyDeviceToDevice, s));
   std::vector<double> G;
   int passes = 0;
@@ -248,7 +262,8 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
   std::vector<double> R(k * k), C(k * k);
   TSVM_CHECK(gemm_tn_f64(c, Wb, k, k, Yb, k, k, n, nullptr, false, dkk));
   TSVM_HIP(c, hipMemcpyAsync(R.data(), dkk, k * k * 8, hipMemcpyDeviceToHost, s));
-  TSVM_CHECK(kernel_product_dev(c, Xs, y, n, Xs, y, n, d, Wb, k, k, gamma, TSVM_PREC_FP32, Qb, k));
+  TSVM_CHECK(kernel_product_dev(c, Xs, y, n, Xs, y, n, d, Wb, k, k, gamma, fprec, Qb, k));
+  TSVM_CHECK(progress("factor product K_Y W"));
   TSVM_CHECK(gemm_tn_f64(c, Wb, k, k, Qb, k, k, n, nullptr, false, dkk));
   TSVM_HIP(c, hipMemcpyAsync(C.data(), dkk, k * k * 8, hipMemcpyDeviceToHost, s));
   TSVM_HIP(c, hipStreamSynchronize(s));
@@ -327,6 +342,7 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
     TSVM_HIP(c, hipEventRecord(e0, s));
     TSVM_CHECK(kernel_product_dev(c, Xs, ones, n, Xs, ones, n, d, Rh, rtr, rtr, gamma,
                                   TSVM_PREC_FP32, Wd, rtr));
+    TSVM_CHECK(progress("backward product of the dual"));
     for (int m = 0; m < 3; m++)
       hipLaunchKernelGGL(grad_reduce_kernel, dim3(nbr), dim3(256), 0, s, (const float*)Xs, n,
                          (int)d, us[m], y, (const float*)Wd, rtr, (int64_t)m * (2 * d + 1), rpb,
@@ -362,6 +378,7 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
     TSVM_HIP(c, hipEventRecord(e0, s));
     TSVM_CHECK(kernel_product_dev(c, Xts, ones, nt, Xs, ones, n, d, Rh, rte, rte, gamma,
                                   TSVM_PREC_FP32, Wd, rte));
+    TSVM_CHECK(progress("backward product of the held-out loss"));
     TSVM_HIP(c, hipEventRecord(e1, s));
     std::vector<float> hw(nt * rte);
     TSVM_HIP(c, hipMemcpyAsync(hw.data(), Wd, nt * rte * 4, hipMemcpyDeviceToHost, s));

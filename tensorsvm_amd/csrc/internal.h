@@ -99,8 +99,8 @@ tsvm_status fnorm_res_dev(Ctx* c, const float* X, const float* y, int64_t n, int
 tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d,
                      const double* alpha, double b, int64_t k, double log_gamma,
                      const double* log_scales, const float* Om, const float* Xt,
-                     const float* yt, int64_t nt, double* vals, double* gD, double* gL,
-                     tsvm_grad_info* info);
+                     const float* yt, int64_t nt, int precision, double* vals, double* gD,
+                     double* gL, tsvm_grad_info* info);
 // tsvm_weighted_syrk_dev (synchronises before returning)
 tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, const double* w,
                               int precision, double budget, double* M, tsvm_syrk_info* info);

@@ -45,8 +45,9 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
   if (k <= 0 || k > n) return fail(c, TSVM_ERR_ARG, "train: need 0 < k <= n");
   struct ProgressScope {  // progress lines for the duration of this call
     Ctx* c;
-    ProgressScope(Ctx* cc, int p) : c(cc) { c->progress = p; }
-    ~ProgressScope() { c->progress = 0; }
+    int saved;
+    ProgressScope(Ctx* cc, int p) : c(cc), saved(cc->progress) { c->progress = p || saved; }
+    ~ProgressScope() { c->progress = saved; }
   } progress_scope(c, prm->progress);
   tsvm_train_info inf;
   memset(&inf, 0, sizeof(inf));

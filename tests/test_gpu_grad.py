@@ -74,6 +74,24 @@ def test_grad_central_differences(ctx):
             assert abs(fL - gL[m]) <= 1e-2 * np.abs(gL).max(), (m, fL, gL[m])
 
 
+def test_grad_fp16_forward_close_to_fp32(ctx):
+    """The forward pass's factor products in the fp16 product: values and gradients close
+    to the FP32 forward's (the backward is FP32 in both)."""
+    import tensorsvm_amd as T
+    n, nt, d, k = 4000, 1000, 12, 48
+    X, y, Xt, yt, om, al, b, g0 = _trained(ctx, n, nt, d, k, 77)
+    dX, dy, dal, dom, dXt, dyt = _dev(X, y, al, om, Xt, yt)
+    lg, s = np.log(g0), np.zeros(d)
+    D32, L32, gD32, gL32, _ = ctx.grad_dev(dX, dy, n, d, dal, b, k, lg, s, dom, dXt, dyt, nt,
+                                           precision=T.PREC_FP32)
+    D16, L16, gD16, gL16, _ = ctx.grad_dev(dX, dy, n, d, dal, b, k, lg, s, dom, dXt, dyt, nt,
+                                           precision=T.PREC_BF16)
+    assert abs(D16 - D32) <= 1e-3 * abs(D32), (D16, D32)
+    assert L16 == L32  # the held-out loss does not use the factor
+    assert np.abs(gD16 - gD32).max() <= 1e-2 * np.abs(gD32).max(), (gD16, gD32)
+    assert np.abs(gL16 - gL32).max() <= 1e-6 * np.abs(gL32).max()
+
+
 def test_grad_autograd_function(ctx):
     """The C-ABI gradient as the backward of a torch.autograd.Function over theta."""
     import torch
