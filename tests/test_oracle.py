@@ -196,3 +196,36 @@ def test_grad_matches_central_differences():
     # log gamma acts like a common shift of the log scales (K depends on gamma e^{2 s_f})
     assert abs(gD[0] - 0.5 * gD[1:].sum()) <= 1e-9 * np.abs(gD).max()
 
+
+def test_decision_and_linear_bias_against_sklearn():
+    """orc_decision and orc_bias_linear on sklearn's own solutions: the exact-kernel decision
+    values of an RBF SVC (dual_coef_ = y_j alpha_j over its support vectors, intercept_ = b)
+    and the bias of a linear SVC (the mean of y_i - w.x_i over free support vectors, which
+    at the optimum is the intercept)."""
+    from sklearn.svm import SVC
+    rng = np.random.default_rng(17)
+    n, d, C, gamma = 300, 5, 2.0, 0.3
+    y = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    X = (rng.normal(size=(n, d)) + 0.6 * y[:, None]).astype(np.float32)
+    Xt = rng.normal(size=(80, d)).astype(np.float32)
+    clf = SVC(kernel="rbf", gamma=gamma, C=C, tol=1e-10).fit(X.astype(np.float64), y)
+    # sklearn's convention: f = dual_coef_ . K(sv, x) + intercept_, positive for classes_[1]
+    sv = clf.support_
+    K = np.exp(-gamma * ((X[sv].astype(np.float64)[:, None, :] - Xt[None].astype(np.float64)) ** 2).sum(-1))
+    f_sk = clf.decision_function(Xt.astype(np.float64))
+    assert np.allclose(clf.dual_coef_[0] @ K + clf.intercept_[0], f_sk, rtol=1e-10, atol=1e-12)
+    assert list(clf.classes_) == [-1.0, 1.0]
+    alpha = np.zeros(n)
+    alpha[sv] = clf.dual_coef_[0] * y[sv]
+    assert np.all(alpha[sv] > 0)
+    keep = alpha > 1e-6  # the oracle skips multipliers at or below 1e-6, as the reference
+    f_or = O.decision(alpha, y.astype(np.float32), X, gamma, clf.intercept_[0], Xt)
+    f_ref = (clf.dual_coef_[0][keep[sv]] @ K[keep[sv]]) + clf.intercept_[0]
+    assert np.allclose(f_or, f_ref, rtol=1e-12, atol=1e-12)
+    assert np.abs(f_or - f_sk).max() <= 1e-5 * np.abs(f_sk).max()
+    lin = SVC(kernel="linear", C=C, tol=1e-12).fit(X.astype(np.float64), y)
+    al = np.zeros(n)
+    al[lin.support_] = lin.dual_coef_[0] * y[lin.support_]
+    b, w = O.bias_linear(al, y.astype(np.float32), X, C)
+    assert np.allclose(w, lin.coef_[0], rtol=1e-9, atol=1e-9)
+    assert abs(b - lin.intercept_[0]) <= 1e-5 * (1 + abs(lin.intercept_[0])), (b, lin.intercept_)
