@@ -358,6 +358,16 @@ class Context:
                                           _ptr(omega), _ptr(U), ctypes.byref(info)), "tsvm_lra_dev")
         return info
 
+    def ipm_dev(self, Z, a, n, k, C, alpha, xi, s=None, maxit=100, stop=STOP_REF, tol=1e-5,
+                precision=PREC_FP32, refine=1, init=0):
+        """Dual IPM on device Z (n x k FP32) and labels a; alpha, xi (and s) device FP64."""
+        prm = ipm_params(C, 2, maxit, stop, tol, precision, refine, 0, init)
+        info = IpmInfo()
+        self._check(self.lib.tsvm_ipm_dev(self.h, _ptr(Z), _ptr(a), n, k, ctypes.byref(prm),
+                                          _ptr(alpha), _ptr(xi), _ptr(s) if s is not None else None,
+                                          ctypes.byref(info)), "tsvm_ipm_dev")
+        return info
+
     def ipm(self, Z, a, C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5,
             precision=PREC_FP32, refine=1, verbose=0, init=0):
         """Dual IPM on Q = Z Z^T; returns (alpha, xi, s, info)."""
