@@ -16,7 +16,9 @@
 //                     solve, the correction again combined with S^-1 a; step bound and
 //                     mu_aff as device reductions
 //   corrector         the same with one SMW solve (S^-1 a is reused), combined step, update
-// Per iteration that is 7 pass pairs over Z (Z^T v, then Z t).
+// The last pass of each SMW solve also accumulates Z^T sol, so Z^T of the combined step is
+// a combination of k-vectors, and the bordered residual pass also accumulates Z^T of the
+// next solve's right-hand side.  Per iteration that is 10 reads of Z.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -95,6 +97,50 @@ __global__ __launch_bounds__(256) void reduce_slots(const double* __restrict__ r
   }
 }
 
+// Fixed-order sum of per-block partials: T[e] = sum_b part[b * m + e] in two stages
+// (slices of 32 blocks, then the slices), so that no wave walks all nblk partials.
+__global__ __launch_bounds__(256) void reduce_parts_kernel(const double* __restrict__ part, int nblk,
+                                                           int m, double* __restrict__ dst) {
+  __shared__ double sh[4][64];
+  const int o = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+  const int b0 = blockIdx.y * 32, b1 = b0 + 32 < nblk ? b0 + 32 : nblk;
+  double s = 0.0;
+  if (o < m)
+#pragma unroll 8
+    for (int b = b0 + g; b < b1; b += 4) s += part[(int64_t)b * m + o];
+  sh[g][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (g == 0 && o < m)
+    dst[(int64_t)blockIdx.y * m + o] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+}
+
+static tsvm_status reduce_parts(Ctx* c, const double* part, int nblk, int m, double* T) {
+  if (nblk < 1 || nblk > 32 * 32) return fail(c, TSVM_ERR_ARG, "reduce_parts: %d partials", nblk);
+  const int S = (int)ceil_div(nblk, 32);
+  const dim3 b(256);
+  if (S == 1) {
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3((unsigned)ceil_div(m, 64), 1), b, 0, c->stream,
+                       part, nblk, m, T);
+  } else {
+    void* p2;
+    TSVM_HIP(c, c->ws.get("reduce_parts2", (size_t)S * m * 8, &p2));
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3((unsigned)ceil_div(m, 64), (unsigned)S), b, 0,
+                       c->stream, part, nblk, m, (double*)p2);
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3((unsigned)ceil_div(m, 64), 1), b, 0, c->stream,
+                       (const double*)p2, S, m, T);
+  }
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+
+// Fused passes over Z (defined below): modes and launcher.
+enum { ZP_SMW = 0, ZP_BORDER = 1, ZP_T = 2, ZP_N = 3 };
+static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int64_t n, int kp,
+                             const double* T, const double* V, const double* winv,
+                             const double* a, const double* D, const double* r5, const double* dx,
+                             const double* sc, int corr, double* out, double* red, double* Tout);
+
 // ------------------------------------------------------------------------------------
 // Z^T V: block b owns a row range; thread t handles float4 column group t % (kp/4) of
 // rows t / (kp/4) + step * R.  Partials per block, then a fixed-order reduction.
@@ -157,19 +203,6 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(const float* __restrict__ Z
   }
 }
 
-// T[e] = sum over blocks of part[b][e]; 64 outputs x 4 partial lanes per block
-__global__ __launch_bounds__(256) void gemv_t_reduce(const double* __restrict__ part, int nblk,
-                                                     int m, double* __restrict__ T) {
-  __shared__ double sh[4][64];
-  const int o = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
-  double s = 0.0;
-  if (o < m)
-    for (int b = g; b < nblk; b += 4) s += part[(int64_t)b * m + o];
-  sh[g][threadIdx.x & 63] = s;
-  __syncthreads();
-  if (g == 0 && o < m) T[o] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
-}
-
 static int gemv_blocks(Ctx* c, int64_t n) {
   int64_t b = (int64_t)c->cu_count * 2;
   int64_t mx = ceil_div(n, 64);
@@ -181,6 +214,10 @@ tsvm_status ipm_gemv_t(Ctx* c, const float* Z, int64_t n, int kp, const double* 
                        int p, double* T) {
   if (kp % 32 != 0 || kp > 1024 || p < 1 || p > 4)
     return fail(c, TSVM_ERR_ARG, "gemv_t: kp=%d p=%d unsupported", kp, p);
+  // two vectors: the wide fused kernel (the form below reads Z at half speed for p = 2)
+  if (kp >= 256 && p == 2 && ldv == n)
+    return ipm_zpass(c, ZP_T, p, p, Z, n, kp, nullptr, V, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, 0, nullptr, nullptr, T);
   const int nblk = gemv_blocks(c, n);
   const int64_t rpb = ceil_div(n, nblk);
   void* part;
@@ -194,11 +231,8 @@ tsvm_status ipm_gemv_t(Ctx* c, const float* Z, int64_t n, int kp, const double* 
     case 3: hipLaunchKernelGGL(gemv_t_kernel<3>, g, b, sm, c->stream, Z, n, kp, V, ldv, rpb, (double*)part); break;
     default: hipLaunchKernelGGL(gemv_t_kernel<4>, g, b, sm, c->stream, Z, n, kp, V, ldv, rpb, (double*)part); break;
   }
-  const int m = p * kp;
-  hipLaunchKernelGGL(gemv_t_reduce, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, c->stream,
-                     (const double*)part, nblk, m, T);
   TSVM_HIP(c, hipGetLastError());
-  return TSVM_OK;
+  return reduce_parts(c, (const double*)part, nblk, p * kp, T);
 }
 
 // ------------------------------------------------------------------------------------
@@ -265,6 +299,9 @@ tsvm_status ipm_gemv_n(Ctx* c, const float* Z, int64_t n, int kp, const double* 
                        double* E, int64_t lde) {
   if (kp % 32 != 0 || kp > 1024 || p < 1 || p > 2)
     return fail(c, TSVM_ERR_ARG, "gemv_n: kp=%d p=%d unsupported", kp, p);
+  if (kp >= 256 && p == 1 && lde == n)
+    return ipm_zpass(c, ZP_N, 1, 0, Z, n, kp, T, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, 0, E, nullptr, nullptr);
   int L = 64;
   while (L > kp / 4) L >>= 1;
   const int F = (int)ceil_div(kp / 4, L);  // <= 4 for kp <= 1024
@@ -280,6 +317,446 @@ tsvm_status ipm_gemv_n(Ctx* c, const float* Z, int64_t n, int kp, const double* 
   }
 #undef GN
   TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Fused passes over Z.  One read of each row z_i serves three things: the products
+// z_i . T_q (the layout of gemv_n_kernel), the row-wise update that consumes them, and the
+// accumulation of Z^T u for the vector u that the update produces (per-block partials,
+// reduced by reduce_parts in a fixed order).
+//   ZP_SMW     last step of the SMW solve (1828-1838): sol_q = V_q - (z_i . T_q) / D_i for
+//              q < P with V_q = D^-1 rhs_q and T_q = M^-1 Z^T V_q; reduces a^T sol_q;
+//              Q = P also accumulates Z^T sol_q, Q = 0 skips it
+//   ZP_BORDER  residual of the bordered Newton system for the step dx, with T = Z^T dx:
+//              r5'_i = D_i dx_i + z_i . T - a_i dy + r5_i (= -rho_1); writes
+//              V = r5' / D (the next SMW solve's scaled right-hand side), accumulates
+//              Z^T V and reduces a^T dx, |r5'|^2, |r5|^2
+// zpass_kernel serves kp < 256: L < 64 lanes share a row as in gemv_n_kernel, each lane
+// loads the per-row operands of its row.  ZP_U rows are in flight per lane.
+// ------------------------------------------------------------------------------------
+constexpr int ZP_U = 4;
+
+template <int MODE, int P, int Q, int F>
+__global__ __launch_bounds__(256) void zpass_kernel(
+    const float* __restrict__ Z, int64_t n, int kp, const double* __restrict__ T,
+    const double* __restrict__ V, const double* __restrict__ winv, const double* __restrict__ a,
+    const double* __restrict__ D, const double* __restrict__ r5, const double* __restrict__ dxv,
+    const double* __restrict__ sc, int corr, double* __restrict__ out, double* __restrict__ red,
+    double* __restrict__ part) {
+  extern __shared__ double shz[];  // 2 x Q x kp: waves 0, 1 store, waves 2, 3 add
+  constexpr int R = MODE == ZP_SMW ? P : 3;
+  constexpr int QA = Q > 0 ? Q : 1;
+  constexpr int NO = MODE == ZP_SMW ? P + 2 : 5;  // per-row operands
+  const int G = kp / 4;
+  int L = 64;
+  while (L > G) L >>= 1;
+  const int RW = 64 / L;  // rows per wave per step
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int seg = lane / L, li = lane % L;
+  double t[P][F][4], acc[QA][F][4];
+  int grp[F];  // float4 group of each f, clamped to a valid one (its t is zero)
+#pragma unroll
+  for (int f = 0; f < F; f++) {
+    const int g = li + f * L;
+    grp[f] = g < G ? g : 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+#pragma unroll
+      for (int q = 0; q < P; q++) t[q][f][j] = g < G ? T[q * kp + 4 * g + j] : 0.0;
+#pragma unroll
+      for (int q = 0; q < QA; q++) acc[q][f][j] = 0.0;
+    }
+  }
+  double rv[R];
+  int op[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    rv[r] = 0.0;
+    op[r] = RED_SUM;
+  }
+  const double dy = MODE == ZP_BORDER ? sc[corr ? SL_DYC : SL_DY] : 0.0;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + wid;
+  const int64_t step = (int64_t)gridDim.x * 4 * RW;
+  for (int64_t base = wave * RW + seg; base < n + seg; base += step * ZP_U) {
+    f32x4_t z[ZP_U][F];
+    double o[ZP_U][NO];
+    int64_t row[ZP_U];
+#pragma unroll
+    for (int u = 0; u < ZP_U; u++) {
+      row[u] = base + u * step;
+      const int64_t rr = row[u] < n ? row[u] : n - 1;
+#pragma unroll
+      for (int f = 0; f < F; f++) z[u][f] = *(const f32x4_t*)(Z + rr * kp + 4 * grp[f]);
+      if constexpr (MODE == ZP_SMW) {
+#pragma unroll
+        for (int q = 0; q < P; q++) o[u][q] = V[q * n + rr];
+        o[u][P] = winv[rr];
+        o[u][P + 1] = a[rr];
+      } else {
+        o[u][0] = D[rr];
+        o[u][1] = dxv[rr];
+        o[u][2] = winv[rr];
+        o[u][3] = a[rr];
+        o[u][4] = r5[rr];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < ZP_U; u++) {
+      double d[P];
+#pragma unroll
+      for (int q = 0; q < P; q++) {
+        double s = 0.0;
+#pragma unroll
+        for (int f = 0; f < F; f++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) s += (double)z[u][f][j] * t[q][f][j];
+        d[q] = s;
+      }
+#pragma unroll
+      for (int q = 0; q < P; q++)
+        for (int m = L / 2; m > 0; m >>= 1) d[q] += __shfl_xor(d[q], m, 64);
+      const bool ok = row[u] < n;
+      const bool wr = ok && li == 0;
+      double uq[QA];
+      if constexpr (MODE == ZP_SMW) {
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+          const double sq = o[u][q] - d[q] * o[u][P];
+          if (wr) {
+            out[q * n + row[u]] = sq;
+            rv[q] += o[u][P + 1] * sq;
+          }
+          if (q < QA) uq[q] = ok ? sq : 0.0;
+        }
+      } else {
+        const double r = o[u][0] * o[u][1] + d[0] - o[u][3] * dy + o[u][4];
+        const double vv = r * o[u][2];
+        if (wr) {
+          out[row[u]] = vv;
+          rv[0] += o[u][3] * o[u][1];
+          rv[1] += r * r;
+          rv[2] += o[u][4] * o[u][4];
+        }
+        uq[0] = ok ? vv : 0.0;
+      }
+      if constexpr (Q > 0) {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+          for (int f = 0; f < F; f++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[q][f][j] += uq[q] * (double)z[u][f][j];
+      }
+    }
+  }
+  if constexpr (Q > 0) {
+    // rows of the RW segments of a wave, then the four waves (two LDS rounds)
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+      for (int f = 0; f < F; f++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          for (int m = L; m < 64; m <<= 1) acc[q][f][j] += __shfl_xor(acc[q][f][j], m, 64);
+    const int half = wid & 1;
+    for (int rnd = 0; rnd < 2; rnd++) {
+      if ((wid >> 1) == rnd && seg == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+          for (int f = 0; f < F; f++) {
+            const int g = li + f * L;
+            if (g < G)
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                double* dst = &shz[(half * Q + q) * kp + 4 * g + j];
+                *dst = rnd == 0 ? acc[q][f][j] : *dst + acc[q][f][j];
+              }
+          }
+      }
+      __syncthreads();
+    }
+    for (int e = threadIdx.x; e < Q * kp; e += 256)
+      part[(int64_t)blockIdx.x * Q * kp + e] = shz[e] + shz[Q * kp + e];
+  }
+  block_reduce_store<R>(rv, op, red);
+}
+
+__device__ inline double f64_from(int lo, int hi) {
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  return f64_from(__builtin_amdgcn_readlane((int)b, l), __builtin_amdgcn_readlane((int)(b >> 32), l));
+}
+// value of a DPP-selected lane (all rows and banks enabled, every source lane exists)
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  return f64_from(__builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false),
+                  __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false));
+}
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1, 0, 3, 2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2, 3, 0, 1]
+constexpr int DPP_HALF_MIRROR = 0x141; // lane i <- 7 - i within 8
+constexpr int DPP_MIRROR = 0x140;      // lane i <- 15 - i within 16
+constexpr int DPP_ROR8 = 0x128;        // lane i <- (i + 8) mod 16 within 16
+// v_permlane32_swap: lanes 0-31 return a(l) + a(l + 32), lanes 32-63 b(l - 32) + b(l)
+__device__ inline double swap_sum32(double a, double b) {
+  const long long A = __double_as_longlong(a), B = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)A, (unsigned)B, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(A >> 32), (unsigned)(B >> 32), false, false);
+  return f64_from(lo[0], hi[0]) + f64_from(lo[1], hi[1]);
+}
+// v_permlane16_swap: rows (16 lanes) 0 and 2 return a(l) + a(l + 16), rows 1 and 3
+// b(l - 16) + b(l)
+__device__ inline double swap_sum16(double a, double b) {
+  const long long A = __double_as_longlong(a), B = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)A, (unsigned)B, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(A >> 32), (unsigned)(B >> 32), false, false);
+  return f64_from(lo[0], hi[0]) + f64_from(lo[1], hi[1]);
+}
+
+// The passes for kp >= 256, where a wave owns whole rows.  Each wave takes a contiguous
+// chunk of rows (a multiple of ZP_U), ZP_U consecutive rows per step, so that the per-row
+// operands and results of a step share cache lines.  Each lane holds F float4 groups of
+// the step's rows.  The P x ZP_U partial products are summed over the wave by a
+// reduce-scatter (lane bit 5 keeps half of the set, bit 4 half of the rest, ... by
+// permlane swaps and DPP moves, no LDS), after which lane l holds product
+// m = l >> (6 - log2(P ZP_U)) (q = m / ZP_U, row u = m % ZP_U) and has loaded that row's
+// operands itself, so the row-wise update runs for all rows at once.  The updated values
+// return to every lane by readlane for the Z^T accumulation.  Besides ZP_SMW and
+// ZP_BORDER:
+//   ZP_T       Z^T V_q for q < P (Q = P), no products
+//   ZP_N       out = Z T (P = 1, Q = 0)
+template <int MODE, int P, int Q, int F>
+__global__ __launch_bounds__(256) void zpass_wide_kernel(
+    const float* __restrict__ Z, int64_t n, int kp, const double* __restrict__ T,
+    const double* __restrict__ V, const double* __restrict__ winv, const double* __restrict__ a,
+    const double* __restrict__ D, const double* __restrict__ r5, const double* __restrict__ dxv,
+    const double* __restrict__ sc, int corr, double* __restrict__ out, double* __restrict__ red,
+    double* __restrict__ part) {
+  extern __shared__ double shz[];  // 2 x Q x kp: waves 0, 1 store, waves 2, 3 add
+  constexpr bool DOT = MODE != ZP_T;
+  constexpr bool RED = MODE == ZP_SMW || MODE == ZP_BORDER;
+  constexpr int R = MODE == ZP_BORDER ? 3 : P;
+  constexpr int QA = Q > 0 ? Q : 1;
+  constexpr int M = P * ZP_U;  // 4 or 8 products per step
+  constexpr int LOGM = M == 4 ? 2 : 3;
+  constexpr int GL = 64 >> LOGM;  // lanes per product after the reduction
+  constexpr int NO = MODE == ZP_BORDER ? 5 : (MODE == ZP_SMW ? 3 : 1);
+  static_assert(M == 4 || M == 8, "P * ZP_U must be 4 or 8");
+  const int G = kp / 4;
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ml = lane >> (6 - LOGM);
+  const int ql = ml / ZP_U, ul = ml % ZP_U;
+  const bool lead = (lane & (GL - 1)) == 0;
+  double t[DOT ? P : 1][F][4], acc[QA][F][4];
+  int grp[F];
+#pragma unroll
+  for (int f = 0; f < F; f++) {
+    const int g = lane + f * 64;
+    grp[f] = g < G ? g : 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if constexpr (DOT)
+#pragma unroll
+        for (int q = 0; q < P; q++) t[q][f][j] = g < G ? T[q * kp + 4 * g + j] : 0.0;
+#pragma unroll
+      for (int q = 0; q < QA; q++) acc[q][f][j] = 0.0;
+    }
+  }
+  double rv[R];
+  int op[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    rv[r] = 0.0;
+    op[r] = RED_SUM;
+  }
+  const double dy = MODE == ZP_BORDER ? sc[corr ? SL_DYC : SL_DY] : 0.0;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + wid;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int64_t chunk = ceil_div(ceil_div(n, nw), ZP_U) * ZP_U;
+  const int64_t r0 = wave * chunk;
+  const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
+  for (int64_t base = r0; base < r1; base += ZP_U) {
+    // rows base .. base + ZP_U - 1; rows past r1 are clamped for the loads and masked
+    f32x4_t z[ZP_U][F];
+#pragma unroll
+    for (int u = 0; u < ZP_U; u++) {
+      const int64_t rr = base + u < n ? base + u : n - 1;
+#pragma unroll
+      for (int f = 0; f < F; f++) z[u][f] = *(const f32x4_t*)(Z + rr * kp + 4 * grp[f]);
+    }
+    const int64_t myrow = base + ul;
+    const bool ok = myrow < r1;
+    const int64_t rl = myrow < n ? myrow : n - 1;
+    double o[NO];
+    if constexpr (MODE == ZP_BORDER) {
+      o[0] = D[rl];
+      o[1] = dxv[rl];
+      o[2] = winv[rl];
+      o[3] = a[rl];
+      o[4] = r5[rl];
+    } else if constexpr (MODE == ZP_SMW) {
+      o[0] = V[ql * n + rl];
+      o[1] = winv[rl];
+      o[2] = a[rl];
+    } else if constexpr (MODE == ZP_T) {
+      o[0] = V[ql * n + rl];
+    }
+    double w;  // this lane's product after the row-wise update
+    if constexpr (DOT) {
+      double v[M];
+#pragma unroll
+      for (int q = 0; q < P; q++)
+#pragma unroll
+        for (int u = 0; u < ZP_U; u++) {
+          double s = 0.0;
+#pragma unroll
+          for (int f = 0; f < F; f++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) s += (double)z[u][f][j] * t[q][f][j];
+          v[q * ZP_U + u] = s;
+        }
+      // reduce-scatter over lane bits 5, 4 (and 3 for M = 8), then sums within GL lanes
+#pragma unroll
+      for (int i = 0; i < M / 2; i++) v[i] = swap_sum32(v[i], v[i + M / 2]);
+#pragma unroll
+      for (int i = 0; i < M / 4; i++) v[i] = swap_sum16(v[i], v[i + M / 4]);
+      if constexpr (M == 8) {
+        const bool hi = (lane & 8) != 0;
+        v[0] = (hi ? v[1] : v[0]) + dpp_f64<DPP_ROR8>(hi ? v[0] : v[1]);
+      }
+      v[0] += dpp_f64<DPP_XOR1>(v[0]);
+      v[0] += dpp_f64<DPP_XOR2>(v[0]);
+      v[0] += dpp_f64<DPP_HALF_MIRROR>(v[0]);
+      if constexpr (M == 4) v[0] += dpp_f64<DPP_MIRROR>(v[0]);
+      if constexpr (MODE == ZP_SMW) {
+        w = o[0] - v[0] * o[1];
+        if (ok && lead) {
+          out[ql * n + myrow] = w;
+#pragma unroll
+          for (int q = 0; q < P; q++)
+            if (q == ql) rv[q] += o[2] * w;
+        }
+      } else if constexpr (MODE == ZP_BORDER) {
+        const double r = o[0] * o[1] + v[0] - o[3] * dy + o[4];
+        w = r * o[2];
+        if (ok && lead) {
+          out[myrow] = w;
+          rv[0] += o[3] * o[1];
+          rv[1] += r * r;
+          rv[2] += o[4] * o[4];
+        }
+      } else {
+        w = v[0];
+        if (ok && lead) out[myrow] = w;
+      }
+    } else {
+      w = o[0];
+    }
+    if constexpr (Q > 0) {
+#pragma unroll
+      for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int u = 0; u < ZP_U; u++) {
+          const double uq = base + u < r1 ? readlane_f64(w, (q * ZP_U + u) * GL) : 0.0;
+#pragma unroll
+          for (int f = 0; f < F; f++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[q][f][j] += uq * (double)z[u][f][j];
+        }
+    }
+  }
+  if constexpr (Q > 0) {
+    const int half = wid & 1;
+    for (int rnd = 0; rnd < 2; rnd++) {
+      if ((wid >> 1) == rnd) {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+          for (int f = 0; f < F; f++) {
+            const int g = lane + f * 64;
+            if (g < G)
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                double* dst = &shz[(half * Q + q) * kp + 4 * g + j];
+                *dst = rnd == 0 ? acc[q][f][j] : *dst + acc[q][f][j];
+              }
+          }
+      }
+      __syncthreads();
+    }
+    for (int e = threadIdx.x; e < Q * kp; e += 256)
+      part[(int64_t)blockIdx.x * Q * kp + e] = shz[e] + shz[Q * kp + e];
+  }
+  if constexpr (RED) block_reduce_store<R>(rv, op, red);
+}
+
+static int zpass_blocks(Ctx* c) { return c->cu_count * 4 < EW_BLOCKS ? c->cu_count * 4 : EW_BLOCKS; }
+
+static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int64_t n, int kp,
+                             const double* T, const double* V, const double* winv,
+                             const double* a, const double* D, const double* r5, const double* dx,
+                             const double* sc, int corr, double* out, double* red, double* Tout) {
+  const int G = kp / 4;
+  const bool wide = G >= 64;
+  bool ok_shape = false;
+  if (mode == ZP_SMW) ok_shape = (p == 2 && q == 2) || (p == 1 && (q == 0 || q == 1));
+  else if (mode == ZP_BORDER) ok_shape = p == 1 && q == 1;
+  else if (mode == ZP_T) ok_shape = wide && (p == 1 || p == 2) && q == p;
+  else if (mode == ZP_N) ok_shape = wide && p == 1 && q == 0;
+  if (kp % 32 != 0 || kp > 1024 || !ok_shape)
+    return fail(c, TSVM_ERR_ARG, "zpass: kp=%d mode=%d p=%d q=%d unsupported", kp, mode, p, q);
+  int L = 64;
+  while (L > G) L >>= 1;
+  const int F = (int)ceil_div(G, L);  // wide: <= 4, else <= 2
+  const int nblk = zpass_blocks(c);
+  void* part = nullptr;
+  if (q > 0) TSVM_HIP(c, c->ws.get("zpass_part", (size_t)nblk * q * kp * 8, &part));
+  const size_t sm = (size_t)2 * q * kp * 8;
+  const dim3 g(nblk), b(256);
+#define ZK(KN, MO, PP, QQ, FF)                                                             \
+  hipLaunchKernelGGL((KN<MO, PP, QQ, FF>), g, b, sm, c->stream, Z, n, kp, T, V, winv, a, D, \
+                     r5, dx, sc, corr, out, red, (double*)part)
+#define ZW(MO, PP, QQ)                                         \
+  if (F == 1) ZK(zpass_wide_kernel, MO, PP, QQ, 1);            \
+  else if (F == 2) ZK(zpass_wide_kernel, MO, PP, QQ, 2);       \
+  else if (F == 3) ZK(zpass_wide_kernel, MO, PP, QQ, 3);       \
+  else ZK(zpass_wide_kernel, MO, PP, QQ, 4);
+#define ZF(MO, PP, QQ)                                         \
+  if (wide) {                                                  \
+    ZW(MO, PP, QQ)                                             \
+  } else {                                                     \
+    if (F == 1) ZK(zpass_kernel, MO, PP, QQ, 1);               \
+    else ZK(zpass_kernel, MO, PP, QQ, 2);                      \
+  }
+  if (mode == ZP_BORDER) {
+    ZF(ZP_BORDER, 1, 1);
+  } else if (mode == ZP_T) {
+    if (p == 2) {
+      ZW(ZP_T, 2, 2);
+    } else {
+      ZW(ZP_T, 1, 1);
+    }
+  } else if (mode == ZP_N) {
+    ZW(ZP_N, 1, 0);
+  } else if (p == 2) {
+    ZF(ZP_SMW, 2, 2);
+  } else if (q == 1) {
+    ZF(ZP_SMW, 1, 1);
+  } else {
+    ZF(ZP_SMW, 1, 0);
+  }
+#undef ZF
+#undef ZW
+#undef ZK
+  TSVM_HIP(c, hipGetLastError());
+  if (q > 0) TSVM_CHECK(reduce_parts(c, (const double*)part, nblk, q * kp, Tout));
   return TSVM_OK;
 }
 
@@ -344,83 +821,38 @@ __global__ void k_scale(int64_t n, int p, const double* __restrict__ rhs, int64_
   }
 }
 
-// sol (+)= V - E/D  (SMW: D^-1 v - D^-1 Z M^-1 Z^T D^-1 v, 1828-1838)
-__global__ void k_smw_fin(int64_t n, int p, int accumulate, const double* __restrict__ V, int64_t ldv,
-                          const double* __restrict__ E, int64_t lde, const double* __restrict__ winv,
-                          double* __restrict__ sol, int64_t lds) {
-  GRID_LOOP(i, n) {
-    double w = winv[i];
-    for (int q = 0; q < p; q++) {
-      double v = V[q * ldv + i] - E[q * lde + i] * w;
-      if (accumulate) sol[q * lds + i] += v;
-      else sol[q * lds + i] = v;
-    }
-  }
-}
-
-// dots of the Newton step: a^T sol0 (and a^T sol1)
-__global__ __launch_bounds__(256) void k_dots(int64_t n, int p, const double* __restrict__ a,
-                                              const double* __restrict__ sol, int64_t lds,
-                                              double* __restrict__ red) {
-  double v[2] = {0, 0};
-  const int op[2] = {RED_SUM, RED_SUM};
-  GRID_LOOP(i, n) {
-    double av = a[i];
-    v[0] += av * sol[i];
-    if (p > 1) v[1] += av * sol[lds + i];
-  }
-  block_reduce_store<2>(v, op, red);
-}
-
 // third solve of the Newton step (1815-1816): dx = S^-1 (a dy - r5) with
 // dy = (r2 + a^T r7) / (a^T b), r2 = -a^T x (predictor) or 0 (corrector).  The SMW
 // operator S^-1 = (D + Z Z^T)^-1 (with the capacitance as factored) is linear and
 // r7 = S^-1 r5, b = S^-1 a are already at hand, so dx = dy b - r7 without another solve.
+// With Tr7 = Z^T r7 and Tb = Z^T b given (kp entries), block 0 also forms
+// T3 = Z^T dx = dy Tb - Tr7 for the bordered refinement.
 __global__ void k_sol3_lin(int64_t n, int corr, const double* __restrict__ b,
                            const double* __restrict__ r7, double* __restrict__ sc,
-                           double* __restrict__ sol3) {
+                           double* __restrict__ sol3, int kp, const double* __restrict__ Tr7,
+                           const double* __restrict__ Tb, double* __restrict__ T3) {
   const double r2 = corr ? 0.0 : -sc[SL_AX];
   const double dy = (r2 + sc[corr ? SL_AR7C : SL_AR7]) / sc[SL_AB];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[corr ? SL_DYC : SL_DY] = dy;
+  if (Tr7 && blockIdx.x == 0)
+    for (int e = threadIdx.x; e < kp; e += blockDim.x) T3[e] = dy * Tb[e] - Tr7[e];
   GRID_LOOP(i, n) sol3[i] = dy * b[i] - r7[i];
-}
-
-// Residual of the bordered Newton system (A dx - a dy = -r5, a^T dx = r2) for dx = sol,
-// A dx = D dx + E (E = Z Z^T dx).  Writes r5' = -rho_1 = A dx - a dy + r5 into out and
-// reduces a^T dx (for rho_2 = r2 - a^T dx), |rho_1|^2 and |r5|^2.
-__global__ __launch_bounds__(256) void k_border_res(int64_t n, int corr, const double* __restrict__ a,
-                                                    const double* __restrict__ r5,
-                                                    const double* __restrict__ sol,
-                                                    const double* __restrict__ E,
-                                                    const double* __restrict__ D,
-                                                    const double* __restrict__ sc,
-                                                    double* __restrict__ out, double* __restrict__ red) {
-  const double dy = sc[corr ? SL_DYC : SL_DY];
-  const double r2 = corr ? 0.0 : -sc[SL_AX];
-  double v[3] = {0, 0, 0};
-  const int op[3] = {RED_SUM, RED_SUM, RED_SUM};
-  GRID_LOOP(i, n) {
-    double ax = D[i] * sol[i] + E[i];
-    double rho1 = -r5[i] - (ax - a[i] * dy);
-    out[i] = -rho1;
-    v[0] += a[i] * sol[i];
-    v[1] += rho1 * rho1;
-    v[2] += r5[i] * r5[i];
-  }
-  (void)r2;
-  block_reduce_store<3>(v, op, red);
 }
 
 // bordered correction: (ddx, ddy) solve the same bordered system for the residual
 // (r5', rho2): ddy = (rho2 + a^T S^-1 r5') / a^T b, ddx = S^-1 (a ddy - r5') = ddy b - s5
-// with s5 = S^-1 r5' and b = S^-1 a (S^-1 linear); dx += ddx, dy += ddy
+// with s5 = S^-1 r5' and b = S^-1 a (S^-1 linear); dx += ddx, dy += ddy.  With Ts5 =
+// Z^T s5 given, block 0 also updates T3 = Z^T dx by ddy Tb - Ts5 (for a further sweep).
 __global__ void k_border_lin(int64_t n, int corr, const double* __restrict__ b,
                              const double* __restrict__ s5, double* __restrict__ sc,
-                             double* __restrict__ sol3) {
+                             double* __restrict__ sol3, int kp, const double* __restrict__ Ts5,
+                             const double* __restrict__ Tb, double* __restrict__ T3) {
   const double r2 = corr ? 0.0 : -sc[SL_AX];
   const double rho2 = r2 - sc[SL_R2B];
   const double ddy = (rho2 + sc[SL_AR7B]) / sc[SL_AB];
   GRID_LOOP(i, n) sol3[i] += ddy * b[i] - s5[i];
+  if (Ts5 && blockIdx.x == 0)
+    for (int e = threadIdx.x; e < kp; e += blockDim.x) T3[e] += ddy * Tb[e] - Ts5[e];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc[SL_DYB] = ddy;
     sc[corr ? SL_DYC : SL_DY] += ddy;
@@ -599,7 +1031,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_HIP(c, c->ws.get(name, (size_t)bytes, out));
     return TSVM_OK;
   };
-  double *a, *e, *dx, *ds, *dxi, *adx, *ads, *adxi, *D, *winv, *rhs, *V, *E, *solp, *solc, *T, *M,
+  double *a, *e, *dx, *ds, *dxi, *adx, *ads, *adxi, *D, *winv, *rhs, *V, *solp, *solc, *T, *M,
       *sc, *red, *s, *sol3;
   int *dinfo, *tables;
 #define WS(name, bytes, var)              \
@@ -617,11 +1049,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   WS("ipm_winv", n * 8, winv);
   WS("ipm_rhs", 2 * n * 8, rhs);
   WS("ipm_V", 2 * n * 8, V);
-  WS("ipm_E", 2 * n * 8, E);
   WS("ipm_solp", 2 * n * 8, solp);
   WS("ipm_solc", n * 8, solc);
   WS("ipm_sol3", n * 8, sol3);
-  WS("ipm_T", 4 * kp * 8, T);
+  WS("ipm_T", 6 * kp * 8, T);
   WS("ipm_M", (int64_t)kp * kp * 8, M);
   WS("ipm_sc", IPM_NSLOT * 8, sc);
   WS("ipm_red", EW_BLOCKS * MAXR * 8, red);
@@ -646,7 +1077,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   hipLaunchKernelGGL(k_labels_to_f64, eg, eb, 0, st, n, af, a);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (int64_t)IPM_NSLOT, sc, 0.0);
 
-  auto reduce = [&](std::vector<int> ops, std::vector<int> slots) -> tsvm_status {
+  auto reduce = [&](std::vector<int> ops, std::vector<int> slots, int nblk = EW_BLOCKS) -> tsvm_status {
     int R = (int)ops.size();
     int buf[2 * MAXR] = {0};
     for (int r = 0; r < R; r++) {
@@ -654,42 +1085,50 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
       buf[MAXR + r] = slots[r];
     }
     TSVM_HIP(c, hipMemcpyAsync(tables, buf, sizeof(buf), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(reduce_slots, dim3(1), dim3(256), 0, st, (const double*)red, EW_BLOCKS, R,
+    hipLaunchKernelGGL(reduce_slots, dim3(1), dim3(256), 0, st, (const double*)red, nblk, R,
                        (const int*)tables, (const int*)(tables + MAXR), sc);
     return TSVM_OK;
   };
+  const int zblk = zpass_blocks(c);
+  // k-vectors: the capacitance solves (two rows), Z^T r7 and Z^T b of the predictor,
+  // Z^T of the latest single solve, Z^T dx of the step being refined
+  double* Tsol = T;
+  double* Tr7 = T + 2 * kp;
+  double* Tb = T + 3 * kp;
+  double* Tc = T + 4 * kp;
+  double* T3 = T + 5 * kp;
   // SMW solve: sol = (D + Z Z^T)^{-1} rhs with the capacitance as factored, p right-hand
   // sides (its error is corrected by the bordered refinement below, whose residual uses
-  // the exact operator)
-  auto smw = [&](double* rh, int pp, double* sol) -> tsvm_status {
-    hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, pp, (const double*)rh, n, (const double*)winv, V, n);
-    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, T));
-    TSVM_CHECK(chol_solve_f64(c, M, kp, T, pp, dinfo));
-    TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, pp, E, n));
-    hipLaunchKernelGGL(k_smw_fin, eg, eb, 0, st, n, pp, 0, (const double*)V, n, (const double*)E, n,
-                       (const double*)winv, sol, n);
-    TSVM_HIP(c, hipGetLastError());
-    return TSVM_OK;
+  // the exact operator).  V = D^-1 rhs must be in place; Z^T V, M^-1 Z^T V, then one
+  // fused pass forms sol, its a^T sol_q (to slots) and, with Tz, Z^T sol_q.
+  auto smw = [&](int pp, double* sol, double* Tz, std::vector<int> slots) -> tsvm_status {
+    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, Tsol));
+    TSVM_CHECK(chol_solve_f64(c, M, kp, Tsol, pp, dinfo));
+    TSVM_CHECK(ipm_zpass(c, ZP_SMW, pp, Tz ? pp : 0, Z, n, kp, Tsol, V, winv, a, nullptr, nullptr,
+                         nullptr, sc, 0, sol, red, Tz));
+    return reduce(std::vector<int>(slots.size(), RED_SUM), slots, zblk);
   };
 
   // Refinement of the bordered Newton system for the step in sol3 (r5 in rhs row 0):
   //   rho = [-r5 - (A dx - a dy); r2 - a^T dx];  (ddx, ddy) from the same Schur
-  //   elimination; dx += ddx, dy += ddy.
+  //   elimination; dx += ddx, dy += ddy.  T3 = Z^T sol3 comes in from the linear
+  //   combination that formed sol3, so the residual pass reads Z once.
   const int border = prm->refine < 0 ? 1 : (prm->refine > 0 ? prm->refine : 0);
   auto border_refine = [&](int corr) -> tsvm_status {
     for (int sw = 0; sw < border; sw++) {
-      TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, sol3, n, 1, T));
-      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, 1, E, n));
-      hipLaunchKernelGGL(k_border_res, eg, eb, 0, st, n, corr, (const double*)a, (const double*)rhs,
-                         (const double*)sol3, (const double*)E, (const double*)D, (const double*)sc,
-                         V + n, red);
-      TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_R2B, SL_BRES, SL_BRHS}));
+      const bool more = sw + 1 < border;
+      // residual, V = r5' / D and Z^T V in one pass
+      TSVM_CHECK(ipm_zpass(c, ZP_BORDER, 1, 1, Z, n, kp, T3, nullptr, winv, a, D, rhs, sol3, sc,
+                           corr, V, red, Tsol));
+      TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_R2B, SL_BRES, SL_BRHS}, zblk));
       // s5 = S^-1 r5'; the correction is a combination of s5 and b = S^-1 a
-      TSVM_CHECK(smw(V + n, 1, solc + 0));
-      hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 1, (const double*)a, (const double*)solc, n, red);
-      TSVM_CHECK(reduce({RED_SUM}, {SL_AR7B}));
+      TSVM_CHECK(chol_solve_f64(c, M, kp, Tsol, 1, dinfo));
+      TSVM_CHECK(ipm_zpass(c, ZP_SMW, 1, more ? 1 : 0, Z, n, kp, Tsol, V, winv, a, nullptr,
+                           nullptr, nullptr, sc, 0, solc, red, more ? Tc : nullptr));
+      TSVM_CHECK(reduce({RED_SUM}, {SL_AR7B}, zblk));
       hipLaunchKernelGGL(k_border_lin, eg, eb, 0, st, n, corr, (const double*)(solp + n),
-                         (const double*)solc, sc, sol3);
+                         (const double*)solc, sc, sol3, kp, (const double*)(more ? Tc : nullptr),
+                         (const double*)Tb, T3);
     }
     return TSVM_OK;
   };
@@ -826,11 +1265,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     // ---- predictor (1727-1748)
     hipLaunchKernelGGL(k_rhs_pred, eg, eb, 0, st, n, C, (const double*)x, (const double*)dx,
                        (const double*)ds, (const double*)dxi, (const double*)a, rhs, n);
-    TSVM_CHECK(smw(rhs, 2, solp));
-    hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 2, (const double*)a, (const double*)solp, n, red);
-    TSVM_CHECK(reduce({RED_SUM, RED_SUM}, {SL_AR7, SL_AB}));
+    hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, 2, (const double*)rhs, n, (const double*)winv, V, n);
+    TSVM_CHECK(smw(2, solp, Tr7, {SL_AR7, SL_AB}));  // r7 = S^-1 r5, b = S^-1 a
     hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 0, (const double*)(solp + n), (const double*)solp,
-                       sc, sol3);
+                       sc, sol3, kp, (const double*)(border ? Tr7 : nullptr), (const double*)Tb, T3);
     TSVM_CHECK(border_refine(0));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 0, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
@@ -843,11 +1281,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     // ---- corrector (1750-1777)
     hipLaunchKernelGGL(k_rhs_corr, eg, eb, 0, st, n, C, (const double*)x, dx, ds, dxi, adx, ads, adxi,
                        (const double*)sc, rhs);
-    TSVM_CHECK(smw(rhs, 1, solc));
-    hipLaunchKernelGGL(k_dots, eg, eb, 0, st, n, 1, (const double*)a, (const double*)solc, n, red);
-    TSVM_CHECK(reduce({RED_SUM}, {SL_AR7C}));
+    hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, 1, (const double*)rhs, n, (const double*)winv, V, n);
+    TSVM_CHECK(smw(1, solc, border ? Tc : nullptr, {SL_AR7C}));
     hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 1, (const double*)(solp + n), (const double*)solc,
-                       sc, sol3);
+                       sc, sol3, kp, (const double*)(border ? Tc : nullptr), (const double*)Tb, T3);
     TSVM_CHECK(border_refine(1));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 1, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
