@@ -2,12 +2,13 @@
 // the GPU (replaces LAPACKE_dpotrf, tensorsvm-train.cu:1709, and LAPACKE_dpotrs, 1834).
 //
 // Blocked right-looking factorisation with 32-column panels; per panel three launches:
-//   chol_diag    one block factors the 32 x 32 diagonal block in LDS and inverts it
-//   chol_panel   rows below: L_i = A_i L_jj^{-T} (one thread per row)
+//   chol_diag    one wave factors the 32 x 32 diagonal block in registers and inverts it
+//   chol_panel   rows below: L_i = A_i L_jj^{-T} (one thread per entry)
 //   chol_update  trailing lower triangle A -= L_p L_p^T (one block per 32 x 32 tile)
 // A failure (non-positive pivot, as dpotrf's info > 0) is recorded in a device int and
 // every later launch (and chol_solve_f64) becomes a no-op, so the host reads one flag.
-// The inverted diagonal blocks are kept and reused by the triangular solves.
+// The inverted diagonal blocks and a transposed copy of the factor are kept for the
+// triangular solves (chol_solve_f64 follows chol_f64 on the same context).
 #include "common.h"
 #include "internal.h"
 
@@ -15,84 +16,95 @@ namespace tsvm {
 
 constexpr int CB = 32;
 
-__global__ __launch_bounds__(256) void chol_diag(double* __restrict__ M, int64_t k, int64_t j0,
-                                                 double* __restrict__ Dinv, int* __restrict__ info) {
-  if (*info != 0) return;
-  __shared__ double A[CB][CB + 1];
-  __shared__ double Li[CB][CB + 1];
-  __shared__ int bad;
-  const int tid = threadIdx.x;
-  const int jb = (int)(k - j0 < CB ? k - j0 : CB);
-  if (tid == 0) bad = 0;
-  for (int e = tid; e < CB * CB; e += 256) {
-    int i = e / CB, j = e % CB;
-    A[i][j] = (i < jb && j < jb) ? M[(j0 + i) * k + j0 + j] : (i == j ? 1.0 : 0.0);
-    Li[i][j] = 0.0;
-  }
-  __syncthreads();
-  for (int j = 0; j < jb; j++) {
-    if (tid == 0) {
-      double s = A[j][j];
-      if (!(s > 0.0)) {
-        bad = j + 1;
-        s = 1.0;
-      }
-      A[j][j] = sqrt(s);
-    }
-    __syncthreads();
-    if (bad) break;
-    if (tid > j && tid < jb) A[tid][j] /= A[j][j];
-    __syncthreads();
-    for (int e = tid; e < CB * CB; e += 256) {
-      int i = e / CB, l = e % CB;
-      if (i > j && l > j && l <= i && i < jb) A[i][l] -= A[i][j] * A[l][j];
-    }
-    __syncthreads();
-  }
-  if (bad) {
-    if (tid == 0) *info = (int)(j0 + bad);
-    return;
-  }
-  // inverse of the lower-triangular block: thread c solves column c
-  if (tid < CB) {
-    const int c = tid;
-    for (int i = 0; i < CB; i++) {
-      if (i < c) continue;
-      double v = (i == c) ? 1.0 : 0.0;
-      for (int t = c; t < i; t++) v -= A[i][t] * Li[t][c];
-      Li[i][c] = v / A[i][i];
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < CB * CB; e += 256) {
-    int i = e / CB, j = e % CB;
-    if (i < jb && j < jb) M[(j0 + i) * k + j0 + j] = (j <= i) ? A[i][j] : 0.0;
-    Dinv[(j0 / CB) * CB * CB + i * CB + j] = (j <= i) ? Li[i][j] : 0.0;
-  }
+__device__ inline double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// One wave factors the 32 x 32 diagonal block in registers (lane i < 32 holds row i) and
+// inverts the factor (lane c < 32 then holds column c of the inverse); the entries
+// another lane needs come by readlane, so there is no barrier.  Rows past the matrix
+// end are identity.
+__global__ __launch_bounds__(64) void chol_diag(double* __restrict__ M, int64_t k, int64_t j0,
+                                                double* __restrict__ Dinv, int* __restrict__ info) {
+  if (*info != 0) return;
+  const int lane = threadIdx.x;
+  const int jb = (int)(k - j0 < CB ? k - j0 : CB);
+  const bool row = lane < jb;
+  double a[CB];
+#pragma unroll
+  for (int t = 0; t < CB; t++)
+    a[t] = row && t < jb ? M[(j0 + lane) * k + j0 + t] : (lane == t ? 1.0 : 0.0);
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < CB; j++) {
+    const double d = readlane_f64(a[j], j);
+    if (!(d > 0.0)) {
+      bad = j + 1;
+      break;
+    }
+    const double s = sqrt(d);
+    a[j] = lane == j ? s : (lane > j ? a[j] / s : a[j]);
+#pragma unroll
+    for (int l = j + 1; l < CB; l++) {
+      const double llj = readlane_f64(a[j], l);
+      if (lane >= l) a[l] -= a[j] * llj;
+    }
+  }
+  if (bad) {
+    if (lane == 0) *info = (int)(j0 + bad);
+    return;
+  }
+  // column c of the inverse: Li[i][c] = (e_c[i] - sum_{c <= t < i} L[i][t] Li[t][c]) / L[i][i],
+  // lane c walks down its column; the factor and the inverse sit in LDS
+  __shared__ double Ls[CB][CB + 1];
+  __shared__ double Is[CB][CB + 1];
+  if (lane < CB)
+#pragma unroll
+    for (int t = 0; t < CB; t++) {
+      Ls[lane][t] = a[t];
+      Is[t][lane] = 0.0;
+    }
+  __syncthreads();
+  if (lane < CB) {
+    const int c = lane;
+#pragma unroll 1
+    for (int i = c; i < CB; i++) {
+      double v = i == c ? 1.0 : 0.0;
+      for (int t = c; t < i; t++) v -= Ls[i][t] * Is[t][c];
+      Is[i][c] = v / Ls[i][i];
+    }
+  }
+  if (row)
+#pragma unroll
+    for (int t = 0; t < CB; t++)
+      if (t < jb) M[(j0 + lane) * k + j0 + t] = t <= lane ? a[t] : 0.0;
+  if (lane < CB)
+    for (int i = 0; i < CB; i++) Dinv[(j0 / CB) * CB * CB + i * CB + lane] = Is[i][lane];
+}
+
+// rows below the diagonal block: L_i[c] = sum_{t <= c} A_i[t] Linv[c][t]; a block takes 8
+// rows, one thread per (row, column)
 __global__ __launch_bounds__(256) void chol_panel(double* __restrict__ M, int64_t k, int64_t j0,
                                                   const double* __restrict__ Dinv,
                                                   const int* __restrict__ info) {
   if (*info != 0) return;
   __shared__ double Li[CB][CB + 1];
+  __shared__ double Ar[8][CB + 1];
   const double* D = Dinv + (j0 / CB) * CB * CB;
   for (int e = threadIdx.x; e < CB * CB; e += 256) Li[e / CB][e % CB] = D[e];
+  const int r = threadIdx.x / CB, c = threadIdx.x % CB;
+  const int64_t i = j0 + CB + blockIdx.x * 8 + r;
+  Ar[r][c] = i < k ? M[i * k + j0 + c] : 0.0;
   __syncthreads();
-  const int64_t i = j0 + CB + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= k) return;
-  double a[CB];
-  double* row = M + i * k + j0;
+  double v = 0.0;
 #pragma unroll
-  for (int t = 0; t < CB; t++) a[t] = row[t];
-  // L_i[c] = sum_{t <= c} A_i[t] Linv[c][t]
-#pragma unroll
-  for (int c = 0; c < CB; c++) {
-    double v = 0.0;
-#pragma unroll
-    for (int t = 0; t <= c; t++) v += a[t] * Li[c][t];
-    row[c] = v;
-  }
+  for (int t = 0; t < CB; t++)
+    if (t <= c) v += Ar[r][t] * Li[c][t];
+  M[i * k + j0 + c] = v;
 }
 
 __global__ __launch_bounds__(256) void chol_update(double* __restrict__ M, int64_t k, int64_t j0,
@@ -121,12 +133,28 @@ __global__ __launch_bounds__(256) void chol_update(double* __restrict__ M, int64
   }
 }
 
-__global__ void zero_upper(double* __restrict__ M, int64_t k, const int* __restrict__ info) {
+// zero the strict upper triangle and write Lt = L^T (32 x 32 tiles through LDS)
+__global__ __launch_bounds__(256) void chol_finish(double* __restrict__ M, int64_t k,
+                                                   double* __restrict__ Lt,
+                                                   const int* __restrict__ info) {
   if (*info != 0) return;
-  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (idx >= k * k) return;
-  int64_t i = idx / k, j = idx - i * k;
-  if (j > i) M[idx] = 0.0;
+  __shared__ double tile[CB][CB + 1];
+  const int64_t i0 = (int64_t)blockIdx.y * CB, j0 = (int64_t)blockIdx.x * CB;
+  for (int e = threadIdx.x; e < CB * CB; e += 256) {
+    const int r = e / CB, c = e % CB;
+    const int64_t i = i0 + r, j = j0 + c;
+    double v = 0.0;
+    if (i < k && j < k) {
+      v = j <= i ? M[i * k + j] : 0.0;
+      if (j > i) M[i * k + j] = 0.0;
+    }
+    tile[r][c] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < CB * CB; e += 256) {
+    const int r = e / CB, c = e % CB;  // Lt[j0 + r][i0 + c] = L[i0 + c][j0 + r]
+    if (j0 + r < k && i0 + c < k) Lt[(j0 + r) * k + i0 + c] = tile[c][r];
+  }
 }
 
 __global__ void set_int(int* p, int v) { *p = v; }
@@ -137,122 +165,105 @@ tsvm_status chol_f64(Ctx* c, double* M, int64_t k, int* info) {
   hipStream_t s = c->stream;
   hipLaunchKernelGGL(set_int, dim3(1), dim3(1), 0, s, info, 0);
   for (int64_t j0 = 0; j0 < k; j0 += CB) {
-    hipLaunchKernelGGL(chol_diag, dim3(1), dim3(256), 0, s, M, k, j0, (double*)dinv, info);
+    hipLaunchKernelGGL(chol_diag, dim3(1), dim3(64), 0, s, M, k, j0, (double*)dinv, info);
     const int64_t rest = k - j0 - CB;
     if (rest <= 0) break;
-    hipLaunchKernelGGL(chol_panel, dim3((unsigned)ceil_div(rest, 256)), dim3(256), 0, s, M, k, j0,
+    hipLaunchKernelGGL(chol_panel, dim3((unsigned)ceil_div(rest, 8)), dim3(256), 0, s, M, k, j0,
                        (const double*)dinv, (const int*)info);
     const unsigned T = (unsigned)ceil_div(rest, CB);
     hipLaunchKernelGGL(chol_update, dim3(T, T), dim3(256), 0, s, M, k, j0, (const int*)info);
   }
-  hipLaunchKernelGGL(zero_upper, dim3((unsigned)ceil_div(k * k, 256)), dim3(256), 0, s, M, k,
-                     (const int*)info);
+  void* lt;
+  TSVM_HIP(c, c->ws.get("chol_lt", (size_t)(k * k * 8), &lt));
+  const unsigned T = (unsigned)ceil_div(k, CB);
+  hipLaunchKernelGGL(chol_finish, dim3(T, T), dim3(256), 0, s, M, k, (double*)lt, (const int*)info);
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
 }
 
-// (L L^T) X = B for p <= 4 right-hand sides (rows of B), one block of 256 threads.
-// Forward: y_b = Linv_bb (b_b - L_{b,<b} y_{<b}); backward: x_b = Linv_bb^T (y_b - L_{>b,b}^T x_{>b}).
-__global__ __launch_bounds__(256) void chol_solve_kernel(const double* __restrict__ L, int64_t k,
-                                                         const double* __restrict__ Dinv,
-                                                         double* __restrict__ B, int p,
-                                                         const int* __restrict__ info) {
+// (L L^T) X = B for p <= 4 right-hand sides (rows of B) in one block of 1024 threads,
+// right-looking:
+//   forward, block b ascending:   y_b = Linv_bb x_b, then x_r -= L_rb y_b for every row below
+//   backward, block b descending: x_b = Linv_bb^T y_b, then x_c -= L_bc^T x_b for every
+//                                 column before
+// At the start of a step every thread issues its loads at once (one entry of Linv_bb
+// into LDS, the 32 entries of L its row or column needs, read from Lt = L^T in the
+// forward sweep so that a wave's loads are coalesced), so a step costs one round of load
+// latency.  Needs k - 32 <= 1024 (one row or column per thread).
+constexpr int SOLVE_THREADS = 1024;
+template <int p>
+__global__ __launch_bounds__(SOLVE_THREADS) void chol_solve_kernel(
+    const double* __restrict__ L, const double* __restrict__ Lt, int64_t k,
+    const double* __restrict__ Dinv, double* __restrict__ B, const int* __restrict__ info) {
   if (*info != 0) return;
   extern __shared__ double smem[];
-  double* X = smem;                 // p x k
-  double* part = smem + 4 * k;      // [8][CB][4] partial sums
-  double* tmp = part + 8 * CB * 4;  // [CB][4]
+  double* X = smem;              // p x k
+  double* Y = smem + 4 * k;      // [4][CB] the current block
+  double* Db = Y + 4 * CB;       // [CB][CB] Linv_bb
   const int tid = threadIdx.x;
-  const int r = tid % CB, lane8 = tid / CB;  // 32 rows x 8 partial lanes
-  for (int64_t e = tid; e < (int64_t)p * k; e += 256) X[e] = B[e];
-  __syncthreads();
+  const int q = tid / CB, rr = tid % CB;  // the block solve: q < p, rr < jb
+  for (int64_t e = tid; e < (int64_t)p * k; e += SOLVE_THREADS) X[e] = B[e];
   const int64_t nb = ceil_div(k, CB);
-  // forward
+  double l[CB];
   for (int64_t b = 0; b < nb; b++) {
     const int64_t j0 = b * CB;
     const int jb = (int)(k - j0 < CB ? k - j0 : CB);
-    // forward: 8 consecutive threads share a row of L (coalesced along t)
-    const int fr = tid >> 3, fg = tid & 7;
-    double acc[4] = {0, 0, 0, 0};
-    if (fr < jb) {
-      const double* Lr = L + (j0 + fr) * k;
-      int64_t t = fg;
-      for (; t + 24 < j0; t += 32) {
-        double l0 = Lr[t], l1 = Lr[t + 8], l2 = Lr[t + 16], l3 = Lr[t + 24];
-        for (int q = 0; q < p; q++)
-          acc[q] += l0 * X[q * k + t] + l1 * X[q * k + t + 8] + l2 * X[q * k + t + 16] +
-                    l3 * X[q * k + t + 24];
-      }
-      for (; t < j0; t += 8) {
-        double l = Lr[t];
-        for (int q = 0; q < p; q++) acc[q] += l * X[q * k + t];
-      }
-    }
-    for (int q = 0; q < 4; q++) part[(fg * CB + fr) * 4 + q] = acc[q];
+    const int64_t r = j0 + jb + tid;  // this thread's row below the block
+    Db[tid] = Dinv[b * CB * CB + tid];
+    if (r < k)
+#pragma unroll
+      for (int t = 0; t < CB; t++) l[t] = t < jb ? Lt[(j0 + t) * k + r] : 0.0;
     __syncthreads();
-    if (tid < CB * 4) {
-      int rr = tid / 4, q = tid % 4;
-      if (rr < jb && q < p) {
-        double s = 0.0;
-        for (int u = 0; u < 8; u++) s += part[(u * CB + rr) * 4 + q];
-        tmp[rr * 4 + q] = X[q * k + j0 + rr] - s;
-      }
+    if (q < p && rr < jb) {
+      double s = 0.0;
+#pragma unroll
+      for (int t = 0; t < CB; t++)
+        if (t <= rr) s += Db[rr * CB + t] * X[q * k + j0 + t];
+      Y[q * CB + rr] = s;
     }
     __syncthreads();
-    if (tid < CB * 4) {
-      int rr = tid / 4, q = tid % 4;
-      if (rr < jb && q < p) {
-        const double* Di = Dinv + b * CB * CB + rr * CB;
-        double s = 0.0;
-        for (int t = 0; t <= rr; t++) s += Di[t] * tmp[t * 4 + q];
-        X[q * k + j0 + rr] = s;
-      }
+    if (q < p && rr < jb) X[q * k + j0 + rr] = Y[q * CB + rr];
+    if (r < k) {
+      double acc[p] = {};
+#pragma unroll
+      for (int t = 0; t < CB; t++)
+#pragma unroll
+        for (int qq = 0; qq < p; qq++) acc[qq] += l[t] * Y[qq * CB + t];
+#pragma unroll
+      for (int qq = 0; qq < p; qq++) X[qq * k + r] -= acc[qq];
     }
     __syncthreads();
   }
-  // backward
   for (int64_t b = nb - 1; b >= 0; b--) {
     const int64_t j0 = b * CB;
     const int jb = (int)(k - j0 < CB ? k - j0 : CB);
-    double acc[4] = {0, 0, 0, 0};
-    if (r < jb) {
-      int64_t t = j0 + CB + lane8;
-      for (; t + 24 < k; t += 32) {
-        double l0 = L[t * k + j0 + r], l1 = L[(t + 8) * k + j0 + r];
-        double l2 = L[(t + 16) * k + j0 + r], l3 = L[(t + 24) * k + j0 + r];
-        for (int q = 0; q < p; q++)
-          acc[q] += l0 * X[q * k + t] + l1 * X[q * k + t + 8] + l2 * X[q * k + t + 16] +
-                    l3 * X[q * k + t + 24];
-      }
-      for (; t < k; t += 8) {
-        double l = L[t * k + j0 + r];
-        for (int q = 0; q < p; q++) acc[q] += l * X[q * k + t];
-      }
-    }
-    for (int q = 0; q < 4; q++) part[(lane8 * CB + r) * 4 + q] = acc[q];
+    const int64_t cc = tid;  // this thread's column before the block
+    Db[tid] = Dinv[b * CB * CB + tid];
+    if (cc < j0)
+#pragma unroll
+      for (int t = 0; t < CB; t++) l[t] = t < jb ? L[(j0 + t) * k + cc] : 0.0;
     __syncthreads();
-    if (tid < CB * 4) {
-      int rr = tid / 4, q = tid % 4;
-      if (rr < jb && q < p) {
-        double s = 0.0;
-        for (int u = 0; u < 8; u++) s += part[(u * CB + rr) * 4 + q];
-        tmp[rr * 4 + q] = X[q * k + j0 + rr] - s;
-      }
+    if (q < p && rr < jb) {
+      double s = 0.0;
+#pragma unroll
+      for (int t = 0; t < CB; t++)
+        if (t >= rr && t < jb) s += Db[t * CB + rr] * X[q * k + j0 + t];
+      Y[q * CB + rr] = s;
     }
     __syncthreads();
-    if (tid < CB * 4) {
-      int rr = tid / 4, q = tid % 4;
-      if (rr < jb && q < p) {
-        // x_rr = sum_{t >= rr} Linv[t][rr] tmp[t]
-        const double* Db = Dinv + b * CB * CB;
-        double s = 0.0;
-        for (int t = rr; t < jb; t++) s += Db[t * CB + rr] * tmp[t * 4 + q];
-        X[q * k + j0 + rr] = s;
-      }
+    if (q < p && rr < jb) X[q * k + j0 + rr] = Y[q * CB + rr];
+    if (cc < j0) {
+      double acc[p] = {};
+#pragma unroll
+      for (int t = 0; t < CB; t++)
+#pragma unroll
+        for (int qq = 0; qq < p; qq++) acc[qq] += l[t] * Y[qq * CB + t];
+#pragma unroll
+      for (int qq = 0; qq < p; qq++) X[qq * k + cc] -= acc[qq];
     }
     __syncthreads();
   }
-  for (int64_t e = tid; e < (int64_t)p * k; e += 256) B[e] = X[e];
+  for (int64_t e = tid; e < (int64_t)p * k; e += SOLVE_THREADS) B[e] = X[e];
 }
 
 tsvm_status chol_solve_f64(Ctx* c, const double* L, int64_t k, double* B, int64_t p,
@@ -260,10 +271,19 @@ tsvm_status chol_solve_f64(Ctx* c, const double* L, int64_t k, double* B, int64_
   if (p < 1 || p > 4) return fail(c, TSVM_ERR_ARG, "chol_solve: 1 <= p <= 4");
   void* dinv;
   TSVM_HIP(c, c->ws.get("chol_dinv", (size_t)(ceil_div(k, CB) * CB * CB * 8), &dinv));
-  size_t sm = (size_t)(4 * k + 8 * CB * 4 + CB * 4) * 8;
-  if (sm > 160 * 1024) return fail(c, TSVM_ERR_ARG, "chol_solve: k=%ld too large", (long)k);
-  hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(256), sm, c->stream, L, k,
-                     (const double*)dinv, B, (int)p, info);
+  if (k - CB > SOLVE_THREADS) return fail(c, TSVM_ERR_ARG, "chol_solve: k=%ld too large", (long)k);
+  const size_t sm = (size_t)(4 * k + 4 * CB + CB * CB) * 8;
+  void* lt;
+  TSVM_HIP(c, c->ws.get("chol_lt", (size_t)(k * k * 8), &lt));
+  const double* di = (const double*)dinv;
+  const double* Lt = (const double*)lt;
+  const dim3 g(1), b(SOLVE_THREADS);
+  switch (p) {
+    case 1: hipLaunchKernelGGL(chol_solve_kernel<1>, g, b, sm, c->stream, L, Lt, k, di, B, info); break;
+    case 2: hipLaunchKernelGGL(chol_solve_kernel<2>, g, b, sm, c->stream, L, Lt, k, di, B, info); break;
+    case 3: hipLaunchKernelGGL(chol_solve_kernel<3>, g, b, sm, c->stream, L, Lt, k, di, B, info); break;
+    default: hipLaunchKernelGGL(chol_solve_kernel<4>, g, b, sm, c->stream, L, Lt, k, di, B, info); break;
+  }
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
 }

@@ -113,7 +113,9 @@ tsvm_status gemm_nn_tall(Ctx* c, const float* A, int64_t lda, int64_t ka, const 
 // triangle is zeroed.  info_dev (device int) receives 0 or the 1-based failing column.
 tsvm_status chol_f64(Ctx* c, double* M, int64_t k, int* info_dev);
 // Solve (L L^T) X = B for p right-hand sides; B is p x k row-major (one RHS per row),
-// overwritten by X.  Skipped (B unchanged) when *info_dev != 0.
+// overwritten by X.  Skipped (B unchanged) when *info_dev != 0.  Uses the inverted
+// diagonal blocks and the transposed factor that the last chol_f64 on the context left
+// in its workspace.
 tsvm_status chol_solve_f64(Ctx* c, const double* L, int64_t k, double* B, int64_t p,
                            const int* info_dev);
 
