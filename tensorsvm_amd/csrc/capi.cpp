@@ -72,6 +72,7 @@ tsvm_status tsvm_create(int device, tsvm_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) x->c.cu_count = prop.multiProcessorCount;
+  tsvm::warm_sym_eig(&x->c);
   *out = x;
   return TSVM_OK;
 }

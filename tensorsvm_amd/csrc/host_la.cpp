@@ -92,45 +92,57 @@ void jacobi_eig(const double* A0, int64_t k, double* w, double* V) {
 const char* host_lapack_name() { return "rocsolver_dsyevd"; }
 
 // Symmetric eigen-decomposition of the k x k core matrix on the GPU (rocSOLVER dsyevd on
-// the context's stream); the host Jacobi solver is the fallback if the library call fails.
-tsvm_status host_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* V) {
+// the context's stream); false if the library call fails.
+static bool gpu_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* V) {
   if (!c->rocblas) {
     rocblas_handle h;
-    if (rocblas_create_handle(&h) == rocblas_status_success) {
-      c->rocblas = (void*)h;
-      rocblas_set_stream(h, c->stream);
-    }
+    if (rocblas_create_handle(&h) != rocblas_status_success) return false;
+    c->rocblas = (void*)h;
+    rocblas_set_stream(h, c->stream);
   }
-  if (c->rocblas) {
-    void *dA, *dD, *dE, *dI;
-    if (c->ws.get("eig_A", (size_t)(k * k * 8), &dA) == hipSuccess &&
-        c->ws.get("eig_D", (size_t)(k * 8), &dD) == hipSuccess &&
-        c->ws.get("eig_E", (size_t)(k * 8), &dE) == hipSuccess &&
-        c->ws.get("eig_I", 16, &dI) == hipSuccess) {
-      // a symmetric row-major matrix is its own column-major form
-      int info = -1;
-      bool ok = hipMemcpyAsync(dA, A, k * k * 8, hipMemcpyHostToDevice, c->stream) == hipSuccess &&
-                rocsolver_dsyevd((rocblas_handle)c->rocblas, rocblas_evect_original,
-                                 rocblas_fill_lower, (rocblas_int)k, (double*)dA, (rocblas_int)k,
-                                 (double*)dD, (double*)dE, (rocblas_int*)dI) == rocblas_status_success &&
-                hipMemcpyAsync(w, dD, k * 8, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
-                hipMemcpyAsync(&info, dI, 4, hipMemcpyDeviceToHost, c->stream) == hipSuccess;
-      std::vector<double> a(k * k);
-      ok = ok && hipMemcpyAsync(a.data(), dA, k * k * 8, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
-           hipStreamSynchronize(c->stream) == hipSuccess;
-      if (ok && info == 0) {
-        // column j of the column-major result = eigenvector j; V[:, j] row-major
-        for (int64_t r = 0; r < k; r++)
-          for (int64_t j = 0; j < k; j++) V[r * k + j] = a[r + j * k];
-        return TSVM_OK;
-      }
-    }
-  }
-  jacobi_eig(A, k, w, V);
+  void *dA, *dD, *dE, *dI;
+  if (c->ws.get("eig_A", (size_t)(k * k * 8), &dA) != hipSuccess ||
+      c->ws.get("eig_D", (size_t)(k * 8), &dD) != hipSuccess ||
+      c->ws.get("eig_E", (size_t)(k * 8), &dE) != hipSuccess ||
+      c->ws.get("eig_I", 16, &dI) != hipSuccess)
+    return false;
+  // a symmetric row-major matrix is its own column-major form
+  int info = -1;
+  bool ok = hipMemcpyAsync(dA, A, k * k * 8, hipMemcpyHostToDevice, c->stream) == hipSuccess &&
+            rocsolver_dsyevd((rocblas_handle)c->rocblas, rocblas_evect_original,
+                             rocblas_fill_lower, (rocblas_int)k, (double*)dA, (rocblas_int)k,
+                             (double*)dD, (double*)dE, (rocblas_int*)dI) == rocblas_status_success &&
+            hipMemcpyAsync(w, dD, k * 8, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+            hipMemcpyAsync(&info, dI, 4, hipMemcpyDeviceToHost, c->stream) == hipSuccess;
+  std::vector<double> a(k * k);
+  ok = ok && hipMemcpyAsync(a.data(), dA, k * k * 8, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+       hipStreamSynchronize(c->stream) == hipSuccess;
+  if (!ok || info != 0) return false;
+  // column j of the column-major result = eigenvector j; V[:, j] row-major
+  for (int64_t r = 0; r < k; r++)
+    for (int64_t j = 0; j < k; j++) V[r * k + j] = a[r + j * k];
+  return true;
+}
+
+// the host Jacobi solver is the fallback if the library call fails
+tsvm_status host_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* V) {
+  if (!gpu_sym_eig(c, A, k, w, V)) jacobi_eig(A, k, w, V);
   return TSVM_OK;
 }
 
-// lower Cholesky in place (row-major, blocked right-looking); 0 or failing column + 1
+// One eigen-decomposition of a 512 x 512 tridiagonal matrix, so that the rocBLAS handle
+// and rocSOLVER's kernels (about 0.17 s the first time in a process) are set up when the
+// context is created rather than inside the first factor construction.
+void warm_sym_eig(Ctx* c) {
+  const int64_t k = 512;
+  std::vector<double> A(k * k, 0.0), w(k), V(k * k);
+  for (int64_t i = 0; i < k; i++) {
+    A[i * k + i] = (double)(i + 1);
+    if (i + 1 < k) A[i * k + i + 1] = A[(i + 1) * k + i] = 0.5;
+  }
+  (void)gpu_sym_eig(c, A.data(), k, w.data(), V.data());
+}
+
 int host_chol(double* M, int64_t k) {
   const int64_t B = 64;
   for (int64_t j0 = 0; j0 < k; j0 += B) {

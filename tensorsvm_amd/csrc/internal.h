@@ -24,7 +24,7 @@ struct Ctx {
   Workspace ws;
   std::string last_error;
   int cu_count = 256;
-  void* rocblas = nullptr;  // rocblas_handle for the k x k eigen-solver, created on first use
+  void* rocblas = nullptr;  // rocblas_handle of the k x k eigen-solver (set up with the context)
   // phase timers of the last tsvm_lra / tsvm_ipm call (seconds)
   double t_phase[16] = {0};
   int progress = 0;  // progress lines on stderr during a training call
@@ -127,6 +127,7 @@ tsvm_status masked_colsum(Ctx* c, const float* U, int64_t n, int64_t k, int64_t 
 // ---- host small dense FP64 linear algebra (host_la.cpp) ---------------------------
 // Symmetric eigen-decomposition (ascending), V column i = eigenvector i, row-major.
 tsvm_status host_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* V);
+void warm_sym_eig(Ctx* c);  // first-use set-up of the eigen-solver (context creation)
 const char* host_lapack_name();
 int host_chol(double* M, int64_t k);  // lower, in place; 0 or failing column + 1
 void host_tri_inv_lower(const double* L, int64_t k, double* Linv);

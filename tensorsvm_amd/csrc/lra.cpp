@@ -184,6 +184,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   TSVM_CHECK(gemm_tn_f64(c, W, k, k, Q, k, k, n, nullptr, false, dG));
   TSVM_HIP(c, hipMemcpyAsync(Cb.data(), dG, k * k * 8, hipMemcpyDeviceToHost, s));
   TSVM_HIP(c, hipStreamSynchronize(s));
+  const double tc0 = now();
   for (int64_t i = 0; i < k; i++)
     for (int64_t j = 0; j < k; j++) Cs[i * k + j] = 0.5 * (Cb[i * k + j] + Cb[j * k + i]);
   // 6. C = L^-1 Cs L^-T, eig, X = L^-T V sqrt(w+)
@@ -201,7 +202,9 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
       Cb[i * k + j] = v;
       Cb[j * k + i] = v;
     }
+  const double tc1 = now();
   TSVM_CHECK(host_sym_eig(c, Cb.data(), k, w.data(), V.data()));
+  const double tc2 = now();
   int64_t realk = k;
   for (int64_t j = 0; j < k; j++) {
     double sc = 0.0;
@@ -216,6 +219,13 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   TSVM_HIP(c, hipMemcpyAsync(dB, T.data(), k * k * 8, hipMemcpyHostToDevice, s));
   double t4 = now();
   inf.t_core = t4 - t3;
+  if (c->progress) {
+    fprintf(stderr,
+            "factor: core %.3f s (C = W^T Q %.3f s, projection %.3f s, eig %.3f s, right "
+            "factor %.3f s)\n",
+            t4 - t3, tc0 - t3, tc1 - tc0, tc2 - tc1, t4 - tc2);
+    fflush(stderr);
+  }
 
   // 7. U = W X
   TSVM_CHECK(gemm_nn_tall(c, W, k, k, dB, k, n, U, k));
