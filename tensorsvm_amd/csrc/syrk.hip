@@ -88,19 +88,30 @@ __global__ __launch_bounds__(256) void hs_hist_kernel(const double* __restrict__
   for (int b = threadIdx.x; b < HS_BINS; b += blockDim.x) part[blockIdx.x * HS_BINS + b] = h[b];
 }
 
-// one block of HS_BINS threads: total counts per bin, then the cutoff bin: the bulk is
-// every bin below `cut`, the largest prefix whose bound sqrt(sum cnt_b 4^(b-63)) stays
-// within the budget.  out: cut (int), slots[0] = bound of the bulk
-__global__ void hs_cutoff_kernel(const int* __restrict__ part, int nblocks, double budget,
-                                 int* __restrict__ cut_out, double* __restrict__ slots) {
+// one block of 8 HS_BINS threads: total counts per bin (8 groups of partials, summed in
+// a fixed order), then the cutoff bin: the bulk is every bin below `cut`, the largest
+// prefix whose bound sqrt(sum cnt_b 4^(b-63)) stays within the budget.  out: cut (int),
+// slots[0] = bound of the bulk
+constexpr int HS_CUT_GROUPS = 8;
+__global__ __launch_bounds__(HS_CUT_GROUPS * HS_BINS) void hs_cutoff_kernel(
+    const int* __restrict__ part, int nblocks, double budget, int* __restrict__ cut_out,
+    double* __restrict__ slots) {
+  __shared__ long long grp[HS_CUT_GROUPS][HS_BINS];
   __shared__ long long cnt[HS_BINS];
-  const int b = threadIdx.x;
+  const int b = threadIdx.x % HS_BINS, g = threadIdx.x / HS_BINS;
   long long s = 0;
-  for (int k = 0; k < nblocks; k++) s += part[k * HS_BINS + b];
-  cnt[b] = s;
+#pragma unroll 8
+  for (int k = g; k < nblocks; k += HS_CUT_GROUPS) s += part[k * HS_BINS + b];
+  grp[g][b] = s;
   __syncthreads();
-  if (b == 0) {
-    double cum = 0.0, kept = 0.0;
+  if (g == 0) {
+    long long t = 0;
+    for (int q = 0; q < HS_CUT_GROUPS; q++) t += grp[q][b];
+    cnt[b] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double cum = 0.0;
     int cut = HS_BINS;
     for (int e = 0; e < HS_BINS; e++) {
       if (cnt[e] == 0) continue;
@@ -111,9 +122,8 @@ __global__ void hs_cutoff_kernel(const int* __restrict__ part, int nblocks, doub
       }
       cum += add;
     }
-    kept = sqrt(cum);
     *cut_out = cut;
-    slots[0] = kept;
+    slots[0] = sqrt(cum);
   }
 }
 
@@ -134,17 +144,28 @@ __global__ __launch_bounds__(256) void hs_count_kernel(const double* __restrict_
   if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
-// one block: exclusive scan of the block counts; slots[1] = number of FP64 rows
-__global__ void hs_scan_kernel(const int* __restrict__ cnt, int nblocks, int* __restrict__ off,
-                               double* __restrict__ slots, int* __restrict__ nh) {
-  if (threadIdx.x != 0) return;
-  int64_t s = 0;
-  for (int b = 0; b < nblocks; b++) {
-    off[b] = (int)s;
-    s += cnt[b];
+// one block of HS_BLOCKS threads: exclusive scan of the block counts (Hillis-Steele in
+// LDS); slots[1] = number of FP64 rows
+__global__ __launch_bounds__(HS_BLOCKS) void hs_scan_kernel(const int* __restrict__ cnt,
+                                                            int nblocks, int* __restrict__ off,
+                                                            double* __restrict__ slots,
+                                                            int* __restrict__ nh) {
+  __shared__ int v[2][HS_BLOCKS];
+  const int t = threadIdx.x;
+  v[0][t] = t < nblocks ? cnt[t] : 0;
+  __syncthreads();
+  int cur = 0;
+  for (int o = 1; o < HS_BLOCKS; o <<= 1) {
+    v[cur ^ 1][t] = v[cur][t] + (t >= o ? v[cur][t - o] : 0);
+    cur ^= 1;
+    __syncthreads();
   }
-  slots[1] = (double)s;
-  *nh = (int)s;
+  // v[cur] holds inclusive sums
+  if (t < nblocks) off[t] = v[cur][t] - (t < nblocks ? cnt[t] : 0);
+  if (t == HS_BLOCKS - 1) {
+    slots[1] = (double)v[cur][t];
+    *nh = v[cur][t];
+  }
 }
 
 // rows of the FP64 part in increasing order; wb = FP32 weight of the bulk rows, 0 for
@@ -517,11 +538,11 @@ tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double bu
   const int64_t rpb = round_up(ceil_div(hs->n_pad, HS_BLOCKS), 256);
   const int nb = (int)ceil_div(hs->n_pad, rpb);
   hipLaunchKernelGGL(hs_hist_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb, part);
-  hipLaunchKernelGGL(hs_cutoff_kernel, dim3(1), dim3(HS_BINS), 0, s, (const int*)part, nb, budget,
+  hipLaunchKernelGGL(hs_cutoff_kernel, dim3(1), dim3(HS_CUT_GROUPS * HS_BINS), 0, s, (const int*)part, nb, budget,
                      cut, slots);
   hipLaunchKernelGGL(hs_count_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb,
                      (const int*)cut, cnt);
-  hipLaunchKernelGGL(hs_scan_kernel, dim3(1), dim3(64), 0, s, (const int*)cnt, nb, off, slots, nh);
+  hipLaunchKernelGGL(hs_scan_kernel, dim3(1), dim3(HS_BLOCKS), 0, s, (const int*)cnt, nb, off, slots, nh);
   hipLaunchKernelGGL(hs_scatter_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, hs->n_pad, rpb,
                      (const int*)cut, (const int*)off, hs->rows, hs->wb);
   TSVM_HIP(c, hipGetLastError());

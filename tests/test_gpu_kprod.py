@@ -81,6 +81,24 @@ def test_kprod_fp16x2(ctx, m, n, d, r):
     assert np.median(err) < 0.35 * np.median(np.abs(Wb - ref))
 
 
+@pytest.mark.parametrize("m,n,d,r", [(300, 257, 18, 40), (1000, 1000, 18, 100), (513, 700, 54, 100),
+                                     (1100, 3001, 18, 300), (129, 64, 32, 256), (77, 33, 5, 7),
+                                     (2048, 2048, 18, 512), (40, 5000, 18, 128)])
+def test_kprod_fp16(ctx, m, n, d, r):
+    """Plain fp16 form (the factor products of the BF16 training form): kernel values and
+    the scaled right-hand side each rounded once to fp16 (relative 2^-11), distance as in
+    BF16; ragged row, column and right-hand-side counts, several column tiles."""
+    import tensorsvm_amd as T
+    X1, y1, X2, y2, A = _case(m, n, d, r, 0.05, m + n + 3)
+    W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, T.PREC_FP16)
+    ref = O.kernel_product(X1, y1, X2, y2, A, 0.05)
+    Kabs = O.kernel_product(X1, np.ones_like(y1), X2, np.ones_like(y2), np.abs(A), 0.05)
+    err = np.abs(W - ref)
+    bound = (2.0 ** -10 + 2.0 ** -13) * Kabs
+    assert (err <= 1.05 * bound + 1e-6 * np.abs(ref).max()).all()
+    assert np.median(err / (bound + 1e-30)) < 0.3
+
+
 def test_kprod_fp16x2_exact_rhs(ctx):
     """With A = I (exact in fp16) every output is one labelled kernel value, so only the
     split of E and the distance remain: the per-entry error must sit well below one fp16
