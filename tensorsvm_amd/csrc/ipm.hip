@@ -135,11 +135,18 @@ static tsvm_status reduce_parts(Ctx* c, const double* part, int nblk, int m, dou
 
 
 // Fused passes over Z (defined below): modes and launcher.
-enum { ZP_SMW = 0, ZP_BORDER = 1, ZP_T = 2, ZP_N = 3 };
+enum { ZP_SMW = 0, ZP_BORDER = 1, ZP_T = 2, ZP_N = 3, ZP_RES = 4 };
+// operands and outputs of the fused residual pass (ZP_RES)
+struct ResArgs {
+  const double *x, *s, *xi;
+  double *dx, *ds, *dxi, *D, *winv, *rhs, *V;
+  double C;
+};
 static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int64_t n, int kp,
                              const double* T, const double* V, const double* winv,
                              const double* a, const double* D, const double* r5, const double* dx,
-                             const double* sc, int corr, double* out, double* red, double* Tout);
+                             const double* sc, int corr, double* out, double* red, double* Tout,
+                             const ResArgs* ra = nullptr);
 
 // ------------------------------------------------------------------------------------
 // Z^T V: block b owns a row range; thread t handles float4 column group t % (kp/4) of
@@ -343,7 +350,7 @@ __global__ __launch_bounds__(256) void zpass_kernel(
     const double* __restrict__ V, const double* __restrict__ winv, const double* __restrict__ a,
     const double* __restrict__ D, const double* __restrict__ r5, const double* __restrict__ dxv,
     const double* __restrict__ sc, int corr, double* __restrict__ out, double* __restrict__ red,
-    double* __restrict__ part) {
+    double* __restrict__ part, const ResArgs ra) {
   extern __shared__ double shz[];  // 2 x Q x kp: waves 0, 1 store, waves 2, 3 add
   constexpr int R = MODE == ZP_SMW ? P : 3;
   constexpr int QA = Q > 0 ? Q : 1;
@@ -531,23 +538,29 @@ __device__ inline double swap_sum16(double a, double b) {
 // ZP_BORDER:
 //   ZP_T       Z^T V_q for q < P (Q = P), no products
 //   ZP_N       out = Z T (P = 1, Q = 0)
+//   ZP_RES     the residual pass with T = Z^T x (1606-1643) fused with the predictor's
+//              right-hand sides (1805-1808), as k_resid + k_rhs_pred + k_scale: e = Z T,
+//              r = e - a y - s + xi - 1, ds = -s x, dxi = -xi (C - x), D, 1/D and the
+//              nine stop-test reductions, r5 = r - ds/x + dxi/(C - x), then V = [r5, a]/D
+//              and Z^T V (P = 1, Q = 2)
 template <int MODE, int P, int Q, int F>
 __global__ __launch_bounds__(256) void zpass_wide_kernel(
     const float* __restrict__ Z, int64_t n, int kp, const double* __restrict__ T,
     const double* __restrict__ V, const double* __restrict__ winv, const double* __restrict__ a,
     const double* __restrict__ D, const double* __restrict__ r5, const double* __restrict__ dxv,
     const double* __restrict__ sc, int corr, double* __restrict__ out, double* __restrict__ red,
-    double* __restrict__ part) {
+    double* __restrict__ part, const ResArgs ra) {
   extern __shared__ double shz[];  // 2 x Q x kp: waves 0, 1 store, waves 2, 3 add
   constexpr bool DOT = MODE != ZP_T;
-  constexpr bool RED = MODE == ZP_SMW || MODE == ZP_BORDER;
-  constexpr int R = MODE == ZP_BORDER ? 3 : P;
+  constexpr bool RED = MODE == ZP_SMW || MODE == ZP_BORDER || MODE == ZP_RES;
+  constexpr int R = MODE == ZP_BORDER ? 3 : (MODE == ZP_RES ? 9 : P);
   constexpr int QA = Q > 0 ? Q : 1;
   constexpr int M = P * ZP_U;  // 4 or 8 products per step
   constexpr int LOGM = M == 4 ? 2 : 3;
   constexpr int GL = 64 >> LOGM;  // lanes per product after the reduction
-  constexpr int NO = MODE == ZP_BORDER ? 5 : (MODE == ZP_SMW ? 3 : 1);
+  constexpr int NO = MODE == ZP_BORDER ? 5 : (MODE == ZP_SMW ? 3 : (MODE == ZP_RES ? 4 : 1));
   static_assert(M == 4 || M == 8, "P * ZP_U must be 4 or 8");
+  static_assert(MODE != ZP_RES || (P == 1 && Q == 2), "the residual pass is P = 1, Q = 2");
   const int G = kp / 4;
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -576,7 +589,14 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
     rv[r] = 0.0;
     op[r] = RED_SUM;
   }
+  if constexpr (MODE == ZP_RES) {
+    rv[7] = -INFINITY;
+    op[7] = RED_MAX;
+    rv[8] = INFINITY;
+    op[8] = RED_MIN;
+  }
   const double dy = MODE == ZP_BORDER ? sc[corr ? SL_DYC : SL_DY] : 0.0;
+  const double yeq = MODE == ZP_RES ? sc[SL_Y] : 0.0;
   const int64_t wave = (int64_t)blockIdx.x * 4 + wid;
   const int64_t nw = (int64_t)gridDim.x * 4;
   const int64_t chunk = ceil_div(ceil_div(n, nw), ZP_U) * ZP_U;
@@ -607,8 +627,13 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
       o[2] = a[rl];
     } else if constexpr (MODE == ZP_T) {
       o[0] = V[ql * n + rl];
+    } else if constexpr (MODE == ZP_RES) {
+      o[0] = ra.x[rl];
+      o[1] = ra.s[rl];
+      o[2] = ra.xi[rl];
+      o[3] = a[rl];
     }
-    double w;  // this lane's product after the row-wise update
+    double w, w1 = 0.0;  // this lane's product(s) after the row-wise update
     if constexpr (DOT) {
       double v[M];
 #pragma unroll
@@ -652,6 +677,35 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
           rv[1] += r * r;
           rv[2] += o[4] * o[4];
         }
+      } else if constexpr (MODE == ZP_RES) {
+        const double xv = o[0], sv = o[1], xiv = o[2], av = o[3], ev = v[0];
+        const double cx = ra.C - xv;
+        const double r = ev - av * yeq - sv + xiv - 1.0;
+        const double dsv = -sv * xv, dxiv = -xiv * cx;
+        const double Dv = sv / xv + xiv / cx;
+        const double wv = 1.0 / Dv;
+        const double r5v = r - dsv / xv + dxiv / cx;
+        w = r5v * wv;
+        w1 = av * wv;
+        if (ok && lead) {
+          ra.dx[myrow] = r;
+          ra.ds[myrow] = dsv;
+          ra.dxi[myrow] = dxiv;
+          ra.D[myrow] = Dv;
+          ra.winv[myrow] = wv;
+          ra.rhs[myrow] = r5v;
+          ra.V[myrow] = w;
+          ra.V[n + myrow] = w1;
+          rv[0] += xv * sv + cx * xiv;
+          rv[1] += r * r;
+          rv[2] += av * xv;
+          rv[3] += xv * ev;
+          rv[4] += xv;
+          rv[5] += xiv;
+          rv[6] += fabs(xv);
+          rv[7] = fmax(rv[7], Dv);
+          rv[8] = fmin(rv[8], Dv);
+        }
       } else {
         w = v[0];
         if (ok && lead) out[myrow] = w;
@@ -664,7 +718,12 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
       for (int q = 0; q < Q; q++)
 #pragma unroll
         for (int u = 0; u < ZP_U; u++) {
-          const double uq = base + u < r1 ? readlane_f64(w, (q * ZP_U + u) * GL) : 0.0;
+          // ZP_RES: both values of row u sit in its lane group
+          const double uq =
+              base + u < r1
+                  ? (MODE == ZP_RES ? readlane_f64(q == 0 ? w : w1, u * GL)
+                                    : readlane_f64(w, (q * ZP_U + u) * GL))
+                  : 0.0;
 #pragma unroll
           for (int f = 0; f < F; f++)
 #pragma unroll
@@ -702,7 +761,8 @@ static int zpass_blocks(Ctx* c) { return c->cu_count * 4 < EW_BLOCKS ? c->cu_cou
 static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int64_t n, int kp,
                              const double* T, const double* V, const double* winv,
                              const double* a, const double* D, const double* r5, const double* dx,
-                             const double* sc, int corr, double* out, double* red, double* Tout) {
+                             const double* sc, int corr, double* out, double* red, double* Tout,
+                             const ResArgs* ra) {
   const int G = kp / 4;
   const bool wide = G >= 64;
   bool ok_shape = false;
@@ -710,6 +770,7 @@ static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int
   else if (mode == ZP_BORDER) ok_shape = p == 1 && q == 1;
   else if (mode == ZP_T) ok_shape = wide && (p == 1 || p == 2) && q == p;
   else if (mode == ZP_N) ok_shape = wide && p == 1 && q == 0;
+  else if (mode == ZP_RES) ok_shape = wide && p == 1 && q == 2 && ra;
   if (kp % 32 != 0 || kp > 1024 || !ok_shape)
     return fail(c, TSVM_ERR_ARG, "zpass: kp=%d mode=%d p=%d q=%d unsupported", kp, mode, p, q);
   int L = 64;
@@ -720,9 +781,10 @@ static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int
   if (q > 0) TSVM_HIP(c, c->ws.get("zpass_part", (size_t)nblk * q * kp * 8, &part));
   const size_t sm = (size_t)2 * q * kp * 8;
   const dim3 g(nblk), b(256);
+  const ResArgs rargs = ra ? *ra : ResArgs{};
 #define ZK(KN, MO, PP, QQ, FF)                                                             \
   hipLaunchKernelGGL((KN<MO, PP, QQ, FF>), g, b, sm, c->stream, Z, n, kp, T, V, winv, a, D, \
-                     r5, dx, sc, corr, out, red, (double*)part)
+                     r5, dx, sc, corr, out, red, (double*)part, rargs)
 #define ZW(MO, PP, QQ)                                         \
   if (F == 1) ZK(zpass_wide_kernel, MO, PP, QQ, 1);            \
   else if (F == 2) ZK(zpass_wide_kernel, MO, PP, QQ, 2);       \
@@ -745,6 +807,8 @@ static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int
     }
   } else if (mode == ZP_N) {
     ZW(ZP_N, 1, 0);
+  } else if (mode == ZP_RES) {
+    ZW(ZP_RES, 1, 2);
   } else if (p == 2) {
     ZF(ZP_SMW, 2, 2);
   } else if (q == 1) {
@@ -1090,6 +1154,9 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     return TSVM_OK;
   };
   const int zblk = zpass_blocks(c);
+  // the fused residual pass runs on the wide-row kernel; above kp = 768 (four float4 groups
+  // per lane) its register demand spills, and the separate passes are kept
+  const bool fuse_res = kp >= 256 && kp <= 768;
   // k-vectors: the capacitance solves (two rows), Z^T r7 and Z^T b of the predictor,
   // Z^T of the latest single solve, Z^T dx of the step being refined
   double* Tsol = T;
@@ -1101,8 +1168,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   // sides (its error is corrected by the bordered refinement below, whose residual uses
   // the exact operator).  V = D^-1 rhs must be in place; Z^T V, M^-1 Z^T V, then one
   // fused pass forms sol, its a^T sol_q (to slots) and, with Tz, Z^T sol_q.
-  auto smw = [&](int pp, double* sol, double* Tz, std::vector<int> slots) -> tsvm_status {
-    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, Tsol));
+  // have_tv: Tsol already holds Z^T V (the fused residual pass formed it)
+  auto smw = [&](int pp, double* sol, double* Tz, std::vector<int> slots,
+                 bool have_tv = false) -> tsvm_status {
+    if (!have_tv) TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, Tsol));
     TSVM_CHECK(chol_solve_f64(c, M, kp, Tsol, pp, dinfo));
     TSVM_CHECK(ipm_zpass(c, ZP_SMW, pp, Tz ? pp : 0, Z, n, kp, Tsol, V, winv, a, nullptr, nullptr,
                          nullptr, sc, 0, sol, red, Tz));
@@ -1166,14 +1235,22 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   int iter = 0;
   const int maxit = prm->maxit > 0 ? prm->maxit : 100;
   for (iter = 0; iter < maxit; iter++) {
-    // ---- residual pass (1606-1643)
-    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, x, n, 1, T));
-    TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, T, 1, e, n));
-    hipLaunchKernelGGL(k_resid, eg, eb, 0, st, n, C, (const double*)x, (const double*)s,
-                       (const double*)xi, (const double*)a, (const double*)e, (const double*)sc, dx,
-                       ds, dxi, D, winv, red);
+    // ---- residual pass (1606-1643); with wide rows it also forms the predictor's scaled
+    // right-hand sides [r5, a]/D and their Z^T (into Tsol) from the same read of Z
+    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, x, n, 1, Tc));
+    if (fuse_res) {
+      const ResArgs ra{x, s, xi, dx, ds, dxi, D, winv, rhs, V, C};
+      TSVM_CHECK(ipm_zpass(c, ZP_RES, 1, 2, Z, n, kp, Tc, nullptr, nullptr, a, nullptr, nullptr,
+                           nullptr, sc, 0, nullptr, red, Tsol, &ra));
+    } else {
+      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, Tc, 1, e, n));
+      hipLaunchKernelGGL(k_resid, eg, eb, 0, st, n, C, (const double*)x, (const double*)s,
+                         (const double*)xi, (const double*)a, (const double*)e, (const double*)sc,
+                         dx, ds, dxi, D, winv, red);
+    }
     TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_MAX, RED_MIN},
-                      {SL_MU, SL_NDX2, SL_AX, SL_XQX, SL_SX, SL_SXI, SL_L1, SL_DMAX, SL_DMIN}));
+                      {SL_MU, SL_NDX2, SL_AX, SL_XQX, SL_SX, SL_SXI, SL_L1, SL_DMAX, SL_DMIN},
+                      fuse_res ? zblk : EW_BLOCKS));
     // hybrid capacitance: split the rows for this iteration's weights (the FP64 row
     // count comes back with the scalars)
     if (bf16) TSVM_CHECK(hybrid_syrk_split(c, &hs, winv, IPM_SYRK_BUDGET, sc + SL_HSB));
@@ -1263,10 +1340,12 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     pending_syrk = true;
     TSVM_CHECK(chol_f64(c, M, kp, dinfo));
     // ---- predictor (1727-1748)
-    hipLaunchKernelGGL(k_rhs_pred, eg, eb, 0, st, n, C, (const double*)x, (const double*)dx,
-                       (const double*)ds, (const double*)dxi, (const double*)a, rhs, n);
-    hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, 2, (const double*)rhs, n, (const double*)winv, V, n);
-    TSVM_CHECK(smw(2, solp, Tr7, {SL_AR7, SL_AB}));  // r7 = S^-1 r5, b = S^-1 a
+    if (!fuse_res) {
+      hipLaunchKernelGGL(k_rhs_pred, eg, eb, 0, st, n, C, (const double*)x, (const double*)dx,
+                         (const double*)ds, (const double*)dxi, (const double*)a, rhs, n);
+      hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, 2, (const double*)rhs, n, (const double*)winv, V, n);
+    }
+    TSVM_CHECK(smw(2, solp, Tr7, {SL_AR7, SL_AB}, fuse_res));  // r7 = S^-1 r5, b = S^-1 a
     hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 0, (const double*)(solp + n), (const double*)solp,
                        sc, sol3, kp, (const double*)(border ? Tr7 : nullptr), (const double*)Tb, T3);
     TSVM_CHECK(border_refine(0));
