@@ -31,8 +31,12 @@ def test_ipm_fp32_vs_sklearn_fixture(ctx, name):
     assert info.iters == io.iters
 
 
+# k < 256 runs the narrow pass kernels; 256 (one float4 group per lane), 300 and 600
+# (a partly masked last group), 1000 (four groups, separate residual pass) the wide ones
 @pytest.mark.parametrize("n,k,C", [(5000, 32, 32.0), (3001, 50, 1.0), (8000, 100, 8.0),
-                                   (4000, 160, 8.0), (6000, 200, 32.0), (3000, 72, 4.0)])
+                                   (4000, 160, 8.0), (6000, 200, 32.0), (3000, 72, 4.0),
+                                   (4000, 256, 8.0), (4000, 300, 32.0), (3000, 600, 8.0),
+                                   (1200, 1000, 8.0)])
 def test_ipm_fp32_matches_oracle(ctx, n, k, C):
     import tensorsvm_amd as T
     Z, y = _problem(n, k, n + k)
@@ -48,7 +52,8 @@ def test_ipm_fp32_matches_oracle(ctx, n, k, C):
     assert np.all((sv_g == sv_o) | amb)
 
 
-@pytest.mark.parametrize("n,k,C", [(5000, 32, 32.0), (8000, 100, 8.0), (6000, 200, 32.0)])
+@pytest.mark.parametrize("n,k,C", [(5000, 32, 32.0), (8000, 100, 8.0), (6000, 200, 32.0),
+                                   (4000, 300, 32.0), (3000, 600, 8.0)])
 def test_ipm_bf16_refined_matches_oracle(ctx, n, k, C):
     import tensorsvm_amd as T
     Z, y = _problem(n, k, 7 * n + k)
