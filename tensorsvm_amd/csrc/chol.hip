@@ -45,8 +45,13 @@ __global__ __launch_bounds__(64) void chol_diag(double* __restrict__ M, int64_t 
       bad = j + 1;
       break;
     }
-    const double s = sqrt(d);
-    a[j] = lane == j ? s : (lane > j ? a[j] / s : a[j]);
+    // 1/sqrt(d): the hardware estimate and two Newton steps y <- y (3 - d y^2) / 2 (each
+    // doubles the correct bits), so the pivot step needs no division on its critical path
+    double rs = __builtin_amdgcn_rsq(d);
+    rs = rs * (1.5 - 0.5 * d * rs * rs);
+    rs = rs * (1.5 - 0.5 * d * rs * rs);
+    const double s = d * rs;
+    a[j] = lane == j ? s : (lane > j ? a[j] * rs : a[j]);
 #pragma unroll
     for (int l = j + 1; l < CB; l++) {
       const double llj = readlane_f64(a[j], l);
@@ -58,15 +63,23 @@ __global__ __launch_bounds__(64) void chol_diag(double* __restrict__ M, int64_t 
     return;
   }
   // column c of the inverse: Li[i][c] = (e_c[i] - sum_{c <= t < i} L[i][t] Li[t][c]) / L[i][i],
-  // lane c walks down its column; the factor and the inverse sit in LDS
+  // lane c walks down its column; the factor, the reciprocal diagonal and the inverse sit
+  // in LDS
   __shared__ double Ls[CB][CB + 1];
   __shared__ double Is[CB][CB + 1];
-  if (lane < CB)
+  __shared__ double Rd[CB];
+  if (lane < CB) {
 #pragma unroll
     for (int t = 0; t < CB; t++) {
       Ls[lane][t] = a[t];
       Is[t][lane] = 0.0;
     }
+    double dl = 0.0;  // L[lane][lane]
+#pragma unroll
+    for (int t = 0; t < CB; t++)
+      if (t == lane) dl = a[t];
+    Rd[lane] = 1.0 / dl;
+  }
   __syncthreads();
   if (lane < CB) {
     const int c = lane;
@@ -74,7 +87,7 @@ __global__ __launch_bounds__(64) void chol_diag(double* __restrict__ M, int64_t 
     for (int i = c; i < CB; i++) {
       double v = i == c ? 1.0 : 0.0;
       for (int t = c; t < i; t++) v -= Ls[i][t] * Is[t][c];
-      Is[i][c] = v / Ls[i][i];
+      Is[i][c] = v * Rd[i];
     }
   }
   if (row)
