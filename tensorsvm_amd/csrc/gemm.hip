@@ -29,11 +29,13 @@ constexpr int TN_R = 16;  // rows per stage
 constexpr int TN_H = TN_R / 16;  // staging passes of 16 rows
 __device__ inline int tn_sw(int row, int col) { return col ^ ((row & 1) << 4); }
 
-template <bool VEC, bool HAS_W>
+// IDX: the n rows taken are rows[0..n) of A, B and w (an index list; the index of a stage's
+// row is loaded one stage before its data, so the data loads do not wait on it).
+template <bool VEC, bool HAS_W, bool IDX = false>
 __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     const float* __restrict__ A, int64_t lda, int64_t ka, const float* __restrict__ B,
     int64_t ldb, int64_t kb, int64_t n, const double* __restrict__ w, int lower_only,
-    int64_t rows_per_split, double* __restrict__ part) {
+    int64_t rows_per_split, double* __restrict__ part, const int* __restrict__ rows = nullptr) {
   const int ta = blockIdx.x, tb = blockIdx.y;
   if (lower_only && tb > ta) return;
   __shared__ double sA[2][TN_R][64];
@@ -53,13 +55,22 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
   double rw[2][TN_H];
   bool rok[2][TN_H];
   // rows past the split load a valid row and are zeroed when staged: no branches, and
-  // nothing waits on the loads before the data is needed
+  // nothing waits on the loads before the data is needed.  With IDX, pidx holds the
+  // physical rows of the stage being loaded (fetched one stage earlier by iload).
+  int pidx[TN_H];
+  auto iload = [&](int64_t st) {
+#pragma unroll
+    for (int h = 0; h < TN_H; h++) {
+      const int64_t row = r0 + st * TN_R + sr + 16 * h;
+      pidx[h] = IDX ? rows[row < r1 ? row : r0] : 0;
+    }
+  };
   auto gload = [&](int64_t st, int slot) {
 #pragma unroll
     for (int h = 0; h < TN_H; h++) {
       const int64_t row = r0 + st * TN_R + sr + 16 * h;
       const bool ok = row < r1;
-      const int64_t rr = ok ? row : r0;
+      const int64_t rr = IDX ? (int64_t)pidx[h] : (ok ? row : r0);
       rok[slot][h] = ok;
       rw[slot][h] = HAS_W ? w[rr] : 1.0;
       const float* pa = A + rr * lda + a0 + sc4;
@@ -119,6 +130,7 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
   auto stage = [&](int64_t st, auto bc) {
     constexpr int b = decltype(bc)::value;
     gload(st + 2, b);
+    if (IDX) iload(st + 3);
     __builtin_amdgcn_sched_barrier(0);
     compute(b);
     __builtin_amdgcn_sched_barrier(0);
@@ -127,8 +139,11 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     __syncthreads();
   };
   if (nst > 0) {
+    if (IDX) iload(0);
     gload(0, 0);
+    if (IDX) iload(1);
     gload(1, 1);
+    if (IDX) iload(2);
     sstore(0, 0);
     __syncthreads();
   }
@@ -148,22 +163,6 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
         int64_t a = a0 + wa + 16 * i + kq + 4 * q, b = b0 + wb + 16 * j + l16;
         if (a < ka && b < kb) P[a * kb + b] = acc[i][j][q];
       }
-}
-
-// rows[0..nh) of A (lda) into the contiguous G (nh x ka) and their weights into wg
-__global__ __launch_bounds__(256) void gather_rows_kernel(const float* __restrict__ A, int64_t lda,
-                                                          int64_t ka, const int* __restrict__ rows,
-                                                          int64_t nh, const double* __restrict__ w,
-                                                          float* __restrict__ G,
-                                                          double* __restrict__ wg) {
-  const int64_t tot = nh * ka;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = e / ka, c = e - i * ka;
-    const int64_t r = rows[i];
-    G[e] = A[r * lda + c];
-    if (c == 0) wg[i] = w ? w[r] : 1.0;
-  }
 }
 
 __global__ void reduce_splits_kernel(const double* __restrict__ part, int64_t nsplit,
@@ -186,21 +185,8 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
                         double* C, const int* rows) {
   if (ka <= 0 || kb <= 0) return TSVM_OK;
   if (lower_only && ka != kb) return fail(c, TSVM_ERR_ARG, "gemm_tn_f64: lower_only needs ka==kb");
-  if (rows) {
-    // a row list: gather the rows (and weights) into a contiguous block first, so that
-    // the GEMM's loads do not wait on the index loads (the gather moves 8 n ka bytes, a
-    // few percent of the GEMM's time)
-    if (A != B || lda != ldb || ka != kb)
-      return fail(c, TSVM_ERR_ARG, "gemm_tn_f64: a row list needs A == B");
-    void *pg, *pw;
-    TSVM_HIP(c, c->ws.get("gemm_gather", (size_t)((n > 0 ? n : 1) * ka * 4), &pg));
-    TSVM_HIP(c, c->ws.get("gemm_gather_w", (size_t)((n > 0 ? n : 1) * 8), &pw));
-    if (n > 0)
-      hipLaunchKernelGGL(gather_rows_kernel, dim3(4096), dim3(256), 0, c->stream, A, lda, ka, rows, n,
-                         w, (float*)pg, (double*)pw);
-    return gemm_tn_f64(c, (const float*)pg, ka, ka, (const float*)pg, ka, kb, n, (const double*)pw,
-                       lower_only, C, nullptr);
-  }
+  if (rows && (A != B || lda != ldb || ka != kb))
+    return fail(c, TSVM_ERR_ARG, "gemm_tn_f64: a row list needs A == B");
   const int64_t ta = ceil_div(ka, 64), tb = ceil_div(kb, 64);
   const int64_t tiles = lower_only ? ta * (ta + 1) / 2 : ta * tb;
   // enough blocks to fill the chip several times; at least 256 rows per split
@@ -215,13 +201,20 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   dim3 grid((unsigned)ta, (unsigned)tb, (unsigned)nsplit);
   // full 64-column tiles with 16-byte rows: vector loads without column masks
   const bool vec = ka % 64 == 0 && kb % 64 == 0 && lda % 4 == 0 && ldb % 4 == 0;
-#define GTN(V, W)                                                                              \
-  hipLaunchKernelGGL((gemm_tn_f64_kernel<V, W>), grid, dim3(256), 0, c->stream, A, lda, ka, B, \
-                     ldb, kb, n, w, lower_only ? 1 : 0, rps, (double*)part)
-  if (vec) {
-    if (w) GTN(true, true); else GTN(true, false);
+#define GTN(V, W, I)                                                                               \
+  hipLaunchKernelGGL((gemm_tn_f64_kernel<V, W, I>), grid, dim3(256), 0, c->stream, A, lda, ka, B, \
+                     ldb, kb, n, w, lower_only ? 1 : 0, rps, (double*)part, rows)
+  if (rows) {
+    // the FP64 rows of the hybrid capacitance: rows read in place through the index list
+    if (vec) {
+      if (w) GTN(true, true, true); else GTN(true, false, true);
+    } else {
+      if (w) GTN(false, true, true); else GTN(false, false, true);
+    }
+  } else if (vec) {
+    if (w) GTN(true, true, false); else GTN(true, false, false);
   } else {
-    if (w) GTN(false, true); else GTN(false, false);
+    if (w) GTN(false, true, false); else GTN(false, false, false);
   }
 #undef GTN
   hipLaunchKernelGGL(reduce_splits_kernel, dim3((unsigned)ceil_div(ka * kb, 256)), dim3(256), 0,

@@ -57,7 +57,7 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
 // C (ka x kb, FP64, row-major) = A^T diag(w) B over n rows; A n x ka (lda), B n x kb
 // (ldb) FP32 row-major; w FP64 or nullptr (= ones).  lower_only: only blocks with
 // block-row >= block-col are computed (for SYRK; the caller mirrors).  rows (device,
-// optional): the n rows taken are rows[0..n) of A, B and w (a gather) instead of 0..n.
+// optional): the n rows taken are rows[0..n) of A, B and w (read in place) instead of 0..n.
 tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const float* B,
                         int64_t ldb, int64_t kb, int64_t n, const double* w, bool lower_only,
                         double* C, const int* rows = nullptr);
@@ -85,7 +85,7 @@ tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridS
 // bound, slots[1] = number of FP64 rows (device doubles); fills rows and wb.
 tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double budget,
                               double* slots);
-// M = I + FP64 part over the nh gathered rows + BF16 part over the bulk.  ev (optional,
+// M = I + FP64 part over the nh listed rows + BF16 part over the bulk.  ev (optional,
 // two events) brackets the BF16 part.
 tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const double* w,
                              int64_t nh, double* M, hipEvent_t* ev);

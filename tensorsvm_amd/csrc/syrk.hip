@@ -7,7 +7,7 @@
 // Low-precision (hybrid) form: the rows are split by t_i = w_i |z_i|^2.  The bulk B
 // (small t) goes through a BF16 MFMA SYRK with FP32 accumulation; the rows that carry
 // the large weights (the free support vectors late in the solve) go through the FP64
-// GEMM, gathered by an index list.  The split keeps the BF16 part's error below a
+// GEMM, read through an index list.  The split keeps the BF16 part's error below a
 // budget: each bulk term w_i z_i z_i^T is computed with three bf16 roundings (z once
 // per solve on both sides, w z once), a relative error of about 3 * 2^-9 that is
 // zero-mean and independent across rows, so |E|_2 ~ c 2^-9 sqrt(sum_B t_i^2).  Because
@@ -513,10 +513,6 @@ tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridS
   hs->rows = (int*)p;
   TSVM_HIP(c, c->ws.get("hs_ints", (size_t)(HS_BLOCKS * (HS_BINS + 2) + 8) * 4, &p));
   hs->ints = (int*)p;
-  // the FP64 rows are gathered into these by gemm_tn_f64 each iteration: sized once for
-  // every row, so that a growing row count never reallocates (and drains the device)
-  TSVM_HIP(c, c->ws.get("gemm_gather", (size_t)(n > 0 ? n : 1) * kp * 4, &p));
-  TSVM_HIP(c, c->ws.get("gemm_gather_w", (size_t)(n > 0 ? n : 1) * 8, &p));
   hipStream_t s = c->stream;
   dim3 g((unsigned)(hs->n_pad / 32), (unsigned)(hs->kpt / 32));
   hipLaunchKernelGGL(make_zbt_kernel, g, dim3(256), 0, s, Z, n, kp, hs->kpt, hs->n_pad,
@@ -554,7 +550,7 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
   hipStream_t s = c->stream;
   const int64_t n = hs->n;
   const int kp = hs->kp, kpt = hs->kpt;
-  // FP64 rows (all rows when every row is FP64: no gather)
+  // FP64 rows (all rows when every row is FP64: no index list)
   if (nh >= n)
     TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, n, w, true, M));
   else
