@@ -87,6 +87,12 @@ def grad_form(ctx, torch, n=200_000, nt=50_000, d=32, k=256, C=1.0, seed=1003):
     Dp2, Lp2, _, _, _ = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg + h2, s0, om, dXt, dyt, nt, False)
     Dm2, Lm2, _, _, _ = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg - h2, s0, om, dXt, dyt, nt, False)
     fdD2, fdL2 = (Dp2 - Dm2) / (2 * h2), (Lp2 - Lm2) / (2 * h2)
+    # Richardson: central differences at 2h and h, (4 FD(h) - FD(2h)) / 3 cancels the h^2
+    # term of the truncation error
+    Dp3, Lp3, _, _, _ = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg + 2 * h, s0, om, dXt, dyt, nt, False)
+    Dm3, Lm3, _, _, _ = ctx.grad_dev(dX, dy, n, d, dal, info.b, k, lg - 2 * h, s0, om, dXt, dyt, nt, False)
+    fdD3, fdL3 = (Dp3 - Dm3) / (4 * h), (Lp3 - Lm3) / (4 * h)
+    rD, rL = (4 * fdD - fdD3) / 3, (4 * fdL - fdL3) / 3
     tf = gi.backward_tflops
     log(f"grad: train {t_train:.2f}s, gradient {gi.t_total:.2f}s (backward {gi.t_backward:.3f}s, "
         f"{tf:.1f} TF/s); dD/dlog(gamma) {gD[0]:.6e} vs FD {fdD:.6e}")
@@ -102,6 +108,8 @@ def grad_form(ctx, torch, n=200_000, nt=50_000, d=32, k=256, C=1.0, seed=1003):
             "dL_rel_err_vs_fd": abs(gL[0] - fdL) / max(abs(fdL), 1e-300),
             "fd_h1e-2": {"dD_rel_err": abs(gD[0] - fdD2) / abs(fdD2),
                          "dL_rel_err": abs(gL[0] - fdL2) / max(abs(fdL2), 1e-300)},
+            "fd_richardson_h1e-3": {"dD_rel_err": abs(gD[0] - rD) / abs(rD),
+                                    "dL_rel_err": abs(gL[0] - rL) / max(abs(rL), 1e-300)},
             "ipm": {"iters": info.ipm.iters, "reason": info.ipm.reason,
                     "gap_rel": info.ipm.gap_rel}, "nsv": info.nsv}
 
