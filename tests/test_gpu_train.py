@@ -27,7 +27,9 @@ def _oracle_train(X, y, k, gamma, C, om):
     return al, b, info, U
 
 
-@pytest.mark.parametrize("n,k", [(3000, 32), (4000, 64)])
+# (25000, 32): the plain method's small configuration, 20,000 training rows, d = 18,
+# gamma = 0.05, C = 32, k = 32 (the oracle takes a few seconds on the host cores)
+@pytest.mark.parametrize("n,k", [(3000, 32), (4000, 64), (25000, 32)])
 def test_train_fp32_matches_oracle(ctx, n, k):
     import tensorsvm_amd as T
     gamma, C = 0.05, 32.0
@@ -47,7 +49,11 @@ def test_train_fp32_matches_oracle(ctx, n, k):
     fo = O.decision(alo, y, X, gamma, bo, Xt)
     acc_g, acc_o = np.mean(np.sign(fg) == yt), np.mean(np.sign(fo) == yt)
     assert abs(acc_g - acc_o) <= 0.001 + 1.0 / len(yt)
-    assert np.allclose(fg, O.decision(al, y, X, gamma, info.b, Xt), atol=1e-4)
+    # the GPU decision function (FP32 products) against FP64 on the same multipliers, to
+    # the FP32 accumulation error of sum_j |alpha_j| K(x_t, x_j)
+    scale = O.decision(np.abs(al), np.ones_like(y), X, gamma, 0.0, Xt)
+    err = np.abs(fg - O.decision(al, y, X, gamma, info.b, Xt))
+    assert (err <= 1e-4 + 1e-5 * scale).all(), (err.max(), scale.max())
 
 
 @pytest.mark.parametrize("form", ["bf16", "split"])
