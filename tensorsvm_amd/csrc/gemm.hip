@@ -17,41 +17,44 @@
 
 namespace tsvm {
 
-// 256 threads; the block owns a 64 x 64 output tile, wave w a 32 x 32 quarter
-// (2 x 2 MFMA tiles of 16 x 16).  Rows are staged 32 at a time through LDS as FP64
-// (the A panel already multiplied by the row weight), double-buffered in LDS, with the
-// global loads two stages ahead in a register ring (a stage is 2048 cycles of FP64 MFMA
-// per wave, less than a loaded memory latency).  LDS rows are 64 doubles (a multiple of
-// the bank count), so odd rows store their columns XOR 16: the four row groups of a
-// fragment read (lanes 16 q .. 16 q + 15 read row 4 s + q) then fall on disjoint banks
-// within each half-wave instead of all on the same 32.
+// 256 threads; the block owns a TILE x TILE output tile (TILE = 64 or 128), wave w a
+// TILE/2 x TILE/2 quarter (TILE/32 x TILE/32 MFMA tiles of 16 x 16; the 128 form has 16
+// independent accumulators per wave and reads half an LDS fragment per MFMA).  Rows are
+// staged TN_R at a time through LDS as FP64 (the A panel already multiplied by the row
+// weight), double-buffered in LDS, with the global loads two stages ahead in a register
+// ring.  LDS rows are TILE doubles (a multiple of the bank count), so odd rows store
+// their columns XOR 16: the four row groups of a fragment read (lanes 16 q .. 16 q + 15
+// read row 4 s + q) then fall on disjoint banks within each half-wave instead of all on
+// the same 32.
 constexpr int TN_R = 16;  // rows per stage
 constexpr int TN_H = TN_R / 16;  // staging passes of 16 rows
 __device__ inline int tn_sw(int row, int col) { return col ^ ((row & 1) << 4); }
 
 // IDX: the n rows taken are rows[0..n) of A, B and w (an index list; the index of a stage's
 // row is loaded one stage before its data, so the data loads do not wait on it).
-template <bool VEC, bool HAS_W, bool IDX = false>
+template <bool VEC, bool HAS_W, bool IDX = false, int TILE = 64>
 __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     const float* __restrict__ A, int64_t lda, int64_t ka, const float* __restrict__ B,
     int64_t ldb, int64_t kb, int64_t n, const double* __restrict__ w, int lower_only,
     int64_t rows_per_split, double* __restrict__ part, const int* __restrict__ rows = nullptr) {
+  constexpr int CG = TILE / 64;  // 64-column groups of a staged row
+  constexpr int NI = TILE / 32;  // MFMA tiles per wave along each side
   const int ta = blockIdx.x, tb = blockIdx.y;
   if (lower_only && tb > ta) return;
-  __shared__ double sA[2][TN_R][64];
-  __shared__ double sB[2][TN_R][64];
+  __shared__ double sA[2][TN_R][TILE];
+  __shared__ double sB[2][TN_R][TILE];
   const int64_t split = blockIdx.z;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
-  const int wa = (wave >> 1) * 32, wb = (wave & 1) * 32;
-  const int64_t a0 = (int64_t)ta * 64, b0 = (int64_t)tb * 64;
+  const int wa = (wave >> 1) * (TILE / 2), wb = (wave & 1) * (TILE / 2);
+  const int64_t a0 = (int64_t)ta * TILE, b0 = (int64_t)tb * TILE;
   const int64_t r0 = split * rows_per_split;
   const int64_t r1 = r0 + rows_per_split < n ? r0 + rows_per_split : n;
   const int64_t nst = (r1 - r0 + TN_R - 1) / TN_R;
-  // staging map: thread -> (row, 4 columns) for two passes of 16 rows
+  // staging map: thread -> (row, 4 columns in each 64-column group) for TN_H passes
   const int sr = tid >> 4, sc4 = (tid & 15) * 4;
-  f32x4_t ra[2][TN_H], rb[2][TN_H];  // [ring slot][h]
+  f32x4_t ra[2][TN_H][CG], rb[2][TN_H][CG];  // [ring slot][h][column group]
   double rw[2][TN_H];
   bool rok[2][TN_H];
   // rows past the split load a valid row and are zeroed when staged: no branches, and
@@ -73,24 +76,26 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
       const int64_t rr = IDX ? (int64_t)pidx[h] : (ok ? row : r0);
       rok[slot][h] = ok;
       rw[slot][h] = HAS_W ? w[rr] : 1.0;
-      const float* pa = A + rr * lda + a0 + sc4;
-      const float* pb = B + rr * ldb + b0 + sc4;
-      f32x4_t va, vb;
-      if (VEC) {
-        va = *(const f32x4_t*)pa;
-        vb = *(const f32x4_t*)pb;
-      } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const bool ca = a0 + sc4 + j < ka, cb = b0 + sc4 + j < kb;
-          const float xa = A[rr * lda + (ca ? a0 + sc4 + j : 0)];
-          const float xb = B[rr * ldb + (cb ? b0 + sc4 + j : 0)];
-          va[j] = ca ? xa : 0.0f;
-          vb[j] = cb ? xb : 0.0f;
+      for (int cg = 0; cg < CG; cg++) {
+        const int64_t ca0 = a0 + cg * 64 + sc4, cb0 = b0 + cg * 64 + sc4;
+        f32x4_t va, vb;
+        if (VEC) {
+          va = *(const f32x4_t*)(A + rr * lda + ca0);
+          vb = *(const f32x4_t*)(B + rr * ldb + cb0);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const bool ca = ca0 + j < ka, cb = cb0 + j < kb;
+            const float xa = A[rr * lda + (ca ? ca0 + j : 0)];
+            const float xb = B[rr * ldb + (cb ? cb0 + j : 0)];
+            va[j] = ca ? xa : 0.0f;
+            vb[j] = cb ? xb : 0.0f;
+          }
         }
+        ra[slot][h][cg] = va;
+        rb[slot][h][cg] = vb;
       }
-      ra[slot][h] = va;
-      rb[slot][h] = vb;
     }
   };
   auto sstore = [&](int buf, int slot) {
@@ -99,27 +104,34 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
       const int r = sr + 16 * h;
       const double wv = rok[slot][h] ? rw[slot][h] : 0.0;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        sA[buf][r][tn_sw(r, sc4 + j)] = wv * (double)ra[slot][h][j];
-        sB[buf][r][tn_sw(r, sc4 + j)] = rok[slot][h] ? (double)rb[slot][h][j] : 0.0;
-      }
+      for (int cg = 0; cg < CG; cg++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          sA[buf][r][tn_sw(r, cg * 64 + sc4 + j)] = wv * (double)ra[slot][h][cg][j];
+          sB[buf][r][tn_sw(r, cg * 64 + sc4 + j)] = rok[slot][h] ? (double)rb[slot][h][cg][j] : 0.0;
+        }
     }
   };
-  f64x4_t acc[2][2];
+  f64x4_t acc[NI][NI];
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < NI; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (f64x4_t){0, 0, 0, 0};
+    for (int j = 0; j < NI; j++) acc[i][j] = (f64x4_t){0, 0, 0, 0};
   auto compute = [&](int buf) {
 #pragma unroll
     for (int s4 = 0; s4 < TN_R / 4; s4++) {
       const int row = 4 * s4 + kq;
-      double av0 = sA[buf][row][tn_sw(row, wa + l16)], av1 = sA[buf][row][tn_sw(row, wa + 16 + l16)];
-      double bv0 = sB[buf][row][tn_sw(row, wb + l16)], bv1 = sB[buf][row][tn_sw(row, wb + 16 + l16)];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv1, acc[1][1], 0, 0, 0);
+      double av[NI], bv[NI];
+#pragma unroll
+      for (int i = 0; i < NI; i++) {
+        av[i] = sA[buf][row][tn_sw(row, wa + 16 * i + l16)];
+        bv[i] = sB[buf][row][tn_sw(row, wb + 16 * i + l16)];
+      }
+#pragma unroll
+      for (int i = 0; i < NI; i++)
+#pragma unroll
+        for (int j = 0; j < NI; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
     }
   };
   // One stage with its ring slot and LDS buffer fixed at compile time (b): register slot
@@ -155,9 +167,9 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
   // column lane%16
   double* P = part + split * ka * kb;
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < NI; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < NI; j++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         int64_t a = a0 + wa + 16 * i + kq + 4 * q, b = b0 + wb + 16 * j + l16;
@@ -187,7 +199,9 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   if (lower_only && ka != kb) return fail(c, TSVM_ERR_ARG, "gemm_tn_f64: lower_only needs ka==kb");
   if (rows && (A != B || lda != ldb || ka != kb))
     return fail(c, TSVM_ERR_ARG, "gemm_tn_f64: a row list needs A == B");
-  const int64_t ta = ceil_div(ka, 64), tb = ceil_div(kb, 64);
+  // 128-column tiles for wide operands (16 accumulators per wave), 64 otherwise
+  const int tile = ka >= 256 && kb >= 256 ? 128 : 64;
+  const int64_t ta = ceil_div(ka, tile), tb = ceil_div(kb, tile);
   const int64_t tiles = lower_only ? ta * (ta + 1) / 2 : ta * tb;
   // enough blocks to fill the chip several times; at least 256 rows per split
   int64_t nsplit = ceil_div((int64_t)c->cu_count * 8, tiles);
@@ -199,11 +213,17 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   void* part;
   TSVM_HIP(c, c->ws.get("gemm_part", (size_t)(nsplit * ka * kb * 8), &part));
   dim3 grid((unsigned)ta, (unsigned)tb, (unsigned)nsplit);
-  // full 64-column tiles with 16-byte rows: vector loads without column masks
-  const bool vec = ka % 64 == 0 && kb % 64 == 0 && lda % 4 == 0 && ldb % 4 == 0;
-#define GTN(V, W, I)                                                                               \
-  hipLaunchKernelGGL((gemm_tn_f64_kernel<V, W, I>), grid, dim3(256), 0, c->stream, A, lda, ka, B, \
-                     ldb, kb, n, w, lower_only ? 1 : 0, rps, (double*)part, rows)
+  // whole tiles with 16-byte rows: vector loads without column masks
+  const bool vec = ka % tile == 0 && kb % tile == 0 && lda % 4 == 0 && ldb % 4 == 0;
+#define GTN(V, W, I)                                                                              \
+  do {                                                                                          \
+    if (tile == 128)                                                                            \
+      hipLaunchKernelGGL((gemm_tn_f64_kernel<V, W, I, 128>), grid, dim3(256), 0, c->stream, A,   \
+                         lda, ka, B, ldb, kb, n, w, lower_only ? 1 : 0, rps, (double*)part, rows); \
+    else                                                                                        \
+      hipLaunchKernelGGL((gemm_tn_f64_kernel<V, W, I, 64>), grid, dim3(256), 0, c->stream, A,    \
+                         lda, ka, B, ldb, kb, n, w, lower_only ? 1 : 0, rps, (double*)part, rows); \
+  } while (0)
   if (rows) {
     // the FP64 rows of the hybrid capacitance: rows read in place through the index list
     if (vec) {
