@@ -18,7 +18,8 @@
 //   corrector         the same with one SMW solve (S^-1 a is reused), combined step, update
 // The last pass of each SMW solve also accumulates Z^T sol, so Z^T of the combined step is
 // a combination of k-vectors, and the bordered residual pass also accumulates Z^T of the
-// next solve's right-hand side.  Per iteration that is 10 reads of Z.
+// next solve's right-hand side.  Per iteration that is 10 reads of Z; for 256 <= kp <= 768
+// the residual pass also forms the predictor's right-hand sides and their Z^T (9 reads).
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
