@@ -271,8 +271,8 @@ __global__ void prep_A_f32(const float* __restrict__ A, int64_t lda, const float
 // -------------------------------------------------------------------------------------
 enum { EM_BF16 = 0, EM_F16 = 1, EM_F16X2 = 2, EM_F16R2 = 3 };
 
-template <int NT, int KD, int CH, int EM, int OCC>
-__global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict__ P,
+template <int NT, int KD, int CH, int EM, int OCC, int THR = 256>
+__global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict__ P,
                                                   const __bf16* __restrict__ Q,
                                                   const void* __restrict__ ATv,
                                                   const float* __restrict__ y1,
@@ -291,8 +291,8 @@ __global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict_
   constexpr int QV = 32 * CH * KD / 8;   // 16-byte vectors per Qop stage (contiguous)
   constexpr int AV = BN * AW * CH / 8;   // 16-byte vectors per AT stage
   constexpr int AVC = BN * AW / 8;       // 16-byte vectors per AT chunk
-  constexpr int NQ = (QV + 255) / 256;
-  constexpr int NA = (AV + 255) / 256;
+  constexpr int NQ = (QV + THR - 1) / THR;
+  constexpr int NA = (AV + THR - 1) / THR;
   __shared__ __attribute__((aligned(16))) __bf16 sQ[2][32 * CH * QS];
   __shared__ __attribute__((aligned(16))) ET sA[2][BN * AS];
 
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict_
   // resident blocks covers consecutive row blocks of one column block, so they stream
   // the same Qop and AT chunks at nearly the same time and share them through L2.
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-  const int64_t rowbase = (tile % nxb) * 256 + wave * 64;
+  const int64_t rowbase = (tile % nxb) * THR + wave * 64;
   const int64_t c0 = (tile / nxb) * BN;
 
   // S^T B operand: this wave's P rows, resident in registers for the whole j loop.
@@ -331,12 +331,12 @@ __global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict_
     const u32x4_t* gq = (const u32x4_t*)(Q + t * (32 * CH) * KD);
 #pragma unroll
     for (int v = 0; v < NQ; v++) {
-      int idx = tid + v * 256;
+      int idx = tid + v * THR;
       sq[v] = gq[idx < QV ? idx : QV - 1];
     }
 #pragma unroll
     for (int v = 0; v < NA; v++) {
-      int idx = tid + v * 256;
+      int idx = tid + v * THR;
       idx = idx < AV ? idx : AV - 1;
       int ch = idx / AVC, rem = idx - ch * AVC;
       const u32x4_t* ga = (const u32x4_t*)(AT + ((t * CH + ch) * r_pad + c0) * AW);
@@ -346,16 +346,16 @@ __global__ __launch_bounds__(256, OCC) void kprod_bf16(const __bf16* __restrict_
   auto sstore = [&](int buf) {
 #pragma unroll
     for (int v = 0; v < NQ; v++) {
-      int idx = tid + v * 256;
-      if (QV % 256 == 0 || idx < QV) {
+      int idx = tid + v * THR;
+      if (QV % THR == 0 || idx < QV) {
         int row = idx / (KD / 8), col = (idx % (KD / 8)) * 8;
         *(u32x4_t*)&sQ[buf][row * QS + col] = sq[v];
       }
     }
 #pragma unroll
     for (int v = 0; v < NA; v++) {
-      int idx = tid + v * 256;
-      if (AV % 256 == 0 || idx < AV) {
+      int idx = tid + v * THR;
+      if (AV % THR == 0 || idx < AV) {
         int ch = idx / AVC, rem = idx - ch * AVC;
         int row = rem / (AW / 8), col = ch * AW + (rem % (AW / 8)) * 8;
         *(u32x4_t*)&sA[buf][row * AS + col] = sa[v];
@@ -586,8 +586,16 @@ static void launch_bf16(int em, const __bf16* P, const __bf16* Q, const void* AT
                          y1, W, ldw, m, r, n_chunks / CH, r_pad, sc, nxb, ntiles);
       break;
     case EM_F16:
-      hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16, OCC>), grid, dim3(256), 0, s, P, Q, AT,
-                         y1, W, ldw, m, r, n_chunks / CH, r_pad, sc, nxb, ntiles);
+      if constexpr (NT == 4) {
+        // one 512-thread block per CU: eight waves share each staged stage
+        const int64_t nxb2 = m_pad / 512, ntiles2 = nxb2 * (r_pad / (NT * 32));
+        dim3 grid2((unsigned)(ntiles2 < cu_count ? ntiles2 : cu_count));
+        hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16, 1, 512>), grid2, dim3(512), 0, s, P,
+                           Q, AT, y1, W, ldw, m, r, n_chunks / CH, r_pad, sc, nxb2, ntiles2);
+      } else {
+        hipLaunchKernelGGL((kprod_bf16<NT, KD, CH, EM_F16, OCC>), grid, dim3(256), 0, s, P, Q,
+                           AT, y1, W, ldw, m, r, n_chunks / CH, r_pad, sc, nxb, ntiles);
+      }
       break;
     case EM_F16X2:
       hipLaunchKernelGGL((kprod_bf16<NT, KD, 1, EM_F16X2, 1>), grid, dim3(256), 0, s, P, Q, AT,
@@ -649,7 +657,8 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
       return fail(c, TSVM_ERR_ARG, "kernel_product(bf16): d=%ld exceeds 84 features", (long)d);
     const bool two_term = em == EM_F16X2 || em == EM_F16R2;
     const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : (r <= 128 || !two_term ? 4 : 8));
-    const int64_t BN = NT * 32, m_pad = round_up(m, 256), r_pad = round_up(r, BN);
+    const int64_t BN = NT * 32, m_pad = round_up(m, em == EM_F16 && NT == 4 ? 512 : 256),
+                  r_pad = round_up(r, BN);
     void *P, *Q, *AT;
     TSVM_HIP(c, c->ws.get("kp_P", m_pad * KD * 2, &P));
     TSVM_HIP(c, c->ws.get("kp_Q", n_pad * KD * 2, &Q));
