@@ -257,8 +257,10 @@ __global__ void prep_A_f32(const float* __restrict__ A, int64_t lda, const float
 }
 
 // -------------------------------------------------------------------------------------
-// Low-precision main kernel.  256 threads = 4 waves; wave w owns rows [64w, 64w+64) of
-// the block's 256 rows (two 32-row MFMA blocks) and all BN = 32*NT columns of the block.
+// Low-precision main kernel.  THR threads = THR/64 waves (256, or 512 for the fp16
+// 128-column form: one block per CU whose eight waves share each staged stage); wave w
+// owns rows [64w, 64w+64) of the block's THR rows (two 32-row MFMA blocks) and all
+// BN = 32*NT columns of the block.
 // Per 32-j chunk: S^T via KD/16 MFMA 32x32x16 bf16 per row block, exp2 on 16 values per
 // lane per row block, then the E.A' MFMA (2*NT per row block and term) for O += E.A'.
 // LDS (double-buffered): Qop chunk 32 x KD, AT chunk BN x AW (row stride padded).
