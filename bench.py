@@ -47,12 +47,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# HBM bytes per launch of the sketch kernel at this workload: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch) scaled by the factor that the same counters give for the
-# bench's 3.28 GB Omega copy (copyBuffer) -- see the profile named in "source"
 # HBM bytes per launch of the factor's product kernel at the default workload (mean of its
-# two launches; rocprof FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, calibrated on a copy)
-KPROD_TRAFFIC = {"fetch_bytes": 5_681_017_946_112, "write_bytes": 3_280_994_304,
+# two launches): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the
+# factor that the same counters give for the bench's 3.28 GB Omega copy (x2.0 and x1.0)
+KPROD_TRAFFIC = {"fetch_bytes": 1_018_808_003_321, "write_bytes": 3_276_800_000,
                  "source": "profiles/kprod_factor_c2_pmc.md"}
 
 
@@ -254,7 +252,7 @@ def main():
     t1, t2 = bf16_info["lra"]["t_product1"], bf16_info["lra"]["t_product2"]
     tr = KPROD_TRAFFIC
     if args.precision == "bf16":
-        peak, kname = PEAK_BF16, "kprod_bf16<4,64,2,EM_F16,2> (fused RBF product, K_Y Omega and K_Y W)"
+        peak, kname = PEAK_BF16, "kprod_bf16<4,64,2,EM_F16,1,512> (fused RBF product, K_Y Omega and K_Y W)"
     else:
         peak, kname = PEAK_FP32, "kprod_f32 (fused RBF product, K_Y Omega and K_Y W)"
     t_kp = 0.5 * (t1 + t2)
