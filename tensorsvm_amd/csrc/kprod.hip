@@ -39,6 +39,11 @@ namespace tsvm {
 
 // 32-j chunks per pipeline stage of the BF16 kernel (the j padding is a multiple of it)
 #define KBF16_CH 2
+// Widest operand (KD) taken by the 512-thread fp16 form.  At KD = 176 and 256 its resident
+// P rows push it past 256 VGPRs (339 and 192 spilled) while the 256-thread form spills
+// none, so those widths keep two 256-thread blocks per CU; at KD = 96-128 both forms
+// spill and the 512-thread form less (4-94 against 92-129).
+#define KBF16_WIDE_MAX_KD 128
 
 // position of j (0..31) inside a permuted 32-j chunk row (see kprod_bf16 B operand)
 __device__ __host__ inline int perm32(int j) {
@@ -258,7 +263,8 @@ __global__ void prep_A_f32(const float* __restrict__ A, int64_t lda, const float
 
 // -------------------------------------------------------------------------------------
 // Low-precision main kernel.  THR threads = THR/64 waves (256, or 512 for the fp16
-// 128-column form: one block per CU whose eight waves share each staged stage); wave w
+// 128-column form with KD <= 128: one block per CU whose eight waves share each staged
+// stage); wave w
 // owns rows [64w, 64w+64) of the block's THR rows (two 32-row MFMA blocks) and all
 // BN = 32*NT columns of the block.
 // Per 32-j chunk: S^T via KD/16 MFMA 32x32x16 bf16 per row block, exp2 on 16 values per
@@ -588,7 +594,7 @@ static void launch_bf16(int em, const __bf16* P, const __bf16* Q, const void* AT
                          y1, W, ldw, m, r, n_chunks / CH, r_pad, sc, nxb, ntiles);
       break;
     case EM_F16:
-      if constexpr (NT == 4) {
+      if constexpr (NT == 4 && KD <= KBF16_WIDE_MAX_KD) {
         // one 512-thread block per CU: eight waves share each staged stage
         const int64_t nxb2 = m_pad / 512, ntiles2 = nxb2 * (r_pad / (NT * 32));
         dim3 grid2((unsigned)(ntiles2 < cu_count ? ntiles2 : cu_count));
@@ -659,7 +665,8 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
       return fail(c, TSVM_ERR_ARG, "kernel_product(bf16): d=%ld exceeds 84 features", (long)d);
     const bool two_term = em == EM_F16X2 || em == EM_F16R2;
     const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : (r <= 128 || !two_term ? 4 : 8));
-    const int64_t BN = NT * 32, m_pad = round_up(m, em == EM_F16 && NT == 4 ? 512 : 256),
+    const int64_t BN = NT * 32, m_pad = round_up(
+                      m, em == EM_F16 && NT == 4 && KD <= KBF16_WIDE_MAX_KD ? 512 : 256),
                   r_pad = round_up(r, BN);
     void *P, *Q, *AT;
     TSVM_HIP(c, c->ws.get("kp_P", m_pad * KD * 2, &P));
