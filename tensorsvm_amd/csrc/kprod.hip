@@ -41,8 +41,8 @@ namespace tsvm {
 #define KBF16_CH 2
 // Widest operand (KD) taken by the 512-thread fp16 form.  At KD = 176 and 256 its resident
 // P rows push it past 256 VGPRs (339 and 192 spilled) while the 256-thread form spills
-// none, so those widths keep two 256-thread blocks per CU; at KD = 96-128 both forms
-// spill and the 512-thread form less (4-94 against 92-129).
+// none, so those widths keep 256-thread blocks (one per CU there: 84-105 KB of LDS each);
+// at KD = 96-128 both forms spill and the 512-thread form less (4-94 against 92-129).
 #define KBF16_WIDE_MAX_KD 128
 
 // position of j (0..31) inside a permuted 32-j chunk row (see kprod_bf16 B operand)
