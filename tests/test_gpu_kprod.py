@@ -84,7 +84,8 @@ def test_kprod_fp16x2(ctx, m, n, d, r):
 @pytest.mark.parametrize("m,n,d,r", [(300, 257, 18, 40), (1000, 1000, 18, 100), (513, 700, 54, 100),
                                      (1100, 3001, 18, 300), (129, 64, 32, 256), (77, 33, 5, 7),
                                      (2048, 2048, 18, 512), (40, 5000, 18, 128),
-                                     (1100, 3001, 41, 300), (700, 900, 43, 200)])
+                                     (1100, 3001, 41, 300), (700, 900, 43, 200),
+                                     (300, 700, 64, 200), (300, 700, 84, 130)])
 def test_kprod_fp16(ctx, m, n, d, r):
     """Plain fp16 form (the factor products of the BF16 training form): kernel values and
     the scaled right-hand side each rounded once to fp16 (relative 2^-11), distance as in
