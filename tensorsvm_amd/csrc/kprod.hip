@@ -408,7 +408,11 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
             e[rb][kap][tt] = hi;
             if constexpr (EM == EM_F16X2) el[rb][kap][tt] = (ET)(v - (float)hi);
           }
-      // O += E . A'_j (per term of the split forms)
+      // O += E . A'_j (per term of the split forms).  In the 512-thread form the wave
+      // raises its issue priority over the product MFMAs, so that of the two waves on a
+      // SIMD the one with MFMAs ready keeps the matrix pipe fed while the other runs its
+      // exp2 work (3.14 -> 3.06 s per product at 1.6M x 1.6M x 512).
+      if constexpr (THR == 512) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int ct = 0; ct < NT; ct++) {
 #pragma unroll
@@ -433,6 +437,7 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
           }
         }
       }
+      if constexpr (THR == 512) __builtin_amdgcn_s_setprio(0);
     }
     // keep the LDS stores of the next stage after this stage's MFMA work
     __builtin_amdgcn_sched_barrier(0);
