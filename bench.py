@@ -137,8 +137,8 @@ def main():
                     help="form of the timed steps (the comparison run, if any, is FP32)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the trainer is "
-              "single-GPU and multi-GPU sharding is not part of this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the trainer "
+              "runs on one GPU and does not shard its work across GPUs", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

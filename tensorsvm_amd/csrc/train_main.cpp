@@ -103,11 +103,24 @@ int main(int argc, char* argv[]) {
       }
     } else if (a == "-stop") {
       std::string p = next();
-      prm.stop_mode = p == "gap" ? TSVM_STOP_GAP : TSVM_STOP_REF;
+      if (p == "gap") prm.stop_mode = TSVM_STOP_GAP;
+      else if (p == "ref") prm.stop_mode = TSVM_STOP_REF;
+      else {
+        fprintf(stderr, "unknown stop rule %s\n", p.c_str());
+        exit(1);
+      }
     } else if (a == "-tol") prm.tol = atof(next());
     else if (a == "-refine") prm.refine = atoi(next());
     else if (a == "-maxit") prm.maxit = atoi(next());
-    else if (a == "-init") prm.init = std::string(next()) == "feasible" ? 1 : 0;
+    else if (a == "-init") {
+      std::string p = next();
+      if (p == "feasible") prm.init = 1;
+      else if (p == "ref") prm.init = 0;
+      else {
+        fprintf(stderr, "unknown initial point %s\n", p.c_str());
+        exit(1);
+      }
+    }
     else if (a == "-q") prm.verbose = 0;
     else if (a == "-json") json_path = next();
     else if (!train_path) train_path = argv[i];
