@@ -62,18 +62,19 @@ __global__ __launch_bounds__(64) void chol_diag(double* __restrict__ M, int64_t 
     if (lane == 0) *info = (int)(j0 + bad);
     return;
   }
-  // column c of the inverse: Li[i][c] = (e_c[i] - sum_{c <= t < i} L[i][t] Li[t][c]) / L[i][i],
-  // lane c walks down its column; the factor, the reciprocal diagonal and the inverse sit
-  // in LDS
+  if (row)
+#pragma unroll
+    for (int t = 0; t < CB; t++)
+      if (t < jb) M[(j0 + lane) * k + j0 + t] = t <= lane ? a[t] : 0.0;
+  // column c of the inverse: Li[i][c] = (e_c[i] - sum_{t < i} L[i][t] Li[t][c]) / L[i][i],
+  // with Li[t][c] = 0 above the diagonal (t < c).  Lane c keeps its column in registers
+  // (unrolled loops, no LDS round trip between the dependent steps); the factor and its
+  // reciprocal diagonal are read from LDS, every lane at the same address.
   __shared__ double Ls[CB][CB + 1];
-  __shared__ double Is[CB][CB + 1];
   __shared__ double Rd[CB];
   if (lane < CB) {
 #pragma unroll
-    for (int t = 0; t < CB; t++) {
-      Ls[lane][t] = a[t];
-      Is[t][lane] = 0.0;
-    }
+    for (int t = 0; t < CB; t++) Ls[lane][t] = a[t];
     double dl = 0.0;  // L[lane][lane]
 #pragma unroll
     for (int t = 0; t < CB; t++)
@@ -83,19 +84,17 @@ __global__ __launch_bounds__(64) void chol_diag(double* __restrict__ M, int64_t 
   __syncthreads();
   if (lane < CB) {
     const int c = lane;
-#pragma unroll 1
-    for (int i = c; i < CB; i++) {
-      double v = i == c ? 1.0 : 0.0;
-      for (int t = c; t < i; t++) v -= Ls[i][t] * Is[t][c];
-      Is[i][c] = v * Rd[i];
-    }
-  }
-  if (row)
+    double x[CB];
 #pragma unroll
-    for (int t = 0; t < CB; t++)
-      if (t < jb) M[(j0 + lane) * k + j0 + t] = t <= lane ? a[t] : 0.0;
-  if (lane < CB)
-    for (int i = 0; i < CB; i++) Dinv[(j0 / CB) * CB * CB + i * CB + lane] = Is[i][lane];
+    for (int i = 0; i < CB; i++) {
+      double v = i == c ? 1.0 : 0.0;
+#pragma unroll
+      for (int t = 0; t < i; t++) v -= Ls[i][t] * x[t];
+      x[i] = i >= c ? v * Rd[i] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < CB; i++) Dinv[(j0 / CB) * CB * CB + i * CB + c] = x[i];
+  }
 }
 
 // rows below the diagonal block: L_i[c] = sum_{t <= c} A_i[t] Linv[c][t]; a block takes 8
