@@ -298,7 +298,6 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
   constexpr int AS = AW * CH + 8;   // LDS row stride (elements) of the AT stage
   constexpr int QV = 32 * CH * KD / 8;   // 16-byte vectors per Qop stage (contiguous)
   constexpr int AV = BN * AW * CH / 8;   // 16-byte vectors per AT stage
-  constexpr int AVC = BN * AW / 8;       // 16-byte vectors per AT chunk
   constexpr int NQ = (QV + THR - 1) / THR;
   constexpr int NA = (AV + THR - 1) / THR;
   __shared__ __attribute__((aligned(16))) __bf16 sQ[2][32 * CH * QS];
@@ -331,7 +330,9 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
       for (int q = 0; q < 16; q++) acc[rb][ct][q] = 0.0f;
 
   // Staging: the Qop stage is one contiguous run; the AT stage is CH chunks, each a
-  // contiguous [r_pad][AW] block of which this column block reads BN rows.  Loads are
+  // contiguous [r_pad][AW] block of which this column block reads BN rows.  Consecutive
+  // lanes take one whole LDS row of the AT stage (the CH chunks' pieces side by side), so
+  // a group of lanes stores one padded row instead of half-rows of two.  Loads are
   // branch-free (out-of-range lanes re-read a valid vector and skip the store) so that
   // they stay in flight across the compute of the current stage.
   u32x4_t sq[NQ], sa[NA];
@@ -346,9 +347,10 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
     for (int v = 0; v < NA; v++) {
       int idx = tid + v * THR;
       idx = idx < AV ? idx : AV - 1;
-      int ch = idx / AVC, rem = idx - ch * AVC;
+      const int row = idx / (CH * AW / 8), j = idx - row * (CH * AW / 8);
+      const int ch = j / (AW / 8), q = j - ch * (AW / 8);
       const u32x4_t* ga = (const u32x4_t*)(AT + ((t * CH + ch) * r_pad + c0) * AW);
-      sa[v] = ga[rem];
+      sa[v] = ga[row * (AW / 8) + q];
     }
   };
   auto sstore = [&](int buf) {
@@ -364,9 +366,8 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
     for (int v = 0; v < NA; v++) {
       int idx = tid + v * THR;
       if (AV % THR == 0 || idx < AV) {
-        int ch = idx / AVC, rem = idx - ch * AVC;
-        int row = rem / (AW / 8), col = ch * AW + (rem % (AW / 8)) * 8;
-        *(u32x4_t*)&sA[buf][row * AS + col] = sa[v];
+        const int row = idx / (CH * AW / 8), j = idx - row * (CH * AW / 8);
+        *(u32x4_t*)&sA[buf][row * AS + j * 8] = sa[v];
       }
     }
   };
