@@ -50,7 +50,7 @@ def log(*a):
 # HBM bytes per launch of the factor's product kernel at the default workload (mean of its
 # two launches): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the
 # factor that the same counters give for the bench's 3.28 GB Omega copy (x2.0 and x1.0)
-KPROD_TRAFFIC = {"fetch_bytes": 251_585_856_448, "write_bytes": 3_276_800_000,
+KPROD_TRAFFIC = {"fetch_bytes": 251_517_896_384, "write_bytes": 3_276_800_000,
                  "source": "profiles/kprod_factor_c2_pmc.md"}
 
 
