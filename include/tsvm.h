@@ -43,6 +43,12 @@ typedef int tsvm_status;
 #define TSVM_PREC_FP16 3       /* E and A fp16 (2^-11 each), one MFMA per tile */
 #define TSVM_PREC_FP16R2 4     /* E fp16 (2^-11), A as two fp16 terms (~2^-22) */
 
+/* Largest rank k of the factor (and width of Z in the interior-point solver): the
+ * capacitance Cholesky and its solves keep a k x k FP64 matrix per workgroup.  The
+ * reference's own limit is the k x k host eigen-solve (LRA, tensorsvm-train.cu:2180-2212);
+ * its usage lines take k = 32 and 400 (README.md). */
+#define TSVM_MAX_RANK 1024
+
 typedef struct tsvm_ctx tsvm_ctx;
 
 /* Context: device ordinal, one HIP stream, reusable device workspace.
@@ -264,6 +270,8 @@ typedef struct {
   double b;           /* bias; rho = -b */
   double b_std;       /* spread of the per-free-SV bias samples (reference "std b") */
   int b_samples;      /* free SVs averaged (<= 50 RBF, <= 5000 linear) */
+  int b_slack_rule;   /* RBF: 1 when the samples are the slack-confirmed free SVs (relative-
+                         gap stop), 0 for the reference's set (1e-6 < alpha < C - 1e-6) */
   int64_t nsv, nfree; /* alpha > 1e-6; and alpha < C - 1e-6 */
   double t_bias, t_total;  /* s */
   double fnorm, fnorm_res; /* -analysis results (0 unless requested) */

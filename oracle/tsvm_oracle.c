@@ -315,24 +315,36 @@ void orc_sym_eig(const double *A0, int64_t k, double *w, double *V) {
 /* ------------------------------------------------------------------------- */
 /* LRA (1941-2249).                                                            */
 /* ------------------------------------------------------------------------- */
-int64_t orc_lra(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
-                double gamma, const float *omega, double *U, double *eig_out) {
+/* round every entry to FP32 (the reference keeps W, Q, C, X and U in float storage) */
+static void round_f32(double *a, int64_t cnt) {
+  for (int64_t i = 0; i < cnt; i++) a[i] = (double)(float)a[i];
+}
+
+int64_t orc_lra_ex(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
+                   double gamma, const float *omega, int mode, double *U, double *eig_out) {
+  const int f32 = mode == ORC_LRA_REF_FP32;
+  if (n <= 0 || k <= 0) return 0;
   double *Om = (double *)malloc(sizeof(double) * n * k);
   double *W = (double *)malloc(sizeof(double) * n * k);
   double *Q = (double *)malloc(sizeof(double) * n * k);
   for (int64_t i = 0; i < n * k; i++) Om[i] = (double)omega[i];
   orc_kernel_product(X, y, n, X, y, n, d, Om, k, gamma, W); /* W = K_Y Omega (2067) */
+  if (f32) round_f32(W, n * k);                            /* float W (Z, 1986-1990) */
   orc_householder_q(W, n, k);                              /* ortho (2075) */
+  if (f32) round_f32(W, n * k);                            /* ortho(float *W, ...) */
   orc_kernel_product(X, y, n, X, y, n, d, W, k, gamma, Q); /* Q = K_Y W (2085) */
+  if (f32) round_f32(Q, n * k);                            /* float Q */
   double *C = (double *)calloc((size_t)(k * k), sizeof(double));
-  for (int64_t i = 0; i < n; i++) /* C = W^T Q (2122-2149) */
+  for (int64_t i = 0; i < n; i++) /* C = W^T Q, FP64 accumulation (2122-2149) */
     for (int64_t a = 0; a < k; a++) {
       double wa = W[i * k + a];
       for (int64_t b = 0; b < k; b++) C[a * k + b] += wa * Q[i * k + b];
     }
+  if (f32) round_f32(C, k * k); /* the FP64 sum lands in float C (2150) */
   for (int64_t a = 0; a < k; a++) /* symmetrise from the lower triangle (2152-2156) */
     for (int64_t b = 0; b < a; b++) {
       double v = 0.5 * (C[a * k + b] + C[b * k + a]);
+      if (f32) v = (double)(float)v;
       C[a * k + b] = v;
       C[b * k + a] = v;
     }
@@ -346,16 +358,28 @@ int64_t orc_lra(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
     else realk--;
     for (int64_t r = 0; r < k; r++) V[r * k + c] *= s;
   }
+  if (f32) round_f32(V, k * k); /* float X (2187, 2208) */
 #pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < n; i++) /* U = W X (2213-2241) */
+  for (int64_t i = 0; i < n; i++) /* U = W X (2213-2241): sgemm in the FP32 form */
     for (int64_t c = 0; c < k; c++) {
-      double acc = 0.0;
-      for (int64_t t = 0; t < k; t++) acc += W[i * k + t] * V[t * k + c];
-      U[i * k + c] = acc;
+      if (f32) {
+        float acc = 0.0f;
+        for (int64_t t = 0; t < k; t++) acc += (float)W[i * k + t] * (float)V[t * k + c];
+        U[i * k + c] = (double)acc;
+      } else {
+        double acc = 0.0;
+        for (int64_t t = 0; t < k; t++) acc += W[i * k + t] * V[t * k + c];
+        U[i * k + c] = acc;
+      }
     }
   if (eig_out) memcpy(eig_out, w, sizeof(double) * k);
   free(Om); free(W); free(Q); free(C); free(w); free(V);
   return realk;
+}
+
+int64_t orc_lra(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
+                double gamma, const float *omega, double *U, double *eig_out) {
+  return orc_lra_ex(X, y, n, d, k, gamma, omega, ORC_LRA_FP64, U, eig_out);
 }
 
 /* ------------------------------------------------------------------------- */

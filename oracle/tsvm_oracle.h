@@ -53,6 +53,18 @@ void orc_sym_eig(const double *A, int64_t k, double *w, double *V);
  * U n x k.  eig_out (may be NULL) receives the k eigenvalues; returns realk. */
 int64_t orc_lra(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
                 double gamma, const float *omega, double *U, double *eig_out);
+/* The same with a choice of storage precision:
+ *   ORC_LRA_FP64      every intermediate in FP64 (orc_lra);
+ *   ORC_LRA_REF_FP32  the reference's plain-method storage: W = K_Y Omega, the
+ *                     orthonormal basis and Q = K_Y W rounded to FP32 (float Z/W/Q),
+ *                     C = W^T Q accumulated in FP64 and stored as float (2143-2157),
+ *                     X = V sqrt(w) as float (2187, 2208) and U = W X as an FP32
+ *                     product (2213-2241).  The kernel products themselves are formed
+ *                     in FP64 and then rounded (cuBLAS's FP32 accumulation order is not
+ *                     reproducible off the reference's device). */
+enum { ORC_LRA_FP64 = 0, ORC_LRA_REF_FP32 = 1 };
+int64_t orc_lra_ex(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
+                   double gamma, const float *omega, int mode, double *U, double *eig_out);
 
 /* Interior-point solver (mpc 1507-1785, NewtonStep 1794-1822, SMWSolve 1824-1840). */
 enum { ORC_STOP_REF = 0, ORC_STOP_GAP = 1 };

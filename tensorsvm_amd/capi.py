@@ -133,7 +133,7 @@ class TrainParams(ctypes.Structure):
 
 class TrainInfo(ctypes.Structure):
     _fields_ = [("lra", LraInfo), ("ipm", IpmInfo), ("b", _D), ("b_std", _D), ("b_samples", _INT),
-                ("nsv", _I64), ("nfree", _I64), ("t_bias", _D), ("t_total", _D),
+                ("b_slack_rule", _INT), ("nsv", _I64), ("nfree", _I64), ("t_bias", _D), ("t_total", _D),
                 ("fnorm", _D), ("fnorm_res", _D), ("t_analysis", _D)]
 
     def as_dict(self):

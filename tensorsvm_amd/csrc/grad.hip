@@ -299,9 +299,9 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
   double* v2 = vec + 4 * n;
   double* v3 = vec + 5 * n;
   double* qv = vec + 6 * n;
-  hipEvent_t e0, e1;
-  TSVM_HIP(c, hipEventCreate(&e0));
-  TSVM_HIP(c, hipEventCreate(&e1));
+  Event e0, e1;
+  TSVM_HIP(c, e0.create());
+  TSVM_HIP(c, e1.create());
   double flops = 0.0;
   float ms_bwd = 0.f;
   if (gD) {
@@ -408,9 +408,9 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
     vals[1] = loss / (double)nt;
     if (gL) {
       TSVM_HIP(c, hipMemcpyAsync(qv, pt.data(), nt * 8, hipMemcpyHostToDevice, s));
-      hipEvent_t e2, e3;
-      TSVM_HIP(c, hipEventCreate(&e2));
-      TSVM_HIP(c, hipEventCreate(&e3));
+      Event e2, e3;
+      TSVM_HIP(c, e2.create());
+      TSVM_HIP(c, e3.create());
       TSVM_HIP(c, hipEventRecord(e2, s));
       const int64_t rpt = ceil_div(nt, GR_BLOCKS);
       const int nbt = (int)ceil_div(nt, rpt);
@@ -422,8 +422,6 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
       TSVM_CHECK(sum_parts(c, part, nbt, (int)d, S));
       float ms2 = 0.f;
       TSVM_HIP(c, hipEventElapsedTime(&ms2, e2, e3));
-      (void)hipEventDestroy(e2);
-      (void)hipEventDestroy(e3);
       double tot = 0.0;
       for (int64_t f = 0; f < d; f++) {
         gL[1 + f] = -2.0 * gamma * S[f];
@@ -436,8 +434,6 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
   } else if (gL) {
     for (int64_t f = 0; f <= d; f++) gL[f] = 0.0;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   inf.realk = k;
   inf.orth_passes = passes;
   inf.t_backward = ms_bwd * 1e-3;

@@ -45,6 +45,20 @@ tsvm_status fail(Ctx* c, tsvm_status st, const char* fmt, ...);
     if (_s != TSVM_OK) return _s;            \
   } while (0)
 
+// A HIP event destroyed with its scope, so that early returns (TSVM_HIP / TSVM_CHECK
+// inside a timed region) do not leak it.
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() = default;
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  hipError_t create() { return hipEventCreate(&e); }
+  operator hipEvent_t() const { return e; }
+};
+
 // ---- kernel product (kprod.hip) -------------------------------------------------
 // W[i, c] = y1_i * sum_j y2_j exp(-gamma |x1_i - x2_j|^2) A[j, c], all device pointers.
 // X1 m x d, X2 n x d row-major; A n x r with leading dimension lda; W m x r, ldw.

@@ -1081,7 +1081,8 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
                     const tsvm_ipm_params* prm, double* x, double* xi, double* s_out,
                     tsvm_ipm_info* info) {
   const double t_start = now();
-  if (n <= 0 || k <= 0 || k > 1024) return fail(c, TSVM_ERR_ARG, "ipm: need n > 0, 0 < k <= 1024");
+  if (n <= 0 || k <= 0 || k > TSVM_MAX_RANK)
+    return fail(c, TSVM_ERR_ARG, "ipm: need n > 0, 0 < k <= %d", TSVM_MAX_RANK);
   if (!(prm->C > 0)) return fail(c, TSVM_ERR_ARG, "ipm: C must be positive");
   hipStream_t st = c->stream;
   const int kp = (int)round_up(k, 32);
@@ -1220,12 +1221,12 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   }
   TSVM_HIP(c, hipGetLastError());
 
-  hipEvent_t ev0, ev1;
-  TSVM_HIP(c, hipEventCreate(&ev0));
-  TSVM_HIP(c, hipEventCreate(&ev1));
-  hipEvent_t evb[2];
-  TSVM_HIP(c, hipEventCreate(&evb[0]));
-  TSVM_HIP(c, hipEventCreate(&evb[1]));
+  Event ev0, ev1, evb0, evb1;
+  TSVM_HIP(c, ev0.create());
+  TSVM_HIP(c, ev1.create());
+  TSVM_HIP(c, evb0.create());
+  TSVM_HIP(c, evb1.create());
+  hipEvent_t evb[2] = {evb0, evb1};
   double t_syrk = 0.0, t_bf16 = 0.0, row_frac_sum = 0.0;
   int n_syrk = 0, n_bf16 = 0;
   bool pending_syrk = false, pending_bf16 = false;
@@ -1375,10 +1376,6 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_HIP(c, hipGetLastError());
   }
   TSVM_HIP(c, hipStreamSynchronize(st));
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
-  (void)hipEventDestroy(evb[0]);
-  (void)hipEventDestroy(evb[1]);
   inf.iters = iter;
   inf.reason = reason;
   inf.t_total = now() - t_start;

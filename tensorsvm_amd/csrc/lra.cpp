@@ -146,9 +146,9 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   std::vector<double> G(k * k), L(k * k), Li(k * k), T(k * k);
 
   // 1. W = K_Y Omega (timed with events on the stream it runs on)
-  hipEvent_t e0, e1;
-  TSVM_HIP(c, hipEventCreate(&e0));
-  TSVM_HIP(c, hipEventCreate(&e1));
+  Event e0, e1;
+  TSVM_HIP(c, e0.create());
+  TSVM_HIP(c, e1.create());
   TSVM_HIP(c, hipEventRecord(e0, s));
   // low-precision forms: the sketch in fp16 (BF16 form) or in the two-term product
   const int sketch_prec = precision == TSVM_PREC_BF16 ? TSVM_PREC_FP16 : precision;
@@ -174,8 +174,6 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   TSVM_HIP(c, hipEventRecord(e1, s));
   TSVM_HIP(c, hipStreamSynchronize(s));
   TSVM_HIP(c, hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   double t3 = now();
   inf.t_product2 = ms * 1e-3;
 
@@ -189,30 +187,84 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
     for (int64_t j = 0; j < k; j++) Cs[i * k + j] = 0.5 * (Cb[i * k + j] + Cb[j * k + i]);
   // 6. C = L^-1 Cs L^-T, eig, X = L^-T V sqrt(w+)
   memcpy(L.data(), G.data(), k * k * 8);
-  if (host_chol(L.data(), k) != 0) return fail(c, TSVM_ERR_NUMERIC, "lra: basis Gram not SPD");
-  host_tri_inv_lower(L.data(), k, Li.data());
-  host_gemm(Li.data(), Cs.data(), k, T.data());  // T = L^-1 Cs
   std::vector<double> LiT(k * k);
-  for (int64_t i = 0; i < k; i++)
-    for (int64_t j = 0; j < k; j++) LiT[i * k + j] = Li[j * k + i];
-  host_gemm(T.data(), LiT.data(), k, Cb.data());  // Cb = L^-1 Cs L^-T
-  for (int64_t i = 0; i < k; i++)
-    for (int64_t j = 0; j < i; j++) {
-      double v = 0.5 * (Cb[i * k + j] + Cb[j * k + i]);
-      Cb[i * k + j] = v;
-      Cb[j * k + i] = v;
-    }
-  const double tc1 = now();
-  TSVM_CHECK(host_sym_eig(c, Cb.data(), k, w.data(), V.data()));
-  const double tc2 = now();
+  double tc1, tc2;
   int64_t realk = k;
-  for (int64_t j = 0; j < k; j++) {
-    double sc = 0.0;
-    if (w[j] > 0) sc = sqrt(w[j]);
-    else realk--;
-    for (int64_t i = 0; i < k; i++) V[i * k + j] *= sc;
+  if (host_chol(L.data(), k) == 0) {
+    host_tri_inv_lower(L.data(), k, Li.data());
+    host_gemm(Li.data(), Cs.data(), k, T.data());  // T = L^-1 Cs
+    for (int64_t i = 0; i < k; i++)
+      for (int64_t j = 0; j < k; j++) LiT[i * k + j] = Li[j * k + i];
+    host_gemm(T.data(), LiT.data(), k, Cb.data());  // Cb = L^-1 Cs L^-T
+    for (int64_t i = 0; i < k; i++)
+      for (int64_t j = 0; j < i; j++) {
+        double v = 0.5 * (Cb[i * k + j] + Cb[j * k + i]);
+        Cb[i * k + j] = v;
+        Cb[j * k + i] = v;
+      }
+    tc1 = now();
+    TSVM_CHECK(host_sym_eig(c, Cb.data(), k, w.data(), V.data()));
+    tc2 = now();
+    for (int64_t j = 0; j < k; j++) {
+      double sc = 0.0;
+      if (w[j] > 0) sc = sqrt(w[j]);
+      else realk--;
+      for (int64_t i = 0; i < k; i++) V[i * k + j] *= sc;
+    }
+    host_gemm(LiT.data(), V.data(), k, T.data());  // X = L^-T V sqrt(w+)
+  } else {
+    // Rank-deficient sketch (gamma = 0 gives K = 1 1^T; duplicated rows): range(K_Y Omega)
+    // has dimension r < k and the basis Gram is singular.  With G = Vg diag(g) Vg^T and the
+    // r directions g_i > 1e-8 max g kept, B = Vg_r diag(g_r^-1/2) makes W B an orthonormal
+    // basis of range(W); C_r = B^T Cs B (r x r) = Vc diag(w) Vc^T and X = B Vc sqrt(w+),
+    // padded with k - r zero columns (zero eigenvalues, clipped like the reference's
+    // non-positive ones, 2199-2210).  U U^T is again the projected kernel P K_Y P.
+    std::vector<double> gw(k), gv(k * k);
+    TSVM_CHECK(host_sym_eig(c, G.data(), k, gw.data(), gv.data()));
+    const double gmax = gw[k - 1];
+    if (!(gmax > 0)) return fail(c, TSVM_ERR_NUMERIC, "lra: the sketch K_Y Omega is zero");
+    int64_t r0 = 0;  // first kept direction (ascending eigenvalues)
+    while (r0 < k && !(gw[r0] > 1e-8 * gmax)) r0++;
+    const int64_t r = k - r0;
+    std::vector<double> B(k * r), Cr(r * r), Tr(k * r), wr(r), Vr(r * r);
+    for (int64_t i = 0; i < k; i++)
+      for (int64_t j = 0; j < r; j++) B[i * r + j] = gv[i * k + r0 + j] / sqrt(gw[r0 + j]);
+    for (int64_t i = 0; i < k; i++)  // Tr = Cs B (k x r)
+      for (int64_t j = 0; j < r; j++) {
+        double acc = 0.0;
+        for (int64_t t = 0; t < k; t++) acc += Cs[i * k + t] * B[t * r + j];
+        Tr[i * r + j] = acc;
+      }
+    for (int64_t i = 0; i < r; i++)  // Cr = B^T Tr, symmetrised
+      for (int64_t j = 0; j < r; j++) {
+        double acc = 0.0;
+        for (int64_t t = 0; t < k; t++) acc += B[t * r + i] * Tr[t * r + j];
+        Cr[i * r + j] = acc;
+      }
+    for (int64_t i = 0; i < r; i++)
+      for (int64_t j = 0; j < i; j++) {
+        double v = 0.5 * (Cr[i * r + j] + Cr[j * r + i]);
+        Cr[i * r + j] = v;
+        Cr[j * r + i] = v;
+      }
+    tc1 = now();
+    TSVM_CHECK(host_sym_eig(c, Cr.data(), r, wr.data(), Vr.data()));
+    tc2 = now();
+    realk = 0;
+    for (int64_t j = 0; j < k; j++) w[j] = j < r0 ? 0.0 : wr[j - r0];
+    for (int64_t j = 0; j < r; j++) {
+      const double sc = wr[j] > 0 ? sqrt(wr[j]) : 0.0;
+      if (wr[j] > 0) realk++;
+      for (int64_t i = 0; i < r; i++) Vr[i * r + j] *= sc;
+    }
+    for (int64_t i = 0; i < k; i++)  // X = B Vr sqrt(w+), columns r0.. (ascending order)
+      for (int64_t j = 0; j < k; j++) {
+        double acc = 0.0;
+        if (j >= r0)
+          for (int64_t t = 0; t < r; t++) acc += B[i * r + t] * Vr[t * r + (j - r0)];
+        T[i * k + j] = acc;
+      }
   }
-  host_gemm(LiT.data(), V.data(), k, T.data());  // X = L^-T V sqrt(w+)
   inf.realk = realk;
   inf.eig_max = w[k - 1];
   inf.eig_min = w[0];

@@ -34,7 +34,8 @@ constexpr int SY_BK = 64;        // rows (the GEMM's K) per SYRK stage
 constexpr int SY_LS = SY_BK + 8;  // LDS row stride (bf16) of a staged panel
 
 __device__ inline int hs_bin(double t) {
-  if (!(t > 0.0)) return 0;
+  if (!(t > 0.0)) return 0;  // zero, negative and NaN weights
+  if (!(t < 1e300)) return HS_BINS - 1;
   int e = ilogb(t) + 64;
   return e < 0 ? 0 : (e > HS_BINS - 1 ? HS_BINS - 1 : e);
 }
@@ -618,9 +619,10 @@ tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, cons
     double hsl[2];
     TSVM_HIP(c, hipMemcpyAsync(hsl, slots, 16, hipMemcpyDeviceToHost, s));
     TSVM_HIP(c, hipStreamSynchronize(s));
-    hipEvent_t ev[2];
-    TSVM_HIP(c, hipEventCreate(&ev[0]));
-    TSVM_HIP(c, hipEventCreate(&ev[1]));
+    Event ev0, ev1;
+    TSVM_HIP(c, ev0.create());
+    TSVM_HIP(c, ev1.create());
+    hipEvent_t ev[2] = {ev0, ev1};
     const int64_t nh = (int64_t)hsl[1];
     tsvm_status st = hybrid_syrk_form(c, &hs, Z, w, nh, M, ev);
     if (st == TSVM_OK) {
@@ -634,8 +636,6 @@ tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, cons
         if (ms > 0) inf.bf16_tflops = hybrid_syrk_flops(&hs) / (ms * 1e-3) * 1e-12;
       }
     }
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
     if (st != TSVM_OK) return st;
   }
   TSVM_HIP(c, hipStreamSynchronize(s));

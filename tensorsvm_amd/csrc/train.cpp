@@ -42,7 +42,9 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
                           tsvm_train_info* info) {
   const double t0 = now();
   const int64_t k = prm->k;
-  if (k <= 0 || k > n) return fail(c, TSVM_ERR_ARG, "train: need 0 < k <= n");
+  if (k <= 0 || k > n || k > TSVM_MAX_RANK)
+    return fail(c, TSVM_ERR_ARG, "train: need 0 < k <= min(n, %d) (k=%ld, n=%ld)", TSVM_MAX_RANK,
+                (long)k, (long)n);
   struct ProgressScope {  // progress lines for the duration of this call
     Ctx* c;
     int saved;
@@ -78,24 +80,32 @@ tsvm_status train_rbf_dev(Ctx* c, const float* X, const float* y, int64_t n, int
   TSVM_CHECK(ipm_dev(c, U, y, n, k, &ip, alpha, (double*)pxi, (double*)ps, &inf.ipm));
   if (prm->verbose) printf("\x1b[95mMPC elapsed time = %.3f seconds\x1b[39m\n", inf.ipm.t_total);
   // bias (writemodel 353-376).  The samples are the first <= 50 free SVs
-  // (1e-6 < alpha < C - 1e-6); a multiplier counts as free only where the IPM's slacks
-  // agree (s < alpha and xi < C - alpha), which at a converged point is the same set and
-  // at a looser stop (relative gap) keeps not-yet-vanished alphas of non-SVs out.
+  // (1e-6 < alpha < C - 1e-6), the reference's set, under the reference stop rule.  Under
+  // the relative-gap stop the multipliers of non-SVs have not yet vanished (~mu/s, above
+  // 1e-6 at c2), so there a free SV must also have IPM slacks below its distances to the
+  // bounds (s < alpha, xi < C - alpha); when no multiplier passes that test the
+  // reference's set is taken, so the bias is NaN only without free SVs, as in the
+  // reference (372).
   const double tb = now();
   std::vector<double> ha(n), hs(n), hxi(n);
   TSVM_HIP(c, hipMemcpyAsync(ha.data(), alpha, n * 8, hipMemcpyDeviceToHost, c->stream));
   TSVM_HIP(c, hipMemcpyAsync(hs.data(), ps, n * 8, hipMemcpyDeviceToHost, c->stream));
   TSVM_HIP(c, hipMemcpyAsync(hxi.data(), pxi, n * 8, hipMemcpyDeviceToHost, c->stream));
   TSVM_HIP(c, hipStreamSynchronize(c->stream));
-  std::vector<int64_t> freeidx;
+  std::vector<int64_t> freeidx, freeref;
+  const bool slack_rule = prm->stop_mode != TSVM_STOP_REF;
   for (int64_t i = 0; i < n; i++)
     if (ha[i] > 1e-6) {
       inf.nsv++;
       if (ha[i] < prm->C - 1e-6) {
         inf.nfree++;
-        if (freeidx.size() < 50 && hs[i] < ha[i] && hxi[i] < prm->C - ha[i]) freeidx.push_back(i);
+        if (freeref.size() < 50) freeref.push_back(i);
+        if (slack_rule && freeidx.size() < 50 && hs[i] < ha[i] && hxi[i] < prm->C - ha[i])
+          freeidx.push_back(i);
       }
     }
+  inf.b_slack_rule = slack_rule && !freeidx.empty() ? 1 : 0;
+  if (!inf.b_slack_rule) freeidx.swap(freeref);
   std::vector<double> v(k);
   TSVM_CHECK(masked_colsum(c, U, n, k, k, alpha, 1e-6, v.data()));
   std::vector<float> urow(k);
@@ -175,6 +185,9 @@ tsvm_status tsvm_train(tsvm_ctx* ctx, const float* X, const float* y, int64_t n,
   TSVM_HIP(c, hipMemcpyAsync(dy, y, n * 4, hipMemcpyHostToDevice, s));
   if (prm->kernel_type == 0) {
     // linear branch (907-953): Z = diag(y) X, k = d
+    if (d > TSVM_MAX_RANK)
+      return tsvm::fail(c, TSVM_ERR_ARG, "train: the linear branch needs d <= %d (d=%ld)",
+                        TSVM_MAX_RANK, (long)d);
     std::vector<float> Z(n * d);
     for (int64_t i = 0; i < n; i++)
       for (int64_t t = 0; t < d; t++) Z[i * d + t] = y[i] * X[i * d + t];
@@ -232,7 +245,9 @@ tsvm_status tsvm_train(tsvm_ctx* ctx, const float* X, const float* y, int64_t n,
   }
   if (prm->kernel_type != 2) return tsvm::fail(c, TSVM_ERR_ARG, "unimplemented -t %d", prm->kernel_type);
   const int64_t k = prm->k;
-  if (k <= 0 || k > n) return tsvm::fail(c, TSVM_ERR_ARG, "train: need 0 < k <= n");
+  if (k <= 0 || k > n || k > TSVM_MAX_RANK)
+    return tsvm::fail(c, TSVM_ERR_ARG, "train: need 0 < k <= min(n, %d) (k=%ld, n=%ld)",
+                      TSVM_MAX_RANK, (long)k, (long)n);
   void* dOm;
   TSVM_HIP(c, c->ws.get("h_trOm", n * k * 4, &dOm));
   TSVM_HIP(c, hipMemcpyAsync(dX, X, n * d * 4, hipMemcpyHostToDevice, s));

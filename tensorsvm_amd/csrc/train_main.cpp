@@ -131,6 +131,10 @@ int main(int argc, char* argv[]) {
     return 1;
   }
   if (model_path.empty()) model_path = std::string(train_path) + ".model";
+  if (prm.kernel_type == 2 && (prm.k <= 0 || prm.k > TSVM_MAX_RANK)) {
+    fprintf(stderr, "Error: -k %ld outside 1..%d\n", (long)prm.k, TSVM_MAX_RANK);
+    return 1;
+  }
   const bool v = prm.verbose != 0;
 
   clock_t before = clock();

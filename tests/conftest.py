@@ -18,3 +18,16 @@ def ctx():
     c = T.Context(0)
     yield c
     c.close()
+
+
+def parity_log(name, **vals):
+    """Append the measured differences of a parity test to $TSVM_PARITY_LOG (JSON lines)
+    when that variable is set; the assertions do not depend on it."""
+    path = os.environ.get("TSVM_PARITY_LOG")
+    if not path:
+        return
+    import json
+    rec = {"test": name}
+    rec.update({k: (float(v) if hasattr(v, "__float__") else v) for k, v in vals.items()})
+    with open(path, "a") as f:
+        f.write(json.dumps(rec) + "\n")

@@ -40,6 +40,8 @@ def lib():
         L.orc_sym_eig.argtypes = [P, I64, P, P]
         L.orc_lra.argtypes = [P, P, I64, I64, I64, D, P, P, P]
         L.orc_lra.restype = I64
+        L.orc_lra_ex.argtypes = [P, P, I64, I64, I64, D, P, ctypes.c_int, P, P]
+        L.orc_lra_ex.restype = I64
         L.orc_mpc.argtypes = [P, P, I64, I64, D, ctypes.c_int, ctypes.c_int, ctypes.c_int, D,
                               P, P, P, ctypes.POINTER(IpmInfo)]
         L.orc_smw_solve.argtypes = [P, P, I64, I64, P]
@@ -132,12 +134,18 @@ def sym_eig(A):
     return w, V
 
 
-def lra(X, y, k, gamma, omega):
+LRA_FP64 = 0
+LRA_REF_FP32 = 1
+
+
+def lra(X, y, k, gamma, omega, mode=LRA_FP64):
+    """Rank-k factor; mode LRA_REF_FP32 follows the reference's FP32 storage of W, Q, C,
+    X and U (the plain method's precision path)."""
     X, y, omega = f32(X), f32(y), f32(omega)
     n, d = X.shape
     U = np.empty((n, k))
     w = np.empty(k)
-    realk = lib().orc_lra(_p(X), _p(y), n, d, k, gamma, _p(omega), _p(U), _p(w))
+    realk = lib().orc_lra_ex(_p(X), _p(y), n, d, k, gamma, _p(omega), mode, _p(U), _p(w))
     return U, w, realk
 
 

@@ -56,3 +56,20 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout.strip() == ""
+
+
+def test_rank_limit_checked_before_reading():
+    """-k above the solver's limit is refused before the training file is read (the factor
+    would otherwise run for minutes before the interior-point solver rejects it)."""
+    r = _run(["-t", "2", "-k", "2048", "absent.libsvm"])
+    assert r.returncode != 0
+    assert "-k 2048 outside 1..1024" in r.stderr
+
+
+def test_three_class_labels_reported(tmp_path):
+    """Labels outside the first two seen are reported (the reference prints an error per
+    such label and continues, tensorsvm-train.cu:890-895)."""
+    p = tmp_path / "three.libsvm"
+    p.write_text("3 1:0.5\n7 1:0.1\n9 1:0.2\n3 1:0.4\n")
+    r = _run(["-t", "2", "-k", "2", str(p)])
+    assert "Error: 1 labels outside {3, 7}" in r.stdout

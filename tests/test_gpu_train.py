@@ -20,11 +20,46 @@ def _split(n, d, seed):
     return X[:ntr], y[:ntr], X[ntr:], y[ntr:]
 
 
-def _oracle_train(X, y, k, gamma, C, om):
-    U, w, realk = O.lra(X, y, k, gamma, om)
+def _oracle_train(X, y, k, gamma, C, om, mode=O.LRA_REF_FP32):
+    """The reference's plain method: the factor with its FP32 storage of W, Q, C, X and U
+    (LRA 1941-2249), the FP64 IPM (mpc 1507-1840) and the bias over the first <= 50 free
+    SVs (writemodel 353-372)."""
+    U, w, realk = O.lra(X, y, k, gamma, om, mode)
     al, xi, s, info = O.mpc(U, y.astype(np.float64), C)
-    b, cnt, sd = O.bias_rbf(al, y, U, C, s, xi)
+    b, cnt, sd = O.bias_rbf(al, y, U, C)
     return al, b, info, U
+
+
+def _free_set(al, C):
+    """The reference's bias samples: the first <= 50 indices with 1e-6 < alpha < C - 1e-6."""
+    return np.flatnonzero((al > 1e-6) & (al < C - 1e-6))[:50]
+
+
+def check_train_parity(name, al, info, alo, bo, io, C, fg, fo, yt):
+    """End-to-end parity with the oracle's plain method: alpha within 1e-4 C, the same
+    support-vector set apart from multipliers within 1e-5 of 0 or C, the primal objective to
+    1e-6, the bias within 1e-4 on the same free-SV samples (else within the shift that
+    swapping samples of spread b_std can make), held-out accuracy within 0.1 point (or one
+    held-out row when fewer than 1000)."""
+    from conftest import parity_log
+    da = np.abs(al - alo).max() / C
+    sg, so = _free_set(al, C), _free_set(alo, C)
+    same = len(sg) == len(so) and np.array_equal(sg, so)
+    ndiff = 0 if same else len(np.setxor1d(sg, so))
+    acc_g, acc_o = np.mean(np.sign(fg) == yt), np.mean(np.sign(fo) == yt)
+    parity_log(name, dalpha_over_C=da, dpobj_rel=abs(info.ipm.pobj - io.pobj) / (1 + abs(io.pobj)),
+               db=abs(info.b - bo), same_bias_samples=bool(same), bias_samples_differing=ndiff,
+               b_std=info.b_std, acc_gpu=acc_g, acc_oracle=acc_o, iters_gpu=info.ipm.iters,
+               iters_oracle=io.iters)
+    assert da <= 1e-4, da
+    assert abs(info.ipm.pobj - io.pobj) <= 1e-6 * (1 + abs(io.pobj))
+    amb = (alo < 1e-5) | (alo > C - 1e-5) | (np.abs(alo - 1e-6) < 1e-5)
+    assert np.all(((al > 1e-6) == (alo > 1e-6)) | amb)
+    if same:
+        assert abs(info.b - bo) <= 1e-4, (info.b, bo)
+    else:
+        assert abs(info.b - bo) <= 1e-4 + 4 * info.b_std * ndiff / max(len(so), 1), (info.b, bo, ndiff)
+    assert abs(acc_g - acc_o) <= max(0.001, 1.0 / len(yt)) + 1e-12, (acc_g, acc_o)
 
 
 # (25000, 32): the plain method's small configuration, 20,000 training rows, d = 18,
@@ -38,17 +73,10 @@ def test_train_fp32_matches_oracle(ctx, n, k):
     al, _, info = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=T.PREC_FP32, omega=om)
     alo, bo, io, U = _oracle_train(X, y, k, gamma, C, om.reshape(-1, k))
     assert info.ipm.reason == io.reason == 0
-    # the two factors agree to ~1e-6 (Cholesky-QR vs Householder, FP32 vs FP64 products),
-    # which moves a near-degenerate QP solution by up to a few 1e-4 C
-    assert np.abs(al - alo).max() <= 1e-3 * C
-    assert abs(info.ipm.pobj - io.pobj) <= 1e-6 * (1 + abs(io.pobj))
-    amb = (alo < 1e-5) | (alo > C - 1e-5) | (np.abs(alo - 1e-6) < 1e-5)
-    assert np.all(((al > 1e-6) == (alo > 1e-6)) | amb)
-    assert abs(info.b - bo) <= 1e-3 + 3 * info.b_std
+    assert info.b_slack_rule == 0  # the reference's free set under the reference stop
     fg = ctx.decision_function(X, y, al, gamma, info.b, Xt)
     fo = O.decision(alo, y, X, gamma, bo, Xt)
-    acc_g, acc_o = np.mean(np.sign(fg) == yt), np.mean(np.sign(fo) == yt)
-    assert abs(acc_g - acc_o) <= 0.001 + 1.0 / len(yt)
+    check_train_parity(f"train_fp32[{n}-{k}]", al, info, alo, bo, io, C, fg, fo, yt)
     # the GPU decision function (FP32 products) against FP64 on the same multipliers, to
     # the FP32 accumulation error of sum_j |alpha_j| K(x_t, x_j)
     scale = O.decision(np.abs(al), np.ones_like(y), X, gamma, 0.0, Xt)

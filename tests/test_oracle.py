@@ -229,3 +229,106 @@ def test_decision_and_linear_bias_against_sklearn():
     b, w = O.bias_linear(al, y.astype(np.float32), X, C)
     assert np.allclose(w, lin.coef_[0], rtol=1e-9, atol=1e-9)
     assert abs(b - lin.intercept_[0]) <= 1e-5 * (1 + abs(lin.intercept_[0])), (b, lin.intercept_)
+
+
+def _scaled_kernel(Xa, Xb, gamma, s):
+    """exp(-gamma sum_f e^{2 s_f} (x_af - x_bf)^2) in FP64."""
+    sc = np.exp(s)
+    return np.exp(-gamma * cdist(Xa.astype(np.float64) * sc, Xb.astype(np.float64) * sc, "sqeuclidean"))
+
+
+def test_grad_values_closed_form():
+    """The values behind orc_grad, restated with numpy: D = sum alpha - 1/2 alpha^T P K_Y P
+    alpha with P = Q Q^T, Q an orthonormal basis of K_Y Omega (numpy QR), and L the mean
+    smoothed hinge of the exact-kernel decision values.  A projector onto range(Omega)
+    instead of range(K_Y Omega), or an unlabelled kernel, fails it."""
+    from datagen import scaled_mixture
+    X, y, g0 = scaled_mixture(200, 4, 5)
+    X, Xt, y, yt = X[:150], X[150:], y[:150], y[150:]
+    n, d, k = X.shape[0], X.shape[1], 10
+    rng = np.random.default_rng(8)
+    om = O.gaussrand(n * k)
+    alpha = rng.uniform(0.0, 2.0, n)
+    alpha[rng.random(n) < 0.3] = 0.0
+    lg, s, b = np.log(g0), rng.normal(0.0, 0.3, d), -0.2
+    D, L, _, _, st = O.grad(X, y, k, om, alpha, b, lg, s, Xt, yt, False)
+    assert st == 0
+    yd = y.astype(np.float64)
+    K = _scaled_kernel(X, X, g0, s)
+    Ky = yd[:, None] * K * yd[None, :]
+    Q, _ = np.linalg.qr(Ky @ om.reshape(n, k).astype(np.float64))
+    P = Q @ Q.T
+    D_np = alpha.sum() - 0.5 * alpha @ (P @ Ky @ P) @ alpha
+    f = _scaled_kernel(Xt, X, g0, s) @ (alpha * yd) + b
+    z = yt.astype(np.float64) * f
+    h = np.where(z <= 0, 0.5 - z, np.where(z < 1, 0.5 * (1 - z) ** 2, 0.0))
+    assert abs(D - D_np) <= 1e-10 * abs(D_np)
+    assert abs(L - h.mean()) <= 1e-12 * max(1.0, abs(L))
+    # the plausible mistakes give other values
+    Qo, _ = np.linalg.qr(om.reshape(n, k).astype(np.float64))
+    Po = Qo @ Qo.T
+    assert abs(D - (alpha.sum() - 0.5 * alpha @ (Po @ Ky @ Po) @ alpha)) > 1e-3 * abs(D_np)
+    assert abs(D - (alpha.sum() - 0.5 * alpha @ (P @ K @ P) @ alpha)) > 1e-3 * abs(D_np)
+
+
+def test_grad_dual_is_trained_objective():
+    """At the solver's optimum, with no eigenvalue clipped, D(theta_0) is minus the primal
+    objective the IPM reports: U U^T = P K_Y P and pobj = 1/2 a^T U U^T a - sum a."""
+    from datagen import scaled_mixture
+    X, y, g0 = scaled_mixture(400, 5, 9)
+    n, d, k = X.shape[0], X.shape[1], 16
+    om = O.gaussrand(n * k)
+    U, w, realk = O.lra(X, y, k, g0, om.reshape(n, k))
+    assert realk == k
+    al, xi, s, info = O.mpc(U, y.astype(np.float64), 1.0, stop="gap", tol=1e-9)
+    assert info.reason == 0
+    D, L, _, _, st = O.grad(X, y, k, om, al, 0.0, np.log(g0), np.zeros(d), want_grad=False)
+    assert st == 0
+    pobj = 0.5 * al @ (U @ (U.T @ al)) - al.sum()
+    assert abs(D + pobj) <= 1e-9 * abs(pobj)
+    assert abs(pobj - info.pobj) <= 1e-7 * abs(pobj)
+
+
+def test_lra_reference_fp32_storage_mode():
+    """LRA_REF_FP32 (the reference's float W, Q, C, X and FP32 U = W X): the factor is
+    FP32-representable, agrees with the FP64 mode to FP32 rounding, and is exact at k = n to
+    FP32 accuracy."""
+    from datagen import gaussian_mixture
+    X, y = gaussian_mixture(400, 6, 12)
+    n, k = 400, 24
+    om = O.gaussrand(n * k).reshape(n, k)
+    U0, w0, r0 = O.lra(X, y, k, 0.1, om, O.LRA_FP64)
+    U1, w1, r1 = O.lra(X, y, k, 0.1, om, O.LRA_REF_FP32)
+    assert np.array_equal(U1, U1.astype(np.float32).astype(np.float64))
+    G0, G1 = U0 @ U0.T, U1 @ U1.T
+    rel = np.abs(G1 - G0).max() / np.abs(G0).max()
+    assert 1e-9 < rel < 1e-5, rel  # rounded, but only by FP32 storage
+    assert r0 == r1 and np.allclose(w1, w0, rtol=1e-5, atol=1e-6 * w0[-1])
+    # k = n: range(K_Y Omega) is everything, U U^T = K_Y to FP32 accuracy
+    m = 60
+    Xs, ys = X[:m], y[:m]
+    Un, _, _ = O.lra(Xs, ys, m, 0.3, O.gaussrand(m * m).reshape(m, m), O.LRA_REF_FP32)
+    yd = ys.astype(np.float64)
+    Ky = yd[:, None] * np.exp(-0.3 * cdist(Xs.astype(np.float64), Xs.astype(np.float64), "sqeuclidean")) * yd[None, :]
+    assert np.abs(Un @ Un.T - Ky).max() <= 1e-5 * np.abs(Ky).max()
+
+
+def test_qp_sensitivity_to_factor_rounding():
+    """How far the dual solution moves when the factor is perturbed at FP32 rounding level:
+    a relative 1e-7 perturbation of U moves alpha by ~20-120x as much (relative to C) on the
+    c0-shaped problems -- the QP is near-degenerate -- but stays within the 1e-4 C bar.
+    The bias, a mean over the first <= 50 free SVs, moves by up to its sampling spread when
+    a multiplier crosses the 1e-6 free-SV threshold."""
+    from datagen import gaussian_mixture
+    X, y = gaussian_mixture(3750, 18, 4000)
+    X, y = X[:3000], y[:3000]
+    n, k, C = 3000, 32, 32.0
+    U, _, _ = O.lra(X, y, k, 0.05, O.gaussrand(n * k).reshape(n, k))
+    a0, _, _, i0 = O.mpc(U, y.astype(np.float64), C)
+    rng = np.random.default_rng(1)
+    Up = U * (1 + 1e-7 * rng.standard_normal(U.shape))
+    a1, _, _, i1 = O.mpc(Up, y.astype(np.float64), C)
+    da = np.abs(a1 - a0).max() / C
+    assert i0.reason == i1.reason == 0
+    assert 10 * 1e-7 <= da <= 1e-4, da
+    assert abs(i1.pobj - i0.pobj) <= 1e-7 * abs(i0.pobj)
