@@ -196,6 +196,41 @@ void orc_kernel_product(const float *X1, const float *y1, int64_t m, const float
   }
 }
 
+/* The same product in the reference's FP32 arithmetic: float norms (vecnorm 2569-2581),
+ * float X_i . X_j (the XIJ sgemm, 2506-2518), the kernel value
+ * y_i y_j expf(-gamma (XI + XJ - 2 XIJ)) with the distance in float (rbf_kergen 2598-2612)
+ * and float accumulation of W_i += buf A_j over all n columns in index order (the K A
+ * sgemm with beta = 1, 2537-2555, and its chunk loop 2017-2058).  cuBLAS's blocking order
+ * and __expf's error are not reproduced (libm's expf instead). */
+void orc_kernel_product_f32(const float *X1, const float *y1, int64_t m, const float *X2,
+                            const float *y2, int64_t n, int64_t d, const float *A, int64_t r,
+                            double gamma, float *W) {
+  float *xn2 = (float *)malloc(sizeof(float) * (n > 0 ? n : 1));
+  for (int64_t j = 0; j < n; j++) {
+    float sum = 0.0f;
+    for (int64_t t = 0; t < d; t++) sum += X2[j * d + t] * X2[j * d + t];
+    xn2[j] = sum;
+  }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < m; i++) {
+    float *w = W + i * r;
+    for (int64_t c = 0; c < r; c++) w[c] = 0.0f;
+    const float *xi = X1 + i * d;
+    float xn1 = 0.0f;
+    for (int64_t t = 0; t < d; t++) xn1 += xi[t] * xi[t];
+    for (int64_t j = 0; j < n; j++) {
+      const float *xj = X2 + j * d;
+      float dot = 0.0f;
+      for (int64_t t = 0; t < d; t++) dot += xi[t] * xj[t];
+      const float dist = xn1 + xn2[j] - 2.0f * dot;
+      const float kv = y1[i] * y2[j] * expf((float)(-gamma * (double)dist));
+      const float *aj = A + j * r;
+      for (int64_t c = 0; c < r; c++) w[c] += kv * aj[c];
+    }
+  }
+  free(xn2);
+}
+
 /* ------------------------------------------------------------------------- */
 /* Orthonormalisation (replaces ortho/CAQR/QR 617-856): thin Householder Q.    */
 /* ------------------------------------------------------------------------- */
@@ -328,12 +363,25 @@ int64_t orc_lra_ex(const float *X, const float *y, int64_t n, int64_t d, int64_t
   double *W = (double *)malloc(sizeof(double) * n * k);
   double *Q = (double *)malloc(sizeof(double) * n * k);
   for (int64_t i = 0; i < n * k; i++) Om[i] = (double)omega[i];
-  orc_kernel_product(X, y, n, X, y, n, d, Om, k, gamma, W); /* W = K_Y Omega (2067) */
-  if (f32) round_f32(W, n * k);                            /* float W (Z, 1986-1990) */
-  orc_householder_q(W, n, k);                              /* ortho (2075) */
-  if (f32) round_f32(W, n * k);                            /* ortho(float *W, ...) */
-  orc_kernel_product(X, y, n, X, y, n, d, W, k, gamma, Q); /* Q = K_Y W (2085) */
-  if (f32) round_f32(Q, n * k);                            /* float Q */
+  float *Wf = f32 ? (float *)malloc(sizeof(float) * n * k) : NULL;
+  float *Af = f32 ? (float *)malloc(sizeof(float) * n * k) : NULL;
+  if (f32) { /* W = K_Y Omega in FP32 arithmetic (2067) */
+    orc_kernel_product_f32(X, y, n, X, y, n, d, omega, k, gamma, Wf);
+    for (int64_t i = 0; i < n * k; i++) W[i] = (double)Wf[i];
+  } else {
+    orc_kernel_product(X, y, n, X, y, n, d, Om, k, gamma, W); /* W = K_Y Omega (2067) */
+  }
+  orc_householder_q(W, n, k); /* ortho (2075) */
+  if (f32) {                  /* float W out of ortho(float *W, ...); Q = K_Y W (2085) */
+    for (int64_t i = 0; i < n * k; i++) Af[i] = (float)W[i];
+    round_f32(W, n * k);
+    orc_kernel_product_f32(X, y, n, X, y, n, d, Af, k, gamma, Wf);
+    for (int64_t i = 0; i < n * k; i++) Q[i] = (double)Wf[i];
+    free(Wf);
+    free(Af);
+  } else {
+    orc_kernel_product(X, y, n, X, y, n, d, W, k, gamma, Q); /* Q = K_Y W (2085) */
+  }
   double *C = (double *)calloc((size_t)(k * k), sizeof(double));
   for (int64_t i = 0; i < n; i++) /* C = W^T Q, FP64 accumulation (2122-2149) */
     for (int64_t a = 0; a < k; a++) {

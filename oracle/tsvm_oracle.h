@@ -40,6 +40,13 @@ void orc_kernel_product(const float *X1, const float *y1, int64_t m,
                         const float *X2, const float *y2, int64_t n, int64_t d,
                         const double *A, int64_t r, double gamma, double *W);
 
+/* The same product in the reference's FP32 arithmetic (float norms, float XIJ, float
+ * distance XI + XJ - 2 XIJ, expf, float accumulation over all n columns in index order).
+ * A n x r and W m x r are float. */
+void orc_kernel_product_f32(const float *X1, const float *y1, int64_t m, const float *X2,
+                            const float *y2, int64_t n, int64_t d, const float *A, int64_t r,
+                            double gamma, float *W);
+
 /* Thin Q of a Householder QR (replaces TSQR ortho/CAQR/QR 617-856): W (n x k) <- Q. */
 void orc_householder_q(double *W, int64_t n, int64_t k);
 /* Symmetric eigen-decomposition by cyclic Jacobi (replaces LAPACKE_dsyevd 2191):
@@ -55,13 +62,13 @@ int64_t orc_lra(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
                 double gamma, const float *omega, double *U, double *eig_out);
 /* The same with a choice of storage precision:
  *   ORC_LRA_FP64      every intermediate in FP64 (orc_lra);
- *   ORC_LRA_REF_FP32  the reference's plain-method storage: W = K_Y Omega, the
- *                     orthonormal basis and Q = K_Y W rounded to FP32 (float Z/W/Q),
- *                     C = W^T Q accumulated in FP64 and stored as float (2143-2157),
- *                     X = V sqrt(w) as float (2187, 2208) and U = W X as an FP32
- *                     product (2213-2241).  The kernel products themselves are formed
- *                     in FP64 and then rounded (cuBLAS's FP32 accumulation order is not
- *                     reproducible off the reference's device). */
+ *   ORC_LRA_REF_FP32  the reference's plain-method arithmetic: both kernel products in
+ *                     FP32 (orc_kernel_product_f32), the orthonormal basis rounded to
+ *                     FP32 (float W), C = W^T Q accumulated in FP64 and stored as float
+ *                     (2143-2157), X = V sqrt(w) as float (2187, 2208) and U = W X as an
+ *                     FP32 product (2213-2241).  The orthonormaliser is the FP64
+ *                     Householder QR (the reference's FP32 TSQR rounding is not
+ *                     restated). */
 enum { ORC_LRA_FP64 = 0, ORC_LRA_REF_FP32 = 1 };
 int64_t orc_lra_ex(const float *X, const float *y, int64_t n, int64_t d, int64_t k,
                    double gamma, const float *omega, int mode, double *U, double *eig_out);

@@ -1376,7 +1376,9 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_HIP(c, hipGetLastError());
   }
   TSVM_HIP(c, hipStreamSynchronize(st));
-  inf.iters = iter;
+  // a failed factorisation is seen at the top of the next iteration, whose update was
+  // skipped: the iterate returned is the one of the failing iteration
+  inf.iters = reason == TSVM_CHOL_FAIL && iter > 0 ? iter - 1 : iter;
   inf.reason = reason;
   inf.t_total = now() - t_start;
   inf.t_syrk = t_syrk;

@@ -36,6 +36,7 @@ def lib():
         L.orc_label_map.argtypes = [P, I64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.orc_label_map.restype = I64
         L.orc_kernel_product.argtypes = [P, P, I64, P, P, I64, I64, P, I64, D, P]
+        L.orc_kernel_product_f32.argtypes = [P, P, I64, P, P, I64, I64, P, I64, D, P]
         L.orc_householder_q.argtypes = [P, I64, I64]
         L.orc_sym_eig.argtypes = [P, I64, P, P]
         L.orc_lra.argtypes = [P, P, I64, I64, I64, D, P, P, P]
@@ -115,6 +116,17 @@ def kernel_product(X1, y1, X2, y2, A, gamma):
     r = A.shape[1]
     W = np.empty((m, r), dtype=np.float64)
     lib().orc_kernel_product(_p(X1), _p(y1), m, _p(X2), _p(y2), n, d, _p(A), r, gamma, _p(W))
+    return W
+
+
+def kernel_product_f32(X1, y1, X2, y2, A, gamma):
+    """The reference's FP32 arithmetic of the same product (float result)."""
+    X1, y1, X2, y2, A = f32(X1), f32(y1), f32(X2), f32(y2), f32(A)
+    m, d = X1.shape
+    n = X2.shape[0]
+    r = A.shape[1]
+    W = np.empty((m, r), dtype=np.float32)
+    lib().orc_kernel_product_f32(_p(X1), _p(y1), m, _p(X2), _p(y2), n, d, _p(A), r, gamma, _p(W))
     return W
 
 

@@ -8,7 +8,7 @@ import pytest
 import oracle_lib as O
 from conftest import parity_log
 from datagen import covtype_like, gaussian_mixture, scaled_mixture
-from test_gpu_train import _oracle_train, check_train_parity
+from test_gpu_train import check_train_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -25,12 +25,9 @@ def test_c1_fp32_train_matches_oracle(ctx):
     k, gamma, C = 400, 1.0, 32.0
     om = O.gaussrand(X.shape[0] * k)
     al, _, info = ctx.train(X, y, k=k, C=C, gamma=gamma, precision=T.PREC_FP32, omega=om)
-    alo, bo, io, U = _oracle_train(X, y, k, gamma, C, om.reshape(-1, k))
-    assert info.ipm.reason == io.reason == 0
+    assert info.ipm.reason == 0
     assert info.lra.realk == k
-    fg = ctx.decision_function(X, y, al, gamma, info.b, Xt)
-    fo = O.decision(alo, y, X, gamma, bo, Xt)
-    check_train_parity("c1_fp32", al, info, alo, bo, io, C, fg, fo, yt)
+    check_train_parity("c1_fp32", ctx, X, y, Xt, yt, k, gamma, C, om, al, info)
 
 
 def test_c1_bf16_train_close_to_fp32(ctx):
