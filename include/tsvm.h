@@ -88,6 +88,24 @@ tsvm_status tsvm_kernel_product_dev(tsvm_ctx *ctx, const float *X1, const float 
                                     int64_t d, const float *A, int64_t r, double gamma,
                                     int precision, float *W);
 
+/* Allocation-free form of tsvm_kernel_product_dev.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops such as scoring test rows against a model's support vectors, harnesses that
+ * forbid hipMalloc).  Same arguments, arithmetic and results as tsvm_kernel_product_dev
+ * (the same kernels); the operand copies it prepares live in a fixed device array of
+ * TSVM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, instead of the
+ * context's workspace.  TSVM_ERR_ARG (before any work is queued) when the operands need
+ * more: about 2(m + n)(3d + 4) + 2 n r bytes in the BF16/fp16 forms and
+ * 4(m + n)(d + 2) + 4 n r (+ 4 m r per 8192-column segment when n > 8192) in FP32.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * contexts or streams) race and must be serialised by the caller.  Enqueued on the
+ * context's stream, not synchronised. */
+#define TSVM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+tsvm_status tsvm_kernel_product_noalloc_dev(tsvm_ctx *ctx, const float *X1, const float *y1,
+                                            int64_t m, const float *X2, const float *y2,
+                                            int64_t n, int64_t d, const float *A, int64_t r,
+                                            double gamma, int precision, float *W);
+
 /* Timings and diagnostics of the low-rank factor construction. */
 typedef struct {
   double t_product1;   /* W = K_Y Omega (s, HIP events on the context's stream) */

@@ -62,6 +62,7 @@ def load():
     kp = [_P, _P, _P, _I64, _P, _P, _I64, _I64, _P, _I64, _D, _INT, _P]
     L.tsvm_kernel_product.argtypes = kp
     L.tsvm_kernel_product_dev.argtypes = kp
+    L.tsvm_kernel_product_noalloc_dev.argtypes = kp
     _bind_optional(L)
     _LIB = L
     return L
@@ -460,6 +461,13 @@ class Context:
                                                     budget, _ptr(M), ctypes.byref(info)),
                     "tsvm_weighted_syrk_dev")
         return info
+
+    def kernel_product_noalloc_dev(self, X1, y1, m, X2, y2, n, d, A, r, gamma, precision, W):
+        """tsvm_kernel_product_noalloc_dev: the same product with no device allocation."""
+        self._check(self.lib.tsvm_kernel_product_noalloc_dev(self.h, _ptr(X1), _ptr(y1), m,
+                                                             _ptr(X2), _ptr(y2), n, d, _ptr(A),
+                                                             r, gamma, precision, _ptr(W)),
+                    "tsvm_kernel_product_noalloc_dev")
 
     def kernel_product_dev(self, X1, y1, m, X2, y2, n, d, A, r, gamma, precision, W):
         self._check(self.lib.tsvm_kernel_product_dev(self.h, _ptr(X1), _ptr(y1), m, _ptr(X2),

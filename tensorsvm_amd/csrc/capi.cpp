@@ -73,6 +73,11 @@ tsvm_status tsvm_create(int device, tsvm_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) x->c.cu_count = prop.multiProcessorCount;
   tsvm::warm_sym_eig(&x->c);
+  if (tsvm::noalloc_scratch_setup(&x->c) != TSVM_OK) {
+    hipStreamDestroy(x->c.stream);
+    delete x;
+    return TSVM_ERR_HIP;
+  }
   *out = x;
   return TSVM_OK;
 }
@@ -109,6 +114,16 @@ tsvm_status tsvm_kernel_product_dev(tsvm_ctx* ctx, const float* X1, const float*
   hipSetDevice(ctx->c.device);
   return tsvm::kernel_product_dev(&ctx->c, X1, y1, m, X2, y2, n, d, A, r, r, gamma, precision,
                                   W, r);
+}
+
+tsvm_status tsvm_kernel_product_noalloc_dev(tsvm_ctx* ctx, const float* X1, const float* y1,
+                                            int64_t m, const float* X2, const float* y2,
+                                            int64_t n, int64_t d, const float* A, int64_t r,
+                                            double gamma, int precision, float* W) {
+  if (!ctx) return TSVM_ERR_ARG;
+  hipSetDevice(ctx->c.device);
+  return tsvm::kernel_product_noalloc_dev(&ctx->c, X1, y1, m, X2, y2, n, d, A, r, gamma,
+                                          precision, W);
 }
 
 tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx* ctx, const float* Z, int64_t n, int64_t k,

@@ -28,6 +28,7 @@ struct Ctx {
   // phase timers of the last tsvm_lra / tsvm_ipm call (seconds)
   double t_phase[16] = {0};
   int progress = 0;  // progress lines on stderr during a training call
+  void* scratch = nullptr;  // this device's copy of the allocation-free product's scratch
 };
 
 // Record a failure on the context and return its status.
@@ -66,6 +67,16 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
                                const float* X2, const float* y2, int64_t n, int64_t d,
                                const float* A, int64_t lda, int64_t r, double gamma,
                                int precision, float* W, int64_t ldw);
+
+// The same product without allocating: every operand buffer is carved from the device's
+// fixed scratch array (TSVM_NOALLOC_SCRATCH_BYTES); lda = r, ldw = r.
+tsvm_status kernel_product_noalloc_dev(Ctx* c, const float* X1, const float* y1, int64_t m,
+                                       const float* X2, const float* y2, int64_t n, int64_t d,
+                                       const float* A, int64_t r, double gamma, int precision,
+                                       float* W);
+// Looks up (once per device, with the device current) the address of that device's copy
+// of the scratch array and stores it in c->scratch.
+tsvm_status noalloc_scratch_setup(Ctx* c);
 
 // ---- dense tall-skinny GEMMs (gemm.hip) -----------------------------------------
 // C (ka x kb, FP64, row-major) = A^T diag(w) B over n rows; A n x ka (lda), B n x kb

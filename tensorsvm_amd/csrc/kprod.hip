@@ -30,6 +30,7 @@
 //   FP32: Pop  [m_pad][KF] f32, Qop [n_pad][KF] f32, KF = roundup(d+2, 2)
 //         Af   [n_pad][r_pad] f32 = y2_j A[j,c]
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "common.h"
@@ -164,6 +165,10 @@ __global__ void amax_kernel(const float* __restrict__ A, int64_t lda, int64_t n,
     atomicMax(out, __float_as_uint(mx));
   }
 }
+
+// clears the amax slot (a kernel rather than a memset node, so that the call can be
+// captured into a graph and replayed)
+__global__ void zero_u32_kernel(unsigned int* __restrict__ p) { *p = 0u; }
 
 // power-of-two scale sc with sc * amax in [2^13, 2^14) (1 for an all-zero A): slot 0
 // holds sc, slot 1 holds 1/sc (both exact)
@@ -644,10 +649,32 @@ __global__ void seg_reduce_kernel(const float* __restrict__ part, int64_t nseg, 
   }
 }
 
-tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t m,
-                               const float* X2, const float* y2, int64_t n, int64_t d,
-                               const float* A, int64_t lda, int64_t r, double gamma,
-                               int precision, float* W, int64_t ldw) {
+// Where the operand buffers of one product come from: the context's workspace (grown on
+// demand), or a fixed device region carved from its start (the allocation-free entry
+// point).  Every buffer of a call is taken before its first launch.
+struct KpBuffers {
+  unsigned char* base = nullptr;  // fixed region, or nullptr for the workspace
+  size_t cap = 0, used = 0;
+  tsvm_status get(Ctx* c, const char* name, size_t bytes, void** out) {
+    if (!base) {
+      TSVM_HIP(c, c->ws.get(name, bytes, out));
+      return TSVM_OK;
+    }
+    const size_t off = (used + 255) / 256 * 256;
+    if (off + bytes > cap)
+      return fail(c, TSVM_ERR_ARG,
+                  "kernel_product_noalloc: operands need more than the %zu MiB of scratch "
+                  "(%zu bytes requested so far)", cap >> 20, off + bytes);
+    *out = base + off;
+    used = off + bytes;
+    return TSVM_OK;
+  }
+};
+
+static tsvm_status kernel_product_impl(Ctx* c, KpBuffers& bufs, const float* X1, const float* y1,
+                                       int64_t m, const float* X2, const float* y2, int64_t n,
+                                       int64_t d, const float* A, int64_t lda, int64_t r,
+                                       double gamma, int precision, float* W, int64_t ldw) {
   if (m <= 0 || r <= 0) return TSVM_OK;
   if (n < 0 || d <= 0 || lda < r || ldw < r || !(gamma >= 0.0))
     return fail(c, TSVM_ERR_ARG, "kernel_product: bad shape m=%ld n=%ld d=%ld r=%ld", (long)m,
@@ -674,10 +701,11 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
     const int64_t BN = NT * 32, m_pad = round_up(
                       m, em == EM_F16 && NT == 4 && KD <= KBF16_WIDE_MAX_KD ? 512 : 256),
                   r_pad = round_up(r, BN);
-    void *P, *Q, *AT;
-    TSVM_HIP(c, c->ws.get("kp_P", m_pad * KD * 2, &P));
-    TSVM_HIP(c, c->ws.get("kp_Q", n_pad * KD * 2, &Q));
-    TSVM_HIP(c, c->ws.get("kp_A", n_pad * r_pad * aw * 2 / 32, &AT));
+    void *P, *Q, *AT, *scv = nullptr;
+    TSVM_CHECK(bufs.get(c, "kp_P", m_pad * KD * 2, &P));
+    TSVM_CHECK(bufs.get(c, "kp_Q", n_pad * KD * 2, &Q));
+    TSVM_CHECK(bufs.get(c, "kp_A", n_pad * r_pad * aw * 2 / 32, &AT));
+    if (em != EM_BF16) TSVM_CHECK(bufs.get(c, "kp_sc", 16, &scv));
     hipLaunchKernelGGL(prep_rows_bf16, dim3((unsigned)ceil_div(m_pad, 256)), dim3(256), 0, s, X1, m,
                        m_pad, (int)d, KD, gl2, (__bf16*)P);
     hipLaunchKernelGGL(prep_cols_bf16, dim3((unsigned)ceil_div(n_pad, 256)), dim3(256), 0, s, X2, n,
@@ -685,11 +713,9 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
     float* sc = nullptr;
     if (em != EM_BF16) {
       // one power-of-two scale brings max |y_j A[j,c]| to [2^13, 2^14) in fp16
-      void* scv;
-      TSVM_HIP(c, c->ws.get("kp_sc", 16, &scv));
       sc = (float*)scv;
       unsigned int* amax = (unsigned int*)(sc + 2);
-      TSVM_HIP(c, hipMemsetAsync(amax, 0, 4, s));
+      hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, amax);
       const int64_t tot = (n > 0 ? n : 1) * r;
       const unsigned nb = (unsigned)std::min<int64_t>(ceil_div(tot, 256), 4096);
       hipLaunchKernelGGL(amax_kernel, dim3(nb), dim3(256), 0, s, A, lda, n, r, amax);
@@ -743,10 +769,22 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
       return fail(c, TSVM_ERR_ARG, "kernel_product(fp32): d=%ld exceeds 128 features", (long)d);
     const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : (r <= 128 ? 4 : 8));
     const int64_t BN = NT * 32, m_pad = round_up(m, 128), r_pad = round_up(r, BN);
-    void *P, *Q, *Af;
-    TSVM_HIP(c, c->ws.get("kp_P", m_pad * KF * 4, &P));
-    TSVM_HIP(c, c->ws.get("kp_Q", n_pad * KF * 4, &Q));
-    TSVM_HIP(c, c->ws.get("kp_A", n_pad * r_pad * 4, &Af));
+    // FP32 sums over segments of 8192 columns (the reference's tile, whose products it
+    // accumulates with beta = 1, 2537-2555), added in FP64 in segment order, bounded by
+    // 8 GiB of partial sums: at most m r nseg floats.  Where that bound binds (m r > 2^28:
+    // 1.6M x 512 takes two segments of 800k columns) each FP32 sum spans more columns; the
+    // reference accumulates all n columns of a product in FP32.
+    const int64_t seg_len = 256;  // chunks of 32 columns
+    int64_t nseg = ceil_div(n_chunks, seg_len);
+    const int64_t max_seg = (int64_t)(8.0 * (1ll << 30) / ((double)m * r * 4.0));
+    if (nseg > max_seg) nseg = max_seg > 1 ? max_seg : 1;
+    const int64_t seg_chunks = ceil_div(n_chunks, nseg);
+    nseg = ceil_div(n_chunks, seg_chunks);
+    void *P, *Q, *Af, *pp = nullptr;
+    TSVM_CHECK(bufs.get(c, "kp_P", m_pad * KF * 4, &P));
+    TSVM_CHECK(bufs.get(c, "kp_Q", n_pad * KF * 4, &Q));
+    TSVM_CHECK(bufs.get(c, "kp_A", n_pad * r_pad * 4, &Af));
+    if (nseg > 1) TSVM_CHECK(bufs.get(c, "kp_seg", (size_t)(nseg * m * r * 4), &pp));
     hipLaunchKernelGGL(prep_rows_f32, dim3((unsigned)ceil_div(m_pad, 256)), dim3(256), 0, s, X1, m,
                        m_pad, (int)d, KF, gl2, (float*)P);
     hipLaunchKernelGGL(prep_cols_f32, dim3((unsigned)ceil_div(n_pad, 256)), dim3(256), 0, s, X2, n,
@@ -754,20 +792,9 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
     hipLaunchKernelGGL(prep_A_f32, dim3((unsigned)ceil_div(n_pad * r_pad, 256)), dim3(256), 0, s,
                        A, lda, y2, n, n_pad, r, r_pad, (float*)Af);
     TSVM_HIP(c, hipGetLastError());
-    // FP32 sums over segments of 8192 columns (the reference's tile, whose products it
-    // accumulates with beta = 1, 2537-2555), added in FP64: the rounding of an output
-    // grows like sqrt(8192) instead of sqrt(n).  Bounded by 8 GiB of partial sums.
-    const int64_t seg_len = 256;  // chunks of 32 columns
-    int64_t nseg = ceil_div(n_chunks, seg_len);
-    const int64_t max_seg = (int64_t)(8.0 * (1ll << 30) / ((double)m * r * 4.0));
-    if (nseg > max_seg) nseg = max_seg > 1 ? max_seg : 1;
-    const int64_t seg_chunks = ceil_div(n_chunks, nseg);
-    nseg = ceil_div(n_chunks, seg_chunks);
     float* Wout = W;
     int64_t ldo = ldw, seg_stride = 0;
     if (nseg > 1) {
-      void* pp;
-      TSVM_HIP(c, c->ws.get("kp_seg", (size_t)(nseg * m * r * 4), &pp));
       Wout = (float*)pp;
       ldo = r;
       seg_stride = m * r;
@@ -806,6 +833,53 @@ tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t
   }
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
+}
+
+tsvm_status kernel_product_dev(Ctx* c, const float* X1, const float* y1, int64_t m,
+                               const float* X2, const float* y2, int64_t n, int64_t d,
+                               const float* A, int64_t lda, int64_t r, double gamma,
+                               int precision, float* W, int64_t ldw) {
+  KpBuffers ws;
+  return kernel_product_impl(c, ws, X1, y1, m, X2, y2, n, d, A, lda, r, gamma, precision, W,
+                             ldw);
+}
+
+// ---- allocation-free kernel product ------------------------------------------------
+// The operand buffers of tsvm_kernel_product_noalloc_dev live in this file-scope device
+// array: one copy per GPU, held until the process ends.  Host code never passes its name
+// to a kernel (on the host it is a stand-in, not a device address); the address of each
+// device's copy is looked up once, with that device current, and kept in g_scratch_dev.
+static __device__ __align__(256) unsigned char scratch[TSVM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+tsvm_status noalloc_scratch_setup(Ctx* c) {
+  if (c->device < 0 || c->device >= kMaxDevices)
+    return fail(c, TSVM_ERR_ARG, "noalloc scratch: device %d beyond %d", c->device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[c->device]) {
+    void* ptr = nullptr;
+    TSVM_HIP(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(c, TSVM_ERR_HIP, "noalloc scratch: no device address");
+    g_scratch_dev[c->device] = ptr;
+  }
+  c->scratch = g_scratch_dev[c->device];
+  return TSVM_OK;
+}
+
+tsvm_status kernel_product_noalloc_dev(Ctx* c, const float* X1, const float* y1, int64_t m,
+                                       const float* X2, const float* y2, int64_t n, int64_t d,
+                                       const float* A, int64_t r, double gamma, int precision,
+                                       float* W) {
+  if (!c->scratch) return fail(c, TSVM_ERR_HIP, "kernel_product_noalloc: scratch not set up");
+  KpBuffers fixed;
+  fixed.base = (unsigned char*)c->scratch;
+  fixed.cap = TSVM_NOALLOC_SCRATCH_BYTES;
+  return kernel_product_impl(c, fixed, X1, y1, m, X2, y2, n, d, A, r, r, gamma, precision, W, r);
 }
 
 }  // namespace tsvm
