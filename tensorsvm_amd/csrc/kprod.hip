@@ -216,6 +216,79 @@ __global__ void prep_A_f16(const float* __restrict__ A, int64_t lda, const float
   }
 }
 
+// ---- operands in MFMA fragment order (the wide-tile kernel) ------------------------
+// Qf[chunk][ks][lane][8]: the A operand of the distance MFMA for 32 columns j of X2 and
+// K-step ks: lane l holds Qop[j = 32 chunk + l % 32][16 ks + 8 (l / 32) + 0..7].  A chunk
+// is one contiguous run of 64 KD bytes.
+__device__ inline int64_t qfrag_index(int64_t j, int t) {  // offset within j's chunk
+  const int jr = (int)(j & 31), ks = t >> 4, hh = (t >> 3) & 1, e = t & 7;
+  return ((int64_t)ks * 64 + (jr + 32 * hh)) * 8 + e;
+}
+__global__ void prep_cols_bf16_frag(const float* __restrict__ X, int64_t n, int64_t n_pad, int d,
+                                    int KD, double gl2, __bf16* __restrict__ Qf) {
+  int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n_pad) return;
+  __bf16* q = Qf + (j >> 5) * (int64_t)(32 * KD);
+  auto put = [&](int t, __bf16 v) { q[qfrag_index(j, t)] = v; };
+  const __bf16 z = (__bf16)0.0f;
+  if (j >= n) {
+    for (int t = 0; t < KD; t++) put(t, z);
+    return;
+  }
+  const float* x = X + j * d;
+  double nrm = 0.0;
+  for (int t = 0; t < d; t++) {
+    float v = x[t];
+    nrm += (double)v * v;
+    float u = (float)(2.0 * gl2 * (double)v);
+    __bf16 hi, lo;
+    split_bf16(u, hi, lo);
+    put(t, hi);
+    put(d + t, lo);
+    put(2 * d + t, hi);
+  }
+  float bb = (float)(-gl2 * nrm);
+  __bf16 bh, bl;
+  split_bf16(bb, bh, bl);
+  put(3 * d, bh);
+  put(3 * d + 1, bl);
+  put(3 * d + 2, (__bf16)1.0f);
+  put(3 * d + 3, (__bf16)1.0f);
+  for (int t = 3 * d + 4; t < KD; t++) put(t, z);
+}
+
+// Af[chunk][ct][kap][lane][tt]: the B operand of the product MFMA for K-step kap of a
+// 32-j chunk and column tile ct: lane l holds A'[j][c] for c = 32 ct + l % 32 and
+// j = 32 chunk + 16 kap + 8 (tt / 4) + 4 (l / 32) + tt % 4 -- the order in which the
+// distance MFMA's C layout leaves the kernel values in the A operand.  One chunk's column
+// tiles are consecutive 2 KB runs.  EM_BF16: bf16 of y_j A[j,c]; otherwise fp16 of
+// sc y_j A[j,c] (sc from fp16_scale_kernel).
+template <typename ET>
+__global__ void prep_A_frag(const float* __restrict__ A, int64_t lda, const float* __restrict__ y,
+                            int64_t n, int64_t r, int64_t r_tiles, const float* __restrict__ sc,
+                            ET* __restrict__ Af) {
+  __shared__ float tile[32][33];
+  const int64_t chunk = blockIdx.x;
+  const int64_t ct = blockIdx.y;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const float scale = sc ? sc[0] : 1.0f;
+  for (int jj = ty; jj < 32; jj += 8) {
+    int64_t j = chunk * 32 + jj, c = ct * 32 + tx;
+    float v = 0.0f;
+    if (j < n && c < r) v = y[j] * A[j * lda + c];
+    tile[jj][tx] = v * scale;
+  }
+  __syncthreads();
+  ET* out = Af + (chunk * r_tiles + ct) * 1024;
+  // thread -> (lane, tt) pairs: 64 lanes x 16 values (2 kap x 8 tt) = 1024 per tile
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int kap = e >> 9, lane = (e >> 3) & 63, tt = e & 7;
+    const int hh = lane >> 5, cc = lane & 31;
+    const int jr = 16 * kap + 8 * (tt >> 2) + 4 * hh + (tt & 3);
+    out[e] = (ET)tile[jr][cc];
+  }
+}
+
 __global__ void prep_rows_f32(const float* __restrict__ X, int64_t m, int64_t m_pad, int d, int KF,
                               double gl2, float* __restrict__ P) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -472,6 +545,193 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
 }
 
 // -------------------------------------------------------------------------------------
+// Wide-tile low-precision kernel (right-hand sides wider than 128 columns).  WAVES waves
+// per block, one block per CU; wave w owns RB 32-row blocks and NT 32-column tiles of the
+// output (RB NT 16 accumulator registers).  Used as RB = 1, NT = 8, WAVES = 8: 128
+// accumulators, two waves per SIMD, a distance tile and its exp2 formed once per 256
+// columns (4 distance MFMAs per 16 product MFMAs; the 128-column kernel above needs 8 per
+// 16).  Wider tiles do not fit: at one wave per SIMD every MFMA result is allocated to the
+// 256 accumulation registers, which 256 product accumulators plus a distance tile exceed.
+// The wave pipelines one chunk deep: while the product MFMAs of chunk c run on E_c, it
+// issues the distance MFMAs of chunk c + 1 and turns their result into E_{c+1}.  Operands
+// come in MFMA fragment order (Qf, Af above): a chunk is one contiguous run in global
+// memory and in LDS, staging is a linear copy and every fragment read is a 16-byte read at
+// lane * 16 (no padding, no bank conflicts).  LDS ring of three chunk buffers: iteration c
+// reads A'_c and Q_{c+1} and writes chunk c + 2; one barrier per chunk.  The outputs are
+// bit-identical to the 128-column kernel's (same operands, same accumulation order).
+// -------------------------------------------------------------------------------------
+template <int RB, int NT, int KD, int EM, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __restrict__ P,
+                                                     const __bf16* __restrict__ Qf,
+                                                     const void* __restrict__ Afv,
+                                                     const float* __restrict__ y1,
+                                                     float* __restrict__ W, int64_t ldw,
+                                                     int64_t m, int64_t r, int64_t n_chunks,
+                                                     int64_t r_tiles, const float* __restrict__ sc,
+                                                     int64_t nxb, int64_t ntiles) {
+  using ET = typename std::conditional<EM == EM_BF16, __bf16, _Float16>::type;
+  using ET8 = typename std::conditional<EM == EM_BF16, bf16x8_t, f16x8_t>::type;
+  constexpr int KS = KD / 16;
+  constexpr int QB = 64 * KD;          // bytes of a chunk's Qf run
+  constexpr int AB = 2048 * NT;        // bytes of a chunk's Af run for NT column tiles
+  constexpr int CB = QB + AB;          // bytes per LDS chunk buffer
+  constexpr int THR = 64 * WAVES;
+  constexpr int NV = (CB / 16 + THR - 1) / THR;  // 16-byte staging vectors per thread
+  constexpr int CBP = NV * THR * 16;   // buffer stride: every thread stores NV vectors
+  constexpr int BR = 32 * RB * WAVES;  // rows per block
+  __shared__ __attribute__((aligned(16))) unsigned char lds[3 * CBP];
+  const unsigned char* __restrict__ Af = (const unsigned char*)Afv;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, h = lane >> 5;
+
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t rowbase = (tile % nxb) * BR + wave * (32 * RB);
+    const int64_t ct0 = (tile / nxb) * NT;
+    // B operand of the distance MFMA: this wave's P rows, resident for the whole j loop
+    bf16x8_t pb[RB][KS];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++)
+        pb[rb][ks] = *(const bf16x8_t*)(P + (rowbase + 32 * rb + l32) * KD + ks * 16 + 8 * h);
+    f32x16_t acc[RB][NT];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+      for (int ct = 0; ct < NT; ct++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc[rb][ct][q] = 0.0f;
+
+    // branch-free staging of chunk t: vector idx < QB/16 from Qf, the rest from Af (an
+    // out-of-range index re-reads the chunk's last vector and skips its store)
+    u32x4_t stg[NV];
+    auto gload = [&](int64_t t) {
+      const u32x4_t* gq = (const u32x4_t*)((const unsigned char*)Qf + t * QB);
+      const u32x4_t* ga = (const u32x4_t*)(Af + (t * r_tiles + ct0) * 2048);
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const int idx = tid + v * THR;
+        const int ia = idx - QB / 16;
+        const u32x4_t* pq = gq + (idx < QB / 16 ? idx : 0);
+        const u32x4_t* pa = ga + (ia < 0 ? 0 : (ia < AB / 16 ? ia : AB / 16 - 1));
+        stg[v] = *(idx < QB / 16 ? pq : pa);
+      }
+    };
+    // unconditional (vectors past the chunk land in the buffer's padding), so that the
+    // stores stay in the loop body's one basic block
+    auto sstore = [&](int buf) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) *(u32x4_t*)(lds + buf * CBP + (tid + v * THR) * 16) = stg[v];
+    };
+    // distance tile S^T (32 j x 32 i) of the chunk in buffer buf, per row block
+    auto dist = [&](int buf, f32x16_t* sd) {
+      bf16x8_t qf[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) qf[ks] = *(const bf16x8_t*)(lds + buf * CBP + ks * 1024 + lane * 16);
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) sd[rb][q] = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+          sd[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qf[ks], pb[rb][ks], sd[rb], 0, 0, 0);
+    };
+    // E = exp2(S^T) in the product's A-operand order: registers 8 kap .. 8 kap + 7
+    auto expo = [&](const f32x16_t* sd, ET8 (*e)[2]) {
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int kap = 0; kap < 2; kap++)
+#pragma unroll
+          for (int tt = 0; tt < 8; tt++) e[rb][kap][tt] = (ET)__builtin_amdgcn_exp2f(sd[rb][8 * kap + tt]);
+    };
+
+    const int64_t nc = n_chunks;
+    gload(0);
+    sstore(0);
+    gload(nc > 1 ? 1 : 0);
+    sstore(1);
+    __syncthreads();
+    ET8 e[RB][2];
+    {
+      f32x16_t s0[RB];
+      dist(0, s0);
+      expo(s0, e);
+    }
+    int bc = 0;  // buffer of chunk c
+    for (int64_t c = 0; c < nc; c++) {
+      const int bn = bc == 2 ? 0 : bc + 1;  // chunk c + 1
+      const int bw = bn == 2 ? 0 : bn + 1;  // chunk c + 2
+      gload(c + 2 < nc ? c + 2 : nc - 1);
+      f32x16_t sn[RB];
+      dist(bn, sn);  // the last iteration forms a spare tile from a copy of chunk nc - 1
+      // O += E_c . A'_c
+#pragma unroll
+      for (int idx = 0; idx < 2 * NT; idx++) {
+        const int ct = idx >> 1, kap = idx & 1;
+        const ET8 bv = *(const ET8*)(lds + bc * CBP + QB + idx * 1024 + lane * 16);
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+          if constexpr (EM == EM_BF16)
+            acc[rb][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[rb][kap], bv, acc[rb][ct], 0, 0, 0);
+          else
+            acc[rb][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[rb][kap], bv, acc[rb][ct], 0, 0, 0);
+        }
+      }
+      expo(sn, e);
+      sstore(bw);
+      // Issue order requested for the body: the staging loads, the Q fragments and four
+      // B fragments, the distance MFMAs (each with a B-fragment read), the product MFMAs
+      // (with the remaining B-fragment reads, and from the 7th on with three VALU
+      // instructions of the next tile's exp2 / fp16 conversion each), the stores.  What the
+      // compiler keeps of it is bounded by the 256 registers of a wave at two waves per
+      // SIMD; measured at 1.6M x 1.6M x 512 (fp16) against the body without the request:
+      // 2.85-2.97 s against 2.92-2.95 s (128-column kernel: 3.05 s).
+      constexpr int PF = 4;
+      __builtin_amdgcn_sched_group_barrier(0x020, NV, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, KS, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, PF, 0);
+#pragma unroll
+      for (int i = 0; i < KS * RB; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 2 * NT * RB; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 2 * NT - KS * RB - PF) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (i >= 6) __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x200, NV, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();
+      bc = bn;
+    }
+    // epilogue: W[i, c] = y1_i O[i, c] (times the fp16 forms' inverse scale, exact);
+    // accumulator register q holds row 8 (q / 4) + 4 h + q % 4, column l32
+    const float unscale = EM == EM_BF16 ? 1.0f : sc[1];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int64_t i = rowbase + 32 * rb + 8 * (q / 4) + 4 * h + (q % 4);
+        if (i >= m) continue;
+        const float yi = y1[i] * unscale;
+#pragma unroll
+        for (int ct = 0; ct < NT; ct++) {
+          const int64_t cc = (ct0 + ct) * 32 + l32;
+          if (cc < r) W[i * ldw + cc] = yi * acc[rb][ct][q];
+        }
+      }
+    }
+    __syncthreads();  // the next tile's prologue overwrites buffers still being read
+  }
+}
+
+// -------------------------------------------------------------------------------------
 // FP32 main kernel.  256 threads = 4 waves; wave w owns rows [32w, 32w+32) of the block's
 // 128 rows and all BN = 32*NT columns.  MFMA 32x32x2 f32 throughout.  LDS (single
 // buffer, register prefetch): Qop chunk 32 x KF, A' chunk 32 x BN (row stride padded).
@@ -671,6 +931,84 @@ struct KpBuffers {
   }
 };
 
+// Widest operand (KD) of the wide-tile kernel
+#define KPW_MAX_KD 128
+
+template <int RB, int NT, int KD, int WAVES>
+static void launch_wide(int em, const __bf16* P, const __bf16* Qf, const void* Af,
+                        const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
+                        int64_t n_chunks, int64_t r_tiles, int64_t nxb, const float* sc,
+                        int cu_count, hipStream_t s) {
+  const int64_t ntiles = nxb * (r_tiles / NT);
+  dim3 grid((unsigned)(ntiles < cu_count ? ntiles : cu_count));
+  if (em == EM_BF16)
+    hipLaunchKernelGGL((kprod_wide<RB, NT, KD, EM_BF16, WAVES>), grid, dim3(64 * WAVES), 0, s, P,
+                       Qf, Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, ntiles);
+  else
+    hipLaunchKernelGGL((kprod_wide<RB, NT, KD, EM_F16, WAVES>), grid, dim3(64 * WAVES), 0, s, P,
+                       Qf, Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, ntiles);
+}
+
+// The one-term low-precision forms with more than 128 right-hand-side columns.
+static tsvm_status product_wide(Ctx* c, KpBuffers& bufs, int em, int KD, const float* X1,
+                                const float* y1, int64_t m, const float* X2, const float* y2,
+                                int64_t n, int64_t d, const float* A, int64_t lda, int64_t r,
+                                double gl2, float* W, int64_t ldw) {
+  hipStream_t s = c->stream;
+  constexpr int NT = 8, WAVES = 8, BR = 32 * WAVES;
+  const int64_t n_pad = round_up(n > 0 ? n : 1, 32), n_chunks = n_pad / 32;
+  const int64_t m_pad = round_up(m, BR), r_pad = round_up(r, 32 * NT);
+  const int64_t r_tiles = r_pad / 32, nxb = m_pad / BR;
+  void *P, *Qf, *Af, *scv = nullptr;
+  TSVM_CHECK(bufs.get(c, "kp_P", m_pad * KD * 2, &P));
+  TSVM_CHECK(bufs.get(c, "kp_Q", n_pad * KD * 2, &Qf));
+  TSVM_CHECK(bufs.get(c, "kp_A", n_pad * r_pad * 2, &Af));
+  if (em != EM_BF16) TSVM_CHECK(bufs.get(c, "kp_sc", 16, &scv));
+  hipLaunchKernelGGL(prep_rows_bf16, dim3((unsigned)ceil_div(m_pad, 256)), dim3(256), 0, s, X1, m,
+                     m_pad, (int)d, KD, gl2, (__bf16*)P);
+  hipLaunchKernelGGL(prep_cols_bf16_frag, dim3((unsigned)ceil_div(n_pad, 256)), dim3(256), 0, s,
+                     X2, n, n_pad, (int)d, KD, gl2, (__bf16*)Qf);
+  float* sc = nullptr;
+  const dim3 ga((unsigned)n_chunks, (unsigned)r_tiles);
+  if (em != EM_BF16) {
+    // one power-of-two scale brings max |y_j A[j,c]| to [2^13, 2^14) in fp16
+    sc = (float*)scv;
+    unsigned int* amax = (unsigned int*)(sc + 2);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, amax);
+    const int64_t tot = (n > 0 ? n : 1) * r;
+    const unsigned nb = (unsigned)std::min<int64_t>(ceil_div(tot, 256), 4096);
+    hipLaunchKernelGGL(amax_kernel, dim3(nb), dim3(256), 0, s, A, lda, n, r, amax);
+    hipLaunchKernelGGL(fp16_scale_kernel, dim3(1), dim3(1), 0, s, amax, sc);
+    hipLaunchKernelGGL(prep_A_frag<_Float16>, ga, dim3(256), 0, s, A, lda, y2, n, r, r_tiles,
+                       (const float*)sc, (_Float16*)Af);
+  } else {
+    hipLaunchKernelGGL(prep_A_frag<__bf16>, ga, dim3(256), 0, s, A, lda, y2, n, r, r_tiles,
+                       (const float*)nullptr, (__bf16*)Af);
+  }
+  TSVM_HIP(c, hipGetLastError());
+  const __bf16 *Pb = (const __bf16*)P, *Qb = (const __bf16*)Qf;
+#define KPW_CASE(kd)                                                                            \
+  case kd:                                                                                      \
+    launch_wide<1, NT, kd, WAVES>(em, Pb, Qb, Af, y1, W, ldw, m, r, n_chunks, r_tiles, nxb, sc, \
+                                  c->cu_count, s);                                              \
+    break;
+  switch (KD) {
+    KPW_CASE(16)
+    KPW_CASE(32)
+    KPW_CASE(48)
+    KPW_CASE(64)
+    KPW_CASE(80)
+    KPW_CASE(96)
+    KPW_CASE(112)
+    KPW_CASE(128)
+    default:
+      return fail(c, TSVM_ERR_ARG, "kernel_product(wide): KD=%d not compiled", KD);
+  }
+#undef KPW_CASE
+  TSVM_HIP(c, hipGetLastError());
+  return TSVM_OK;
+}
+
 static tsvm_status kernel_product_impl(Ctx* c, KpBuffers& bufs, const float* X1, const float* y1,
                                        int64_t m, const float* X2, const float* y2, int64_t n,
                                        int64_t d, const float* A, int64_t lda, int64_t r,
@@ -697,6 +1035,8 @@ static tsvm_status kernel_product_impl(Ctx* c, KpBuffers& bufs, const float* X1,
     if (KD == 0)
       return fail(c, TSVM_ERR_ARG, "kernel_product(bf16): d=%ld exceeds 84 features", (long)d);
     const bool two_term = em == EM_F16X2 || em == EM_F16R2;
+    if (!two_term && r > 128 && KD <= KPW_MAX_KD)
+      return product_wide(c, bufs, em, KD, X1, y1, m, X2, y2, n, d, A, lda, r, gl2, W, ldw);
     const int NT = (r <= 32) ? 1 : (r <= 64 ? 2 : (r <= 128 || !two_term ? 4 : 8));
     const int64_t BN = NT * 32, m_pad = round_up(
                       m, em == EM_F16 && NT == 4 && KD <= KBF16_WIDE_MAX_KD ? 512 : 256),

@@ -114,3 +114,21 @@ def test_kprod_fp16x2_exact_rhs(ctx):
     rel = np.abs(W - ref) / np.abs(ref)
     assert np.median(rel) < 2.0 ** -14
     assert rel.max() < 2.0 ** -12
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_kprod_wide_tiles_match_narrow(ctx, prec):
+    """Right-hand sides wider than 128 columns run the 256-column-tile kernel; its outputs
+    are bit-identical to the 128-column kernel's on each half (same distance tiles, same
+    accumulation order).  For fp16 both halves get the same power-of-two scale by carrying
+    the same largest |A| entry."""
+    import tensorsvm_amd as T
+    p = T.PREC_BF16 if prec == "bf16" else T.PREC_FP16
+    m, n, d, r = 1000, 3000, 18, 256
+    X1, y1, X2, y2, A = _case(m, n, d, r, 0.05, 123)
+    A[0, 0], A[0, 128] = 7.5, 7.5
+    W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, p)
+    W0 = ctx.kernel_product(X1, y1, X2, y2, np.ascontiguousarray(A[:, :128]), 0.05, p)
+    W1 = ctx.kernel_product(X1, y1, X2, y2, np.ascontiguousarray(A[:, 128:]), 0.05, p)
+    assert np.array_equal(W[:, :128], W0)
+    assert np.array_equal(W[:, 128:], W1)
