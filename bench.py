@@ -24,8 +24,12 @@ Reported beside the timing:
                 gradient of the trained dual and of the held-out smoothed hinge in
                 (log gamma, 32 log-scales), backward flops / time against the FP32 peak,
                 with a central-difference check of the log-gamma component (h = 1e-3)
-  cpu_baseline  the CPU oracle's kernel product (FP64, all host threads) on a bounded row
-                sample, extrapolated to the two full products of the factor construction
+  cpu_baseline  the same algorithm on the host cores through a CPU BLAS (oracle/cpu_blas.py:
+                the reference's FP32 tile pipeline for the kernel products, its FP64
+                interior-point solver), timed on bounded samples and extrapolated: one
+                product on a 50,000 x 50,000 slice scaled by (n / 50,000)^2, two of them;
+                solver iterations on 200,000 rows of a factor-shaped matrix scaled by
+                n / 200,000 and by the GPU run's iteration count
 """
 import argparse
 import json
@@ -50,7 +54,7 @@ def log(*a):
 # HBM bytes per launch of the factor's product kernel at the default workload (mean of its
 # two launches): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the
 # factor that the same counters give for the bench's 3.28 GB Omega copy (x2.0 and x1.0)
-KPROD_TRAFFIC = {"fetch_bytes": 251_517_896_384, "write_bytes": 3_276_800_000,
+KPROD_TRAFFIC = {"fetch_bytes": 408_385_681_856, "write_bytes": 3_276_800_000,
                  "source": "profiles/kprod_factor_c2_pmc.md"}
 
 
@@ -112,6 +116,44 @@ def grad_form(ctx, torch, n=200_000, nt=50_000, d=32, k=256, C=1.0, seed=1003):
                     "gap_rel": info.ipm.gap_rel}, "nsv": info.nsv}
 
 
+def cpu_baseline(X, y, gamma, k, ntr, iters, prod_rows=50_000, ipm_rows=200_000):
+    """The reference's plain method on the host cores with a CPU BLAS (MKL through torch's
+    CPU kernels), timed on bounded samples of the workload and extrapolated."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cpu_blas
+    th = torch.get_num_threads()
+    m = min(prod_rows, ntr)
+    g = torch.Generator().manual_seed(3)
+    Xs, ys = torch.from_numpy(X[:m]), torch.from_numpy(y[:m])
+    A = torch.randn(m, k, generator=g)
+    t0 = time.perf_counter()
+    cpu_blas.kernel_product_tiles(Xs, ys, Xs, ys, A, gamma)
+    tp = time.perf_counter() - t0
+    prod_est = tp * (ntr / m) ** 2 * 2.0
+    del A
+    ni = min(ipm_rows, ntr)
+    Z = torch.randn(ni, k, generator=g, dtype=torch.float64) * 0.05
+    a = torch.where(torch.rand(ni, generator=g) < 0.5, 1.0, -1.0).double()
+    t0 = time.perf_counter()
+    cpu_blas.mpc(Z, a, 32.0, iter_limit=0)
+    t_init = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cpu_blas.mpc(Z, a, 32.0, iter_limit=2)
+    t_it = (time.perf_counter() - t0 - t_init) / 2.0
+    ipm_est = t_it * (ntr / ni) * iters
+    log(f"cpu baseline: product sample {tp:.1f}s -> {prod_est:.0f}s, ipm {t_it:.2f}s/it -> {ipm_est:.0f}s")
+    return {"value": round(prod_est + ipm_est, 1), "unit": "s", "cores": th, "kind": "port",
+            "extrapolated": True, "blas": "MKL through torch CPU kernels (sgemm, dgemm, potrf)",
+            "factor_products_s": round(prod_est, 1), "ipm_s": round(ipm_est, 1),
+            "sample": f"the reference's FP32 tile pipeline (8192^2 tiles: XIJ sgemm, exp, K A "
+                      f"sgemm) for one product over a {m} x {m} slice, k={k}: {tp:.2f}s, "
+                      f"x (n/{m})^2 x 2 products; its FP64 IPM (dsyrk-as-GEMM capacitance, "
+                      f"potrf, SMW gemv) {t_it:.3f}s per iteration on {ni} x {k}, x n/{ni} x "
+                      f"{iters} iterations (the GPU run's count); {th} threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,7 +163,6 @@ def main():
     ap.add_argument("--k", type=int, default=512)
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-rows", type=int, default=384)
     ap.add_argument("--no-grad", action="store_true", help="skip the differentiation form")
     ap.add_argument("--data", choices=("gaussian", "covtype"), default="gaussian",
                     help="gaussian: the mixture of the default workload (d=18); covtype: the "
@@ -210,6 +251,7 @@ def main():
             f"{info.ipm.iters} it, reason {info.ipm.reason}, gap {info.ipm.gap_rel:.2e})")
     acc_bf16, alpha_bf16 = accuracy(info)
     bf16_info = info.as_dict()
+    bf16_b = info.b
     value = float(np.mean(times))
 
     grad_timed = None
@@ -241,8 +283,11 @@ def main():
     if not args.no_fp32 and args.precision == "bf16":
         dt32, info32 = run(T.PREC_FP32)
         acc32, alpha32 = accuracy(info32)
+        f32 = ctx.decision_function(X, y, alpha32, gamma, info32.b, Xte)
+        f16 = ctx.decision_function(X, y, alpha_bf16, gamma, bf16_b, Xte)
         fp32 = {"train_s": dt32, "accuracy": acc32, "info": info32.as_dict(),
-                "alpha_max_abs_diff_vs_bf16": float(np.abs(alpha32 - alpha_bf16).max())}
+                "alpha_max_abs_diff_vs_bf16": float(np.abs(alpha32 - alpha_bf16).max()),
+                "heldout_sign_agreement_vs_bf16": float(np.mean(np.sign(f32) == np.sign(f16)))}
         log(f"fp32 train {dt32:.2f}s (lra {info32.lra.t_total:.2f}s), accuracy {acc32:.4f}")
 
     # roofline of the dominant kernel.  BF16 form: both products of the factor run the
@@ -252,7 +297,8 @@ def main():
     t1, t2 = bf16_info["lra"]["t_product1"], bf16_info["lra"]["t_product2"]
     tr = KPROD_TRAFFIC
     if args.precision == "bf16":
-        peak, kname = PEAK_BF16, "kprod_bf16<4,64,2,EM_F16,1,512> (fused RBF product, K_Y Omega and K_Y W)"
+        peak, kname = PEAK_BF16, ("kprod_wide<1,8,64,EM_F16,8> (fused RBF product, 256-column "
+                                  "tiles, K_Y Omega and K_Y W)")
     else:
         peak, kname = PEAK_FP32, "kprod_f32 (fused RBF product, K_Y Omega and K_Y W)"
     t_kp = 0.5 * (t1 + t2)
@@ -270,23 +316,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        import oracle_lib as O
-        m = args.cpu_rows
-        A = np.random.default_rng(3).standard_normal((ntr, k))
-        t0 = time.perf_counter()
-        O.kernel_product(X[:m], y[:m], X, y, A, gamma)
-        tc = time.perf_counter() - t0
-        est = tc * (ntr / m) * 2.0
-        cpu = {"value": round(est, 1), "unit": "s", "cores": O.lib().orc_num_threads(), "kind": "port",
-               "sample": f"oracle kernel product (FP64) of {m} rows x {ntr} columns x k={k} took "
-                         f"{tc:.1f}s; extrapolated x{ntr / m:.0f} rows x 2 products (the factor "
-                         "construction's >98% of flops; IPM not included)"}
-        log(f"cpu baseline: sample {tc:.1f}s -> {est:.0f}s")
+        cpu = cpu_baseline(X, y, gamma, k, ntr, int(bf16_info["ipm"]["iters"]))
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(value * 1000, 1), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16/fp16" if args.precision == "bf16" else "fp32", "data": "synthetic",
         "config": {"workload": f"{'gaussian-mixture' if args.data == 'gaussian' else 'covtype-like'} "
                                f"n={args.n / 1e6:g}M ({ntr / 1e6:g}M train / "
                                f"{(args.n - ntr) / 1e6:g}M held-out), d={d}, k={k}, "
@@ -307,7 +343,10 @@ def main():
                 "t_capacitance_s": round(bf16_info["ipm"]["t_syrk"], 3),
                 "t_bf16_part_s": round(bf16_info["ipm"]["t_syrk_bf16"], 3),
                 "fp32_form_fp64_tflops": (round(fp32["info"]["ipm"]["syrk_tflops"], 1) if fp32 else None),
-                "timed_form_capacitance_tflops": round(bf16_info["ipm"]["syrk_tflops"], 1)},
+                "capacitance_tflops": round(bf16_info["ipm"]["syrk_tflops"], 1),
+                "capacitance_frac_of_bf16_peak": round(bf16_info["ipm"]["syrk_tflops"] / PEAK_BF16, 4),
+                "capacitance_ms_per_iter": round(1e3 * bf16_info["ipm"]["t_syrk"]
+                                                 / max(bf16_info["ipm"]["iters"], 1), 3)},
         "grad": grad,
         "grad_timed": grad_timed,
     }
