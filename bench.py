@@ -174,6 +174,9 @@ def main():
                          "size, forward products in the timed form's precision")
     ap.add_argument("--progress", action="store_true",
                     help="progress lines on stderr from the library (row slabs of each product)")
+    ap.add_argument("--fp64-ref", action="store_true",
+                    help="train once more with FP64 factor products (outside the timed steps) and "
+                         "state the timed form's and the FP32 form's error against it")
     ap.add_argument("--precision", choices=("bf16", "fp32"), default="bf16",
                     help="form of the timed steps (the comparison run, if any, is FP32)")
     args = ap.parse_args()
@@ -290,6 +293,30 @@ def main():
                 "heldout_sign_agreement_vs_bf16": float(np.mean(np.sign(f32) == np.sign(f16)))}
         log(f"fp32 train {dt32:.2f}s (lra {info32.lra.t_total:.2f}s), accuracy {acc32:.4f}")
 
+    fp64_ref = None
+    if args.fp64_ref:
+        dt64, info64 = run(T.PREC_FP64)
+        acc64, alpha64 = accuracy(info64)
+        f64 = ctx.decision_function(X, y, alpha64, gamma, info64.b, Xte)
+        p64 = info64.ipm.pobj
+
+        def err(alpha_x, b_x, ipm_x):
+            fx = ctx.decision_function(X, y, alpha_x, gamma, b_x, Xte)
+            return {"alpha_max_abs_diff_over_C": float(np.abs(alpha_x - alpha64).max() / C),
+                    "alpha_mean_abs_diff_over_C": float(np.abs(alpha_x - alpha64).mean() / C),
+                    "b_abs_diff": abs(b_x - info64.b),
+                    "pobj_rel_err": abs(ipm_x["pobj"] - p64) / abs(p64),
+                    "gap_rel": ipm_x["gap_rel"],
+                    "heldout_sign_agreement": float(np.mean(np.sign(fx) == np.sign(f64)))}
+        fp64_ref = {"form": "both factor products on the FP64 MFMA (factor stored in FP32), "
+                            "FP64 capacitance, same stop rule",
+                    "train_s": round(dt64, 2), "accuracy": acc64, "b": info64.b,
+                    "pobj": p64, "gap_rel": info64.ipm.gap_rel, "iters": info64.ipm.iters,
+                    "t_product1_s": round(info64.lra.t_product1, 2),
+                    "timed_form_vs_fp64": err(alpha_bf16, bf16_b, bf16_info["ipm"]),
+                    "fp32_vs_fp64": (err(alpha32, info32.b, fp32["info"]["ipm"]) if fp32 else None)}
+        log(f"fp64 reference train {dt64:.1f}s, accuracy {acc64:.4f}")
+
     # roofline of the dominant kernel.  BF16 form: both products of the factor run the
     # same kernel (fp16 product), so its average launch is the mean of the two; the
     # FP32 form likewise runs kprod_f32 twice.
@@ -349,6 +376,7 @@ def main():
                                                  / max(bf16_info["ipm"]["iters"], 1), 3)},
         "grad": grad,
         "grad_timed": grad_timed,
+        "fp64_ref": fp64_ref,
     }
     # the timed form under its own name; with BF16 timed, "fp32" holds the comparison run
     out[args.precision] = {"train_s": value, "accuracy": acc_bf16, "info": bf16_info}

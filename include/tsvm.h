@@ -42,6 +42,11 @@ typedef int tsvm_status;
 #define TSVM_PREC_FP16X2 2     /* E as two fp16 terms (~2^-22), A fp16 (2^-11) */
 #define TSVM_PREC_FP16 3       /* E and A fp16 (2^-11 each), one MFMA per tile */
 #define TSVM_PREC_FP16R2 4     /* E fp16 (2^-11), A as two fp16 terms (~2^-22) */
+/* FP64 throughout (distance, exp, product on the FP64 MFMA; FP32 output): the reference
+ * against which the other forms' error is stated.  As a training / factor form: both
+ * products of the factor in FP64, the rest as TSVM_PREC_FP32 (factor stored in FP32,
+ * FP64 capacitance). */
+#define TSVM_PREC_FP64 5
 
 /* Largest rank k of the factor (and width of Z in the interior-point solver): the
  * capacitance Cholesky and its solves keep a k x k FP64 matrix per workgroup.  The
@@ -167,7 +172,8 @@ tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int
  *             (1 - z)^2 / 2 (0 < z < 1), 0 (z >= 1)
  * g_dual, g_loss (d + 1 each, host; component 0 = d/dlog gamma) may be NULL.  The
  * backward pass recomputes kernel tiles in the fused FP32 product over derivative
- * right-hand sides; nothing of size n^2 is stored.  precision: the forward pass's two
+ * right-hand sides; nothing of size n^2 is stored.  The held-out decision values behind
+ * L come from the FP64 product, so that differences of L resolve its derivative.  precision: the forward pass's two
  * products of the factor (K_Y Omega, K_Y W) in TSVM_PREC_FP32, or in the fp16 product
  * with TSVM_PREC_BF16 (about 8x faster at large n; the backward stays FP32).  Device: X
  * (n x d), y, alpha (FP64), omega (n x k, not modified), Xt (nt x d), yt; host:

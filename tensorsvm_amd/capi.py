@@ -15,6 +15,8 @@ PREC_BF16 = 1
 PREC_FP16X2 = 2  # kernel values as two fp16 terms; as a training form: BF16 with a two-term sketch
 PREC_FP16 = 3    # kernel product only: kernel values and right-hand side in fp16
 PREC_FP16R2 = 4  # kernel product only: right-hand side as two fp16 terms
+PREC_FP64 = 5    # FP64 products (the reference of the other forms); as a training form:
+                 # both factor products in FP64, the rest as the FP32 form
 
 STOP_REF = 0
 STOP_GAP = 1

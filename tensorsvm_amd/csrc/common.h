@@ -9,6 +9,7 @@ typedef __attribute__((ext_vector_type(8))) _Float16 f16x8_t;
 typedef __attribute__((ext_vector_type(16))) float f32x16_t;
 typedef __attribute__((ext_vector_type(4))) float f32x4_t;
 typedef __attribute__((ext_vector_type(4))) double f64x4_t;
+typedef __attribute__((ext_vector_type(2))) double f64x2_t;
 typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 
 #define TSVM_LOG2E 1.4426950408889634

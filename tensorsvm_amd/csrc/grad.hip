@@ -380,8 +380,16 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
                                   TSVM_PREC_FP32, Wd, rte));
     TSVM_CHECK(progress("backward product of the held-out loss"));
     TSVM_HIP(c, hipEventRecord(e1, s));
-    std::vector<float> hw(nt * rte);
+    // the decision values behind L once more on the FP64 product (column 0 of the
+    // right-hand side, q = alpha y): L then carries FP64 accumulation error (~1e-15)
+    // instead of FP32's (~1e-7), so that central differences of L resolve its derivative
+    void* pf;
+    TSVM_HIP(c, c->ws.get("gr_f64", (size_t)(nt * 4), &pf));
+    TSVM_CHECK(kernel_product_dev(c, Xts, ones, nt, Xs, ones, n, d, Rh, rte, 1, gamma,
+                                  TSVM_PREC_FP64, (float*)pf, 1));
+    std::vector<float> hw(nt * rte), hf(nt);
     TSVM_HIP(c, hipMemcpyAsync(hw.data(), Wd, nt * rte * 4, hipMemcpyDeviceToHost, s));
+    TSVM_HIP(c, hipMemcpyAsync(hf.data(), pf, nt * 4, hipMemcpyDeviceToHost, s));
     std::vector<float> hy(nt);
     TSVM_HIP(c, hipMemcpyAsync(hy.data(), yt, nt * 4, hipMemcpyDeviceToHost, s));
     TSVM_HIP(c, hipStreamSynchronize(s));
@@ -390,7 +398,7 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
     std::vector<double> pt(nt);
     double loss = 0.0;
     for (int64_t t = 0; t < nt; t++) {
-      const double z = (double)hy[t] * ((double)hw[t * rte] + b);
+      const double z = (double)hy[t] * ((double)hf[t] + b);
       double h, dh;
       if (z <= 0.0) {
         h = 0.5 - z;

@@ -844,6 +844,165 @@ __global__ __launch_bounds__(256) void kprod_f32(const float* __restrict__ P,
 }
 
 // -------------------------------------------------------------------------------------
+// FP64 kernel (TSVM_PREC_FP64): the distance, exp and product in double on
+// v_mfma_f64_16x16x4f64, the FP64 reference of the low-precision and FP32 forms.
+// Operands: P64 [m_pad][KF] = [x, 1, -g'|x|^2], Q64 [n_pad][KF] = [2 g' x, -g'|x|^2, 1]
+// (KF = roundup(d + 2, 4)), A64 [n_pad][r_pad] = y_j A[j, c].  256 threads; wave w owns
+// 32 rows (two 16-row tiles) and the block's 128 columns (8 tiles of 16): 64 FP64
+// accumulators per tile pair.  Per 16-column chunk j: S^T (16 j x 16 i) = Q64_j P64_i^T
+// (KF / 4 MFMAs per row tile; item q of lane l holds j = l / 16 + 4 q, i = l % 16), E =
+// exp2(S^T) in FP64, and O (i x c) += E A64_j with K-step q taking j = 4 q + l / 16, the
+// j that item q of the lane holds.  One LDS stage (register prefetch of the next).
+// -------------------------------------------------------------------------------------
+__global__ void prep_rows_f64(const float* __restrict__ X, int64_t m, int64_t m_pad, int d, int KF,
+                              double gl2, double* __restrict__ P) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= m_pad) return;
+  double* p = P + i * KF;
+  if (i >= m) {
+    for (int t = 0; t < KF; t++) p[t] = 0.0;
+    return;
+  }
+  const float* x = X + i * d;
+  double nrm = 0.0;
+  for (int t = 0; t < d; t++) {
+    nrm += (double)x[t] * x[t];
+    p[t] = x[t];
+  }
+  p[d] = 1.0;
+  p[d + 1] = -gl2 * nrm;
+  for (int t = d + 2; t < KF; t++) p[t] = 0.0;
+}
+__global__ void prep_cols_f64(const float* __restrict__ X, int64_t n, int64_t n_pad, int d, int KF,
+                              double gl2, double* __restrict__ Q) {
+  int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= n_pad) return;
+  double* q = Q + j * KF;
+  if (j >= n) {
+    for (int t = 0; t < KF; t++) q[t] = 0.0;
+    return;
+  }
+  const float* x = X + j * d;
+  double nrm = 0.0;
+  for (int t = 0; t < d; t++) {
+    nrm += (double)x[t] * x[t];
+    q[t] = 2.0 * gl2 * (double)x[t];
+  }
+  q[d] = -gl2 * nrm;
+  q[d + 1] = 1.0;
+  for (int t = d + 2; t < KF; t++) q[t] = 0.0;
+}
+__global__ void prep_A_f64(const float* __restrict__ A, int64_t lda, const float* __restrict__ y,
+                           int64_t n, int64_t n_pad, int64_t r, int64_t r_pad, double* __restrict__ Ad) {
+  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= n_pad * r_pad) return;
+  int64_t j = idx / r_pad, c = idx - j * r_pad;
+  Ad[idx] = (j < n && c < r) ? (double)y[j] * (double)A[j * lda + c] : 0.0;
+}
+
+template <int KF>
+__global__ __launch_bounds__(256) void kprod_f64(const double* __restrict__ P,
+                                                 const double* __restrict__ Q,
+                                                 const double* __restrict__ Ad,
+                                                 const float* __restrict__ y1,
+                                                 float* __restrict__ W, int64_t ldw, int64_t m,
+                                                 int64_t r, int64_t n_chunks, int64_t r_pad) {
+  constexpr int NT = 8, BN = 16 * NT, KS = KF / 4;
+  constexpr int QV = 16 * KF / 2;      // 16-byte vectors (two doubles) of a Q chunk
+  constexpr int AV = 16 * BN / 2;      // of an A64 chunk slice
+  constexpr int NV = (QV + AV + 255) / 256;
+  __shared__ __attribute__((aligned(16))) double sQ[16 * KF];
+  __shared__ __attribute__((aligned(16))) double sA[16 * BN];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int64_t rowbase = (int64_t)blockIdx.x * 128 + wave * 32;
+  const int64_t c0 = (int64_t)blockIdx.y * BN;
+  // B operand of the distance MFMA: row i = l16 of each 16-row tile, K index 4 s + kq
+  double pb[2][KS];
+#pragma unroll
+  for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) pb[rt][ks] = P[(rowbase + 16 * rt + l16) * KF + 4 * ks + kq];
+  f64x4_t acc[2][NT];
+#pragma unroll
+  for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++) acc[rt][ct] = (f64x4_t){0, 0, 0, 0};
+  f64x2_t stage[NV];
+  auto gload = [&](int64_t t) {
+    const f64x2_t* gq = (const f64x2_t*)(Q + t * 16 * KF);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const int idx = tid + v * 256;
+      int a = idx - QV;
+      a = a < 0 ? 0 : (a < AV ? a : AV - 1);
+      const int row = a / (BN / 2), col = (a % (BN / 2)) * 2;
+      const f64x2_t* pa = (const f64x2_t*)(Ad + (t * 16 + row) * r_pad + c0 + col);
+      const f64x2_t* pq = gq + (idx < QV ? idx : 0);
+      stage[v] = *(idx < QV ? pq : pa);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const int idx = tid + v * 256;
+      if (idx < QV) {
+        *(f64x2_t*)&sQ[idx * 2] = stage[v];
+      } else if (idx < QV + AV) {
+        const int a = idx - QV;
+        *(f64x2_t*)&sA[(a / (BN / 2)) * BN + (a % (BN / 2)) * 2] = stage[v];
+      }
+    }
+  };
+  gload(0);
+  for (int64_t t = 0; t < n_chunks; t++) {
+    sstore();
+    __syncthreads();
+    gload(t + 1 < n_chunks ? t + 1 : t);
+    f64x4_t sd[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++) sd[rt] = (f64x4_t){0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const double qa = sQ[l16 * KF + 4 * ks + kq];
+#pragma unroll
+      for (int rt = 0; rt < 2; rt++)
+        sd[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa, pb[rt][ks], sd[rt], 0, 0, 0);
+    }
+    double e[2][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) e[rt][q] = exp2(sd[rt][q]);
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const double bv = sA[(4 * q + kq) * BN + ct * 16 + l16];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(e[rt][q], bv, acc[rt][ct], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  // item q of lane l holds row l / 16 + 4 q, column l % 16 of each 16 x 16 tile
+#pragma unroll
+  for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int64_t i = rowbase + 16 * rt + kq + 4 * q;
+      if (i >= m) continue;
+      const double yi = y1[i];
+#pragma unroll
+      for (int ct = 0; ct < NT; ct++) {
+        const int64_t c = c0 + ct * 16 + l16;
+        if (c < r) W[i * ldw + c] = (float)(yi * acc[rt][ct][q]);
+      }
+    }
+}
+
+// -------------------------------------------------------------------------------------
 // host launchers
 // -------------------------------------------------------------------------------------
 // Tiling per arithmetic (measured on the 1.6M x 1.6M x 512 sketch shape): the
@@ -1099,6 +1258,42 @@ static tsvm_status kernel_product_impl(Ctx* c, KpBuffers& bufs, const float* X1,
         return fail(c, TSVM_ERR_ARG, "kernel_product(bf16): KD=%d not compiled", KD);
     }
 #undef KP_BF16_CASE
+  } else if (precision == TSVM_PREC_FP64) {
+    const int64_t n_pad = round_up(n > 0 ? n : 1, 16), n_chunks = n_pad / 16;
+    // operand width: the smallest compiled size holding [x, norm terms] (zero padded)
+    static const int kfs[] = {4, 8, 12, 16, 20, 24, 28, 32, 36, 40, 48, 56, 64, 80, 96, 112, 132};
+    int KF = 0;
+    for (int v : kfs)
+      if (v >= d + 2) { KF = v; break; }
+    if (KF == 0)
+      return fail(c, TSVM_ERR_ARG, "kernel_product(fp64): d=%ld exceeds 130 features", (long)d);
+    const int64_t m_pad = round_up(m, 128), r_pad = round_up(r, 128);
+    void *P, *Q, *Ad;
+    TSVM_CHECK(bufs.get(c, "kp_P", m_pad * KF * 8, &P));
+    TSVM_CHECK(bufs.get(c, "kp_Q", n_pad * KF * 8, &Q));
+    TSVM_CHECK(bufs.get(c, "kp_A", n_pad * r_pad * 8, &Ad));
+    hipLaunchKernelGGL(prep_rows_f64, dim3((unsigned)ceil_div(m_pad, 256)), dim3(256), 0, s, X1, m,
+                       m_pad, (int)d, KF, gl2, (double*)P);
+    hipLaunchKernelGGL(prep_cols_f64, dim3((unsigned)ceil_div(n_pad, 256)), dim3(256), 0, s, X2, n,
+                       n_pad, (int)d, KF, gl2, (double*)Q);
+    hipLaunchKernelGGL(prep_A_f64, dim3((unsigned)ceil_div(n_pad * r_pad, 256)), dim3(256), 0, s,
+                       A, lda, y2, n, n_pad, r, r_pad, (double*)Ad);
+    TSVM_HIP(c, hipGetLastError());
+    const dim3 grid((unsigned)(m_pad / 128), (unsigned)(r_pad / 128));
+#define KP_F64_CASE(kf)                                                                          \
+  case kf:                                                                                       \
+    hipLaunchKernelGGL((kprod_f64<kf>), grid, dim3(256), 0, s, (const double*)P, (const double*)Q, \
+                       (const double*)Ad, y1, W, ldw, m, r, n_chunks, r_pad);                     \
+    break;
+    switch (KF) {
+      KP_F64_CASE(4) KP_F64_CASE(8) KP_F64_CASE(12) KP_F64_CASE(16) KP_F64_CASE(20)
+      KP_F64_CASE(24) KP_F64_CASE(28) KP_F64_CASE(32) KP_F64_CASE(36) KP_F64_CASE(40)
+      KP_F64_CASE(48) KP_F64_CASE(56) KP_F64_CASE(64) KP_F64_CASE(80) KP_F64_CASE(96)
+      KP_F64_CASE(112) KP_F64_CASE(132)
+      default:
+        return fail(c, TSVM_ERR_ARG, "kernel_product(fp64): KF=%d not compiled", KF);
+    }
+#undef KP_F64_CASE
   } else if (precision == TSVM_PREC_FP32) {
     const int64_t n_pad = round_up(n > 0 ? n : 1, 32), n_chunks = n_pad / 32;
     static const int kfs[] = {4, 8, 12, 16, 20, 24, 34, 56, 64, 96, 130};

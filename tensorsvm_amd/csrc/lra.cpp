@@ -130,7 +130,8 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
                     double gamma, int precision, float* Om, float* U, tsvm_lra_info* info) {
   if (n <= 0 || d <= 0 || k <= 0 || k > n)
     return fail(c, TSVM_ERR_ARG, "lra: need 0 < k <= n (n=%ld k=%ld)", (long)n, (long)k);
-  if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16 && precision != TSVM_PREC_FP16X2)
+  if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16 &&
+      precision != TSVM_PREC_FP16X2 && precision != TSVM_PREC_FP64)
     return fail(c, TSVM_ERR_ARG, "lra: unknown precision %d", precision);
   hipStream_t s = c->stream;
   tsvm_lra_info inf;
@@ -169,7 +170,8 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   // 4. Q = K_Y W (into the Omega buffer)
   float* Q = Om;
   TSVM_HIP(c, hipEventRecord(e0, s));
-  const int q_prec = precision == TSVM_PREC_FP32 ? TSVM_PREC_FP32 : TSVM_PREC_FP16;
+  const int q_prec = precision == TSVM_PREC_FP32 || precision == TSVM_PREC_FP64 ? precision
+                                                                               : TSVM_PREC_FP16;
   TSVM_CHECK(product_rows(c, "factor: K_Y W", X, y, n, d, W, k, gamma, q_prec, Q));
   TSVM_HIP(c, hipEventRecord(e1, s));
   TSVM_HIP(c, hipStreamSynchronize(s));

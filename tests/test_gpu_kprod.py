@@ -132,3 +132,15 @@ def test_kprod_wide_tiles_match_narrow(ctx, prec):
     W1 = ctx.kernel_product(X1, y1, X2, y2, np.ascontiguousarray(A[:, 128:]), 0.05, p)
     assert np.array_equal(W[:, :128], W0)
     assert np.array_equal(W[:, 128:], W1)
+
+
+@pytest.mark.parametrize("m,n,d,r", [(300, 257, 18, 40), (1000, 1000, 18, 512), (513, 700, 54, 100),
+                                     (129, 64, 32, 256), (77, 33, 5, 7), (200, 3001, 41, 130)])
+def test_kprod_fp64(ctx, m, n, d, r):
+    """FP64 form: distance, exp and product in double; only the FP32 output rounds."""
+    import tensorsvm_amd as T
+    X1, y1, X2, y2, A = _case(m, n, d, r, 0.05, m + n + 4)
+    W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, T.PREC_FP64)
+    ref = O.kernel_product(X1, y1, X2, y2, A, 0.05)
+    err = np.abs(W - ref)
+    assert (err <= 2.0 ** -24 * np.abs(ref) + 1e-12 * np.abs(ref).max()).all(), err.max()
