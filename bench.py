@@ -177,8 +177,9 @@ def main():
     ap.add_argument("--fp64-ref", action="store_true",
                     help="train once more with FP64 factor products (outside the timed steps) and "
                          "state the timed form's and the FP32 form's error against it")
-    ap.add_argument("--precision", choices=("bf16", "fp32"), default="bf16",
-                    help="form of the timed steps (the comparison run, if any, is FP32)")
+    ap.add_argument("--precision", choices=("bf16", "fp32", "fp8"), default="bf16",
+                    help="form of the timed steps (the comparison run, if any, is FP32); fp8: the "
+                         "BF16 form with the FP8 e4m3 pass of the capacitance")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the trainer "
@@ -242,7 +243,8 @@ def main():
         f = ctx.decision_function(X, y, alpha, gamma, info.b, Xte)
         return float(np.mean(np.sign(f) == yte)), alpha
 
-    prec = T.PREC_BF16 if args.precision == "bf16" else T.PREC_FP32
+    prec = {"bf16": T.PREC_BF16, "fp32": T.PREC_FP32, "fp8": T.PREC_FP8}[args.precision]
+    low = args.precision in ("bf16", "fp8")  # fp16/bf16 factor products
     for w in range(args.warmup):
         dt, _ = run(prec)
         log(f"warmup {w}: {dt:.2f}s")
@@ -261,7 +263,7 @@ def main():
     if args.grad_timed:
         # the multipliers of the timed run are still in dal (the comparison run comes later)
         dXte, dyte = torch.from_numpy(Xte).to(dev), torch.from_numpy(yte).to(dev)
-        gprec = T.PREC_BF16 if args.precision == "bf16" else T.PREC_FP32
+        gprec = T.PREC_BF16 if low else T.PREC_FP32
         torch.cuda.synchronize()
         tg0 = time.perf_counter()
         Dg, Lg, gDg, gLg, gi = ctx.grad_dev(dX, dy, ntr, d, dal, info.b, k, float(np.log(gamma)),
@@ -271,7 +273,7 @@ def main():
         tg = time.perf_counter() - tg0
         grad_timed = {"workload": f"gradient of the timed model in (log gamma, {d} log-scales), "
                                   f"{ntr} train + {Xte.shape[0]} held-out rows",
-                      "forward_products": "fp16" if args.precision == "bf16" else "fp32",
+                      "forward_products": "fp16" if low else "fp32",
                       "t_grad_s": round(tg, 3), "t_backward_s": round(gi.t_backward, 3),
                       "backward_tflops": round(gi.backward_tflops, 1),
                       "roofline": {"bound": "mfma", "achieved": round(gi.backward_tflops, 1),
@@ -283,7 +285,7 @@ def main():
         del dXte, dyte
 
     fp32 = None
-    if not args.no_fp32 and args.precision == "bf16":
+    if not args.no_fp32 and low:
         dt32, info32 = run(T.PREC_FP32)
         acc32, alpha32 = accuracy(info32)
         f32 = ctx.decision_function(X, y, alpha32, gamma, info32.b, Xte)
@@ -323,7 +325,7 @@ def main():
     flops = 2.0 * ntr * ntr * (d + k)
     t1, t2 = bf16_info["lra"]["t_product1"], bf16_info["lra"]["t_product2"]
     tr = KPROD_TRAFFIC
-    if args.precision == "bf16":
+    if low:
         peak, kname = PEAK_BF16, ("kprod_wide<1,8,64,EM_F16,8> (fused RBF product, 256-column "
                                   "tiles, K_Y Omega and K_Y W)")
     else:
@@ -349,7 +351,8 @@ def main():
         "metric": METRIC, "value": round(value, 3), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(value * 1000, 1), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16/fp16" if args.precision == "bf16" else "fp32", "data": "synthetic",
+        "dtype": {"bf16": "bf16/fp16", "fp8": "bf16/fp16/fp8-e4m3", "fp32": "fp32"}[args.precision],
+        "data": "synthetic",
         "config": {"workload": f"{'gaussian-mixture' if args.data == 'gaussian' else 'covtype-like'} "
                                f"n={args.n / 1e6:g}M ({ntr / 1e6:g}M train / "
                                f"{(args.n - ntr) / 1e6:g}M held-out), d={d}, k={k}, "
@@ -366,6 +369,7 @@ def main():
                 "bf16_tflops": round(bf16_info["ipm"]["syrk_bf16_tflops"], 1),
                 "bf16_frac_of_peak": round(bf16_info["ipm"]["syrk_bf16_tflops"] / PEAK_BF16, 4),
                 "bf16_row_frac": round(bf16_info["ipm"]["bf16_row_frac"], 4),
+                "fp8_row_frac": round(bf16_info["ipm"]["fp8_row_frac"], 4),
                 "bf16_iters": bf16_info["ipm"]["syrk_bf16_iters"],
                 "t_capacitance_s": round(bf16_info["ipm"]["t_syrk"], 3),
                 "t_bf16_part_s": round(bf16_info["ipm"]["t_syrk_bf16"], 3),

@@ -47,6 +47,11 @@ typedef int tsvm_status;
  * products of the factor in FP64, the rest as TSVM_PREC_FP32 (factor stored in FP32,
  * FP64 capacitance). */
 #define TSVM_PREC_FP64 5
+/* Training / interior-point form: TSVM_PREC_BF16 with a third capacitance tier -- the rows
+ * with the smallest t_i = w_i |z_i|^2 (the non-support vectors late in a solve) on the
+ * FP8 e4m3 MFMA with per-column power-of-two scales and FP32 accumulation, under an error
+ * budget of 2^-5 (tsvm_weighted_syrk_dev: the same split). */
+#define TSVM_PREC_FP8 6
 
 /* Largest rank k of the factor (and width of Z in the interior-point solver): the
  * capacitance Cholesky and its solves keep a k x k FP64 matrix per workgroup.  The
@@ -148,12 +153,16 @@ tsvm_status tsvm_lra_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t 
  * TSVM_PREC_BF16 (hybrid): rows with small t_i = w_i |z_i|^2 run on BF16 MFMA (FP32
  * accumulate) while a bound on sqrt(sum t_i^2) over them stays within budget (<= 0: 4,
  * the interior-point default), the others on FP64 MFMA.  The BF16 part's error is
- * about 2^-9 times that bound in norm, against lambda_min(M) >= 1. */
+ * about 2^-9 times that bound in norm, against lambda_min(M) >= 1.  TSVM_PREC_FP8: the
+ * same with the smallest-t rows first taken into an FP8 e4m3 tier (per-column scales)
+ * while its error bound stays within 2^-5 (k > 128). */
 typedef struct {
   int64_t n_fp64_rows; /* rows computed in FP64 */
   double bf16_bound;   /* bound on sqrt(sum t_i^2) over the BF16 rows */
-  double t_bf16;       /* s in the BF16 part */
+  double t_bf16;       /* s in the BF16 (and FP8) part */
   double bf16_tflops;  /* n k (k+1) / t_bf16 */
+  int64_t n_fp8_rows;  /* TSVM_PREC_FP8: rows in the FP8 tier */
+  double fp8_bound;    /* TSVM_PREC_FP8: error bound of the FP8 tier */
 } tsvm_syrk_info;
 tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int64_t k,
                                    const double *w, int precision, double budget, double *M,
@@ -254,6 +263,7 @@ typedef struct {
   double syrk_bf16_tflops; /* n k (k+1) flops per BF16 part / its average time */
   double bf16_row_frac;    /* rows in the BF16 part, averaged over iterations */
   int syrk_bf16_iters;     /* iterations with a BF16 part */
+  double fp8_row_frac;     /* TSVM_PREC_FP8: rows in the FP8 tier, averaged over iterations */
 } tsvm_ipm_info;
 void tsvm_ipm_default_params(tsvm_ipm_params *p);
 /* Mehrotra predictor-corrector for the SVM dual with Q = Z Z^T, every O(n) and O(nk)

@@ -6,9 +6,10 @@ it.  This module is a thin ctypes binding of that C-ABI for Python callers; it n
 computes anything itself and raises if the library is missing.
 """
 from . import capi  # noqa: F401
-from .capi import (PREC_BF16, PREC_FP16, PREC_FP16R2, PREC_FP16X2, PREC_FP32, PREC_FP64, Context,
+from .capi import (PREC_BF16, PREC_FP16, PREC_FP16R2, PREC_FP16X2, PREC_FP32, PREC_FP64, PREC_FP8,
+                   Context,
                    Model, TsvmError, gaussrand_fill, library_path, load)
 
 __all__ = ["capi", "Context", "Model", "TsvmError", "gaussrand_fill", "load", "library_path",
            "PREC_FP32", "PREC_BF16", "PREC_FP16X2", "PREC_FP16",
-           "PREC_FP16R2", "PREC_FP64"]
+           "PREC_FP16R2", "PREC_FP64", "PREC_FP8"]

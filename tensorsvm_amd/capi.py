@@ -17,6 +17,7 @@ PREC_FP16 = 3    # kernel product only: kernel values and right-hand side in fp1
 PREC_FP16R2 = 4  # kernel product only: right-hand side as two fp16 terms
 PREC_FP64 = 5    # FP64 products (the reference of the other forms); as a training form:
                  # both factor products in FP64, the rest as the FP32 form
+PREC_FP8 = 6     # training / IPM form: BF16 with an FP8 e4m3 capacitance tier
 
 STOP_REF = 0
 STOP_GAP = 1
@@ -102,14 +103,15 @@ class IpmInfo(ctypes.Structure):
                 ("pobj", _D), ("dobj", _D), ("gap_rel", _D), ("condD", _D), ("y", _D),
                 ("t_total", _D), ("t_syrk", _D), ("syrk_tflops", _D), ("refine_res", _D),
                 ("t_syrk_bf16", _D), ("syrk_bf16_tflops", _D), ("bf16_row_frac", _D),
-                ("syrk_bf16_iters", _INT)]
+                ("syrk_bf16_iters", _INT), ("fp8_row_frac", _D)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class SyrkInfo(ctypes.Structure):
-    _fields_ = [("n_fp64_rows", _I64), ("bf16_bound", _D), ("t_bf16", _D), ("bf16_tflops", _D)]
+    _fields_ = [("n_fp64_rows", _I64), ("bf16_bound", _D), ("t_bf16", _D), ("bf16_tflops", _D),
+                ("n_fp8_rows", _I64), ("fp8_bound", _D)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -95,6 +95,9 @@ tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
 
 // ---- weighted Gram M = I + Z^T diag(w) Z of the capacitance (syrk.hip) -----------
 // Hybrid form: bulk rows (small w_i |z_i|^2) on BF16 MFMA, the rest on FP64 MFMA.
+// With the FP8 tier, an iteration whose smallest-t rows fit the FP8 budget and whose
+// remaining bulk is small replaces the BF16 pass by an FP8 pass over those rows and sends
+// the rest to the FP64 part (the cheaper of the two by a per-row cost model).
 struct HybridSyrk {
   int64_t n = 0, n_pad = 0;
   int kp = 0, kpt = 0;
@@ -103,17 +106,32 @@ struct HybridSyrk {
   float* wb = nullptr;     // FP32 weights of the bulk rows (0 for FP64 rows), n_pad
   int* rows = nullptr;     // indices of the FP64 rows
   int* ints = nullptr;     // split scratch (histograms, counts, offsets, cut, count)
+  // FP8 tier (fp8_eps > 0): the smallest-t rows on the FP8 MFMA with per-column scales
+  double fp8_eps = 0.0;    // error budget of the tier (0: no FP8 tier)
+  float* w8 = nullptr;     // sqrt(w_i) of the FP8 rows, 0 elsewhere (n_pad)
+  float* zmax = nullptr;   // max_i |z_ic| per column (kpt, set up once per solve)
+  float* sc8 = nullptr;    // per-column power-of-two scales of the iteration (kpt)
+  void* Z8 = nullptr;      // e4m3 of sc8_c sqrt(w_i) z_ic (n_pad x kpt, chunked as ZbT)
+  double r8 = 0.0;         // cost of the FP8 pass (packing included) / the BF16 pass
+  double r64 = 0.0;        // cost of one FP64 row / one BF16-pass row
 };
 // Workspace and the once-per-solve operands (ZbT, nz) for Z (n x kp FP32, device).
-tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridSyrk* hs);
+// fp8_eps > 0 (kp > 128 only) adds the FP8 tier with that error budget.
+tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridSyrk* hs,
+                              double fp8_eps = 0.0);
+// error budget of the FP8 tier of TSVM_PREC_FP8
+double hybrid_fp8_eps();
 // Splits the rows for weights w (device): slots[0] = sqrt(sum over bulk rows of t_i^2)
-// bound, slots[1] = number of FP64 rows (device doubles); fills rows and wb.
+// bound, slots[1] = number of FP64 rows, slots[2] = error bound of the FP8 tier,
+// slots[3] = its number of rows, 0 unless the iteration takes the FP8 pass (device
+// doubles); fills rows, wb (and w8, sc8).
 tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double budget,
                               double* slots);
-// M = I + FP64 part over the nh listed rows + BF16 part over the bulk.  ev (optional,
-// two events) brackets the BF16 part.
+// M = I + FP64 part over the nh listed rows + BF16 part over the bulk, or, when n8 > 0,
+// the FP8 part over the n8 FP8 rows instead.  ev (optional, two events) brackets the
+// BF16 / FP8 part.
 tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const double* w,
-                             int64_t nh, double* M, hipEvent_t* ev);
+                             int64_t nh, double* M, hipEvent_t* ev, int64_t n8 = 0);
 // flops of one BF16 part, SYRK convention (rows streamed x k (k + 1))
 double hybrid_syrk_flops(const HybridSyrk* hs);
 // tsvm_fnorm_res_dev (fnorm.hip): |K_Y|_F, |K_Y - U U^T|_F; U has row stride ldu

@@ -1091,7 +1091,8 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   // on FP64 (split per iteration so that the BF16 error bound stays within
   // IPM_SYRK_BUDGET; lambda_min(M) >= 1 makes that bound the refinement's contraction
   // factor up to a constant).  The plain form builds it in FP64 throughout.
-  const bool bf16 = prm->precision == TSVM_PREC_BF16 || prm->precision == TSVM_PREC_FP16X2;
+  const bool fp8 = prm->precision == TSVM_PREC_FP8;
+  const bool bf16 = prm->precision == TSVM_PREC_BF16 || prm->precision == TSVM_PREC_FP16X2 || fp8;
   void* p;
   auto ws = [&](const char* name, int64_t bytes, void** out) -> tsvm_status {
     TSVM_HIP(c, c->ws.get(name, (size_t)bytes, out));
@@ -1137,7 +1138,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     Z = Zp;
   }
   HybridSyrk hs;
-  if (bf16) TSVM_CHECK(hybrid_syrk_setup(c, Z, n, kp, &hs));
+  if (bf16) TSVM_CHECK(hybrid_syrk_setup(c, Z, n, kp, &hs, fp8 ? hybrid_fp8_eps() : 0.0));
 #undef WS
   const dim3 eg(EW_BLOCKS), eb(256);
   hipLaunchKernelGGL(k_labels_to_f64, eg, eb, 0, st, n, af, a);
@@ -1227,7 +1228,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   TSVM_HIP(c, evb0.create());
   TSVM_HIP(c, evb1.create());
   hipEvent_t evb[2] = {evb0, evb1};
-  double t_syrk = 0.0, t_bf16 = 0.0, row_frac_sum = 0.0;
+  double t_syrk = 0.0, t_bf16 = 0.0, row_frac_sum = 0.0, fp8_frac_sum = 0.0;
   int n_syrk = 0, n_bf16 = 0;
   bool pending_syrk = false, pending_bf16 = false;
   double hsc[IPM_NSLOT];
@@ -1331,9 +1332,11 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_HIP(c, hipEventRecord(ev0, st));
     if (bf16) {
       const int64_t nh = (int64_t)hsc[SL_HSN];
-      TSVM_CHECK(hybrid_syrk_form(c, &hs, Z, winv, nh, M, evb));
+      const int64_t n8 = fp8 ? (int64_t)hsc[SL_HS8N] : 0;
+      TSVM_CHECK(hybrid_syrk_form(c, &hs, Z, winv, nh, M, evb, n8));
       pending_bf16 = nh < n;
       row_frac_sum += 1.0 - (double)nh / (double)n;
+      fp8_frac_sum += (double)n8 / (double)n;
     } else {
       TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, n, winv, true, M));
       hipLaunchKernelGGL(k_add_identity, dim3((unsigned)ceil_div(kp, 256)), dim3(256), 0, st, M, (int64_t)kp);
@@ -1387,6 +1390,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   inf.syrk_bf16_iters = n_bf16;
   if (t_bf16 > 0) inf.syrk_bf16_tflops = hybrid_syrk_flops(&hs) * n_bf16 / t_bf16 * 1e-12;
   if (n_syrk > 0) inf.bf16_row_frac = row_frac_sum / n_syrk;
+  if (n_syrk > 0) inf.fp8_row_frac = fp8_frac_sum / n_syrk;
   if (info) *info = inf;
   return TSVM_OK;
 }

@@ -35,6 +35,8 @@ enum IpmSlot {
   SL_BRHS,       // |r5|^2
   SL_HSB,        // hybrid capacitance: error bound sqrt(sum t_i^2) of the BF16 rows
   SL_HSN,        // hybrid capacitance: number of FP64 rows (must follow SL_HSB)
+  SL_HS8B,       // hybrid capacitance: error bound of the FP8 rows (must follow SL_HSN)
+  SL_HS8N,       // hybrid capacitance: number of FP8 rows (must follow SL_HS8B)
   IPM_NSLOT = 32
 };
 

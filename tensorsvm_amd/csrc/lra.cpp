@@ -131,8 +131,9 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   if (n <= 0 || d <= 0 || k <= 0 || k > n)
     return fail(c, TSVM_ERR_ARG, "lra: need 0 < k <= n (n=%ld k=%ld)", (long)n, (long)k);
   if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16 &&
-      precision != TSVM_PREC_FP16X2 && precision != TSVM_PREC_FP64)
+      precision != TSVM_PREC_FP16X2 && precision != TSVM_PREC_FP64 && precision != TSVM_PREC_FP8)
     return fail(c, TSVM_ERR_ARG, "lra: unknown precision %d", precision);
+  if (precision == TSVM_PREC_FP8) precision = TSVM_PREC_BF16;  // its factor is the BF16 form's
   hipStream_t s = c->stream;
   tsvm_lra_info inf;
   memset(&inf, 0, sizeof(inf));

@@ -29,13 +29,22 @@
 namespace tsvm {
 
 constexpr int HS_BLOCKS = 1024;  // blocks of the split passes (each a contiguous row range)
+// error budget of the FP8 tier (TSVM_PREC_FP8): the BF16 tier's 2^-9 x its budget of 16
+constexpr double HYBRID_FP8_EPS = 1.0 / 32.0;
+// per-row costs of the FP8 tier's choice, in units of one BF16-pass row (measured on
+// MI355X at n = 1.6M / 4M, kpt = 512 / 1024): the FP8 SYRK runs at half the BF16 pass's
+// time, the e4m3 packing (3 bytes per entry of HBM traffic) adds a share that falls as
+// 1 / kpt against the k^2 work (0.61 at 512, 0.35 at 1024); one FP64 row through the
+// index list costs about 20 BF16-pass rows
+__host__ inline double hybrid_fp8_cost(int kpt) { return 0.52 + 0.35 * 1024.0 / kpt; }
+constexpr double HYBRID_F64_COST = 20.0;
 constexpr int HS_BINS = 128;     // bins of t_i by binary exponent: bin = floor(log2 t) + 64
 constexpr int SY_BK = 64;        // rows (the GEMM's K) per SYRK stage
 constexpr int SY_LS = SY_BK + 8;  // LDS row stride (bf16) of a staged panel
 
 __device__ inline int hs_bin(double t) {
-  if (!(t > 0.0)) return 0;  // zero, negative and NaN weights
-  if (!(t < 1e300)) return HS_BINS - 1;
+  if (t == 0.0) return 0;
+  if (!(t > 0.0 && t < 1e300)) return HS_BINS - 1;  // negative, NaN, huge: the FP64 part
   int e = ilogb(t) + 64;
   return e < 0 ? 0 : (e > HS_BINS - 1 ? HS_BINS - 1 : e);
 }
@@ -91,12 +100,20 @@ __global__ __launch_bounds__(256) void hs_hist_kernel(const double* __restrict__
 
 // one block of 8 HS_BINS threads: total counts per bin (8 groups of partials, summed in
 // a fixed order), then the cutoff bin: the bulk is every bin below `cut`, the largest
-// prefix whose bound sqrt(sum cnt_b 4^(b-63)) stays within the budget.  out: cut (int),
-// slots[0] = bound of the bulk
+// prefix whose bound sqrt(sum cnt_b 4^(b-63)) stays within the budget.  out: cut[0] =
+// cut, cut[2] = cut8 (0: no FP8 pass), slots[0] = bound of the bulk, slots[2] = bound of
+// the FP8 rows, slots[3] = their number.
+// FP8 tier (eps8 > 0): the smallest-t bins while 2^-3 sqrt(sum t^2) (three e4m3 roundings
+// per term, 2^-4 each) plus the flush bound n8 t_max 2^-17 (an entry below the subnormal
+// step of its column's scale, at most 2^-16 of the largest, becomes 0) stays within eps8.
+// The iteration takes the FP8 pass (over bins < cut8, the rest in FP64) when the per-row
+// cost model says it is cheaper than the BF16 pass: r8 n + r64 nb < n, nb the bulk rows
+// that would move to FP64 (costs in units of one BF16-pass row).  Otherwise the split is
+// exactly the BF16 form's.
 constexpr int HS_CUT_GROUPS = 8;
 __global__ __launch_bounds__(HS_CUT_GROUPS * HS_BINS) void hs_cutoff_kernel(
-    const int* __restrict__ part, int nblocks, double budget, int* __restrict__ cut_out,
-    double* __restrict__ slots) {
+    const int* __restrict__ part, int nblocks, double budget, double eps8, double r8, double r64,
+    int* __restrict__ cut_out, double* __restrict__ slots) {
   __shared__ long long grp[HS_CUT_GROUPS][HS_BINS];
   __shared__ long long cnt[HS_BINS];
   const int b = threadIdx.x % HS_BINS, g = threadIdx.x / HS_BINS;
@@ -111,21 +128,48 @@ __global__ __launch_bounds__(HS_CUT_GROUPS * HS_BINS) void hs_cutoff_kernel(
     cnt[b] = t;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double cum = 0.0;
-    int cut = HS_BINS;
-    for (int e = 0; e < HS_BINS; e++) {
-      if (cnt[e] == 0) continue;
-      double add = (double)cnt[e] * ldexp(1.0, 2 * (e - 63));
-      if (sqrt(cum + add) > budget) {
-        cut = e;
-        break;
-      }
-      cum += add;
+  if (threadIdx.x != 0) return;
+  double cum = 0.0;
+  int cut = HS_BINS;
+  for (int e = 0; e < HS_BINS; e++) {
+    if (cnt[e] == 0) continue;
+    const double add = (double)cnt[e] * ldexp(1.0, 2 * (e - 63));
+    if (sqrt(cum + add) > budget) {
+      cut = e;
+      break;
     }
-    *cut_out = cut;
-    slots[0] = sqrt(cum);
+    cum += add;
   }
+  int cut8 = 0;
+  double cum8 = 0.0, n8 = 0.0;
+  if (eps8 > 0.0) {
+    for (int e = 0; e < cut; e++) {
+      if (cnt[e] == 0) continue;  // cut8 stays one above the last non-empty bin taken
+      const double add = (double)cnt[e] * ldexp(1.0, 2 * (e - 63));
+      const double err = ldexp(sqrt(cum8 + add), -3) + (n8 + cnt[e]) * ldexp(1.0, e - 63 - 17);
+      if (err > eps8) break;
+      cum8 += add;
+      n8 += cnt[e];
+      cut8 = e + 1;
+    }
+    double ntot = 0.0, nb = 0.0;
+    for (int e = 0; e < HS_BINS; e++) ntot += (double)cnt[e];
+    for (int e = cut8; e < cut; e++) nb += (double)cnt[e];
+    if (!(n8 > 0.0 && r8 * ntot + r64 * nb < ntot)) {
+      cut8 = 0;
+      cum8 = 0.0;
+      n8 = 0.0;
+    }
+  }
+  if (n8 > 0.0) {  // FP8 pass: no BF16 rows, FP64 from cut8 up
+    cut = cut8;
+    cum = 0.0;
+  }
+  cut_out[0] = cut;
+  cut_out[2] = cut8;
+  slots[0] = sqrt(cum);
+  slots[2] = n8 > 0.0 ? ldexp(sqrt(cum8), -3) + n8 * ldexp(1.0, cut8 - 1 - 63 - 17) : 0.0;
+  slots[3] = n8;
 }
 
 // per-block counts of the FP64 rows (bin >= cut)
@@ -171,15 +215,21 @@ __global__ __launch_bounds__(HS_BLOCKS) void hs_scan_kernel(const int* __restric
 
 // rows of the FP64 part in increasing order; wb = FP32 weight of the bulk rows, 0 for
 // the FP64 rows and the padding
+// FP8 rows (bin < cut[2], when w8 is given): wb = 0, w8 = sqrt(w) (FP32) and
+// w8max (bits of a non-negative float) = their maximum
 __global__ __launch_bounds__(256) void hs_scatter_kernel(const double* __restrict__ w,
                                                          const double* __restrict__ nz, int64_t n,
                                                          int64_t n_pad, int64_t rpb,
                                                          const int* __restrict__ cut,
                                                          const int* __restrict__ off,
                                                          int* __restrict__ rows,
-                                                         float* __restrict__ wb) {
+                                                         float* __restrict__ wb,
+                                                         float* __restrict__ w8,
+                                                         unsigned int* __restrict__ w8max) {
   __shared__ int wsum[4];
-  const int cb = *cut;
+  const int cb = cut[0];
+  const int c8 = w8 ? cut[2] : 0;
+  float mx = 0.0f;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.x * rpb;
   int64_t r1 = r0 + rpb < n_pad ? r0 + rpb : n_pad;
@@ -187,13 +237,22 @@ __global__ __launch_bounds__(256) void hs_scatter_kernel(const double* __restric
   for (int64_t c0 = r0; c0 < r1; c0 += 256) {
     const int64_t i = c0 + threadIdx.x;
     bool hi = false;
-    float wf = 0.0f;
+    float wf = 0.0f, w8f = 0.0f;
     if (i < r1 && i < n) {
       const double wi = w[i];
-      hi = hs_bin(wi * nz[i]) >= cb;
-      wf = hi ? 0.0f : (float)wi;
+      const int bin = hs_bin(wi * nz[i]);
+      hi = bin >= cb;
+      if (bin < c8) {  // wi >= 0 here: negative and NaN weights are in the last bin
+        w8f = (float)sqrt(wi);
+        mx = fmaxf(mx, w8f);
+      } else if (!hi) {
+        wf = (float)wi;
+      }
     }
-    if (i < r1) wb[i] = wf;
+    if (i < r1) {
+      wb[i] = wf;
+      if (w8) w8[i] = w8f;
+    }
     const unsigned long long m = __ballot(hi);
     const int below = __popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) wsum[wave] = __popcll(m);
@@ -205,6 +264,43 @@ __global__ __launch_bounds__(256) void hs_scatter_kernel(const double* __restric
     base += tot;
     __syncthreads();
   }
+  if (w8) {
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0) atomicMax(w8max, __float_as_uint(mx));
+  }
+}
+
+// per-column power-of-two scales of the FP8 tier: |y_ic| = sqrt(w_i) |z_ic| is below
+// min(sqrt(t_max), w8max zmax_c) (t_max = 2^(cut8 - 64) bounds t_i = w_i |z_i|^2 in the
+// tier); sc8[c] = 2^e with that bound times sc8[c] in [128, 256) (e4m3 holds 448)
+__global__ void fp8_scales_kernel(const float* __restrict__ zmax, int kpt,
+                                  const int* __restrict__ cut, const unsigned int* __restrict__ w8max,
+                                  float* __restrict__ sc8) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= kpt) return;
+  const double tb = sqrt(ldexp(1.0, cut[2] - 64));
+  const double cb = (double)__uint_as_float(*w8max) * (double)zmax[col];
+  const double bnd = cb < tb ? cb : tb;
+  int e = 0;
+  if (bnd > 0.0) {
+    int ex;
+    (void)frexp(bnd, &ex);  // bnd = f 2^ex, f in [0.5, 1)
+    e = 8 - ex;
+    e = e > 100 ? 100 : (e < -100 ? -100 : e);
+  }
+  sc8[col] = ldexpf(1.0f, e);
+}
+
+// zmax[c] = max_i |z_ic| (the FP8 tier's static column bounds)
+__global__ void colmax_kernel(const float* __restrict__ Z, int64_t n, int kp, int kpt,
+                              unsigned int* __restrict__ zmax_bits) {
+  const int col = blockIdx.y * 64 + (threadIdx.x & 63);
+  if (col >= kpt) return;
+  float mx = 0.0f;
+  if (col < kp)
+    for (int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += (int64_t)gridDim.x * 4)
+      mx = fmaxf(mx, fabsf(Z[i * kp + col]));
+  atomicMax(zmax_bits + col, __float_as_uint(mx));
 }
 
 // scales 8 bf16 by 8 FP32 factors, one rounding each
@@ -479,16 +575,170 @@ __global__ __launch_bounds__(256, 1) void syrk_bf16_256_kernel(const __bf16* __r
 // M (kp x kp FP64) += sum over splits of the BF16 partials, in split order.  Threads
 // read lower entries (a >= b, coalesced along b) and add each sum to both mirrored
 // entries, so M stays exactly symmetric.
+// sc (FP8 tier, may be null): the per-column scales of the operands; the sum is divided
+// by sc_a sc_b
 __global__ void syrk_accum_kernel(const float* __restrict__ part, int nsplit, int kpt, int kp,
-                                  double* __restrict__ M) {
+                                  double* __restrict__ M, const float* __restrict__ sc = nullptr) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= (int64_t)kp * kp) return;
   const int a = (int)(idx / kp), b = (int)(idx % kp);
   if (b > a) return;
   double s = 0.0;
   for (int t = 0; t < nsplit; t++) s += (double)part[(int64_t)t * kpt * kpt + (int64_t)a * kpt + b];
+  if (sc) s /= (double)sc[a] * (double)sc[b];
   M[idx] += s;
   if (a != b) M[(int64_t)b * kp + a] += s;
+}
+
+// FP8 pass (kpt a multiple of 256): Z8 (chunked like ZbT, one byte per entry) holds
+// y_ic = e4m3(sc8_c sqrt(w_i) z_ic), w8 = sqrt(w_i) on the FP8 rows and 0 elsewhere, so
+// sum_i y_i y_i^T / (sc8_a sc8_b) is the FP8 rows' part of Z^T diag(w) Z.  sc8_c is a power
+// of two: the products are exact, one e4m3 rounding (round to nearest even) per entry.
+// Block: one 64-row chunk x 256 columns; a thread converts 8 rows of a column.
+__device__ inline unsigned int pack_fp8x4(float a, float b, float c, float d) {
+  int v = 0;
+  v = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, v, false);
+  v = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, v, true);
+  return (unsigned int)v;
+}
+typedef __attribute__((ext_vector_type(8))) int i32x8_t;
+typedef __attribute__((ext_vector_type(2))) unsigned int u32x2_t;
+
+__global__ __launch_bounds__(256) void pack_fp8_kernel(const __bf16* __restrict__ ZbT, int kpt,
+                                                       const float* __restrict__ w8,
+                                                       const float* __restrict__ sc8,
+                                                       unsigned char* __restrict__ Z8) {
+  const int64_t ch = blockIdx.x;
+  const int rg = threadIdx.x & 7, cl = threadIdx.x >> 3;
+  const f32x4_t w0 = *(const f32x4_t*)(w8 + ch * SY_BK + rg * 8);
+  const f32x4_t w1 = *(const f32x4_t*)(w8 + ch * SY_BK + rg * 8 + 4);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int col = blockIdx.y * 256 + cl + 32 * q;
+    const int64_t off = (ch * kpt + col) * SY_BK + rg * 8;
+    const bf16x8_t zb = __builtin_bit_cast(bf16x8_t, *(const u32x4_t*)(ZbT + off));
+    const float sc = sc8[col];
+    float y[8];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      y[t] = (float)zb[t] * w0[t] * sc;
+      y[t + 4] = (float)zb[t + 4] * w1[t] * sc;
+    }
+    u32x2_t pk;
+    pk[0] = pack_fp8x4(y[0], y[1], y[2], y[3]);
+    pk[1] = pack_fp8x4(y[4], y[5], y[6], y[7]);
+    *(u32x2_t*)(Z8 + off) = pk;
+  }
+}
+
+// FP8 SYRK (kpt a multiple of 256): part[split] (lower 256 x 256 tiles) = sum over the
+// split's rows of y_i y_i^T from Z8.  Same tiling as syrk_bf16_256_kernel (4 waves, each a
+// 128 x 128 quarter, 256 accumulators); a 64-row chunk is one K step of
+// v_mfma_scale_f32_32x32x64_f8f6f4 with unit block scales (the per-column scales are
+// divided out when the partials are summed).  LDS rows of 80 bytes per column (64 K bytes
+// + padding: 16 lanes' 16-byte reads at a stride of 20 banks cover all 64 banks); a lane's
+// fragment is the 32 K bytes 32 (lane / 32) .. + 32 of its column.  Double-buffered LDS
+// with the next chunk's loads in registers, one barrier per chunk; on a diagonal tile one
+// panel is staged.
+constexpr int S8_LS = 80;  // LDS bytes per staged column
+
+__global__ __launch_bounds__(256, 1) void syrk_fp8_256_kernel(const unsigned char* __restrict__ Z8,
+                                                              int64_t n_pad, int kpt, int ntile,
+                                                              int nsplit, int64_t stages_per_split,
+                                                              float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) unsigned char sm[2][2][256 * S8_LS];
+  const int nlow = ntile * (ntile + 1) / 2;
+  const int bid = blockIdx.x;
+  const int xcd = bid % 8, grp = bid / 8;
+  const int tile = grp % nlow;
+  const int split = (grp / nlow) * 8 + xcd;
+  if (split >= nsplit) return;
+  int ta = 0;
+  while ((ta + 1) * (ta + 2) / 2 <= tile) ta++;
+  const int tb = tile - ta * (ta + 1) / 2;
+  const bool diag = ta == tb;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, h = lane >> 5;
+  const int wa = (wave >> 1) * 128, wbo = (wave & 1) * 128;
+  const int64_t s0 = (int64_t)split * stages_per_split;
+  const int64_t nst_all = n_pad / SY_BK;
+  const int64_t s1 = s0 + stages_per_split < nst_all ? s0 + stages_per_split : nst_all;
+  const int64_t nst = s1 > s0 ? s1 - s0 : 0;
+  f32x16_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
+  // staging map: 16-byte piece idx = tid + 256 v of a panel is column idx / 4, K bytes
+  // 16 (idx % 4) .. + 16
+  u32x4_t v[2][4];
+  auto gload = [&](int64_t st) {
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      const unsigned char* pan = Z8 + ((s0 + st) * kpt + (p && !diag ? tb : ta) * 256) * SY_BK;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int idx = tid + 256 * q;
+        if (p == 0 || !diag) v[p][q] = *(const u32x4_t*)(pan + (idx >> 2) * SY_BK + (idx & 3) * 16);
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int idx = tid + 256 * q;
+        if (p == 0 || !diag) *(u32x4_t*)&sm[buf][p][(idx >> 2) * S8_LS + (idx & 3) * 16] = v[p][q];
+      }
+  };
+  if (nst > 0) {
+    gload(0);
+    sstore(0);
+    __syncthreads();
+  }
+  const int pb_sel = diag ? 0 : 1;
+  for (int64_t st = 0; st < nst; st++) {
+    const int buf = (int)(st & 1);
+    gload(st + 1 < nst ? st + 1 : st);
+    i32x8_t fa[4], fb[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const unsigned char* pa = &sm[buf][0][(wa + 32 * i + l32) * S8_LS + 32 * h];
+      const unsigned char* pb = &sm[buf][pb_sel][(wbo + 32 * i + l32) * S8_LS + 32 * h];
+      const u32x4_t a0 = *(const u32x4_t*)pa, a1 = *(const u32x4_t*)(pa + 16);
+      const u32x4_t b0 = *(const u32x4_t*)pb, b1 = *(const u32x4_t*)(pb + 16);
+      fa[i] = (i32x8_t){(int)a0[0], (int)a0[1], (int)a0[2], (int)a0[3],
+                        (int)a1[0], (int)a1[1], (int)a1[2], (int)a1[3]};
+      fb[i] = (i32x8_t){(int)b0[0], (int)b0[1], (int)b0[2], (int)b0[3],
+                        (int)b1[0], (int)b1[1], (int)b1[2], (int)b1[3]};
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0,
+                                                                   0, 127, 0, 127);
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+    sstore(buf ^ 1);
+    __syncthreads();
+  }
+  if (diag && wbo > wa) return;  // strictly above the diagonal
+  // C layout of the 32x32 MFMA: register q holds row 8(q/4) + 4h + q%4, column l32
+  float* P = part + (int64_t)split * kpt * kpt;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int a = ta * 256 + wa + 32 * i + 8 * (q / 4) + 4 * h + (q % 4);
+        const int bcol = tb * 256 + wbo + 32 * j + l32;
+        P[(int64_t)a * kpt + bcol] = acc[i][j][q];
+      }
 }
 
 __global__ void add_identity_kernel(double* __restrict__ M, int64_t k) {
@@ -498,10 +748,17 @@ __global__ void add_identity_kernel(double* __restrict__ M, int64_t k) {
 
 // ---- host side ----------------------------------------------------------------------
 
-tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridSyrk* hs) {
+__global__ void zero_u32s_kernel(unsigned int* __restrict__ p, int cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) p[i] = 0u;
+}
+
+tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridSyrk* hs,
+                              double fp8_eps) {
   hs->n = n;
   hs->kp = kp;
   hs->kpt = kp <= 128 ? 128 : (int)round_up(kp, 256);
+  hs->fp8_eps = hs->kpt >= 256 ? fp8_eps : 0.0;
   hs->n_pad = round_up(n > 0 ? n : 1, SY_BK);
   void* p;
   TSVM_HIP(c, c->ws.get("hs_ZbT", (size_t)hs->kpt * hs->n_pad * 2, &p));
@@ -515,6 +772,22 @@ tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridS
   TSVM_HIP(c, c->ws.get("hs_ints", (size_t)(HS_BLOCKS * (HS_BINS + 2) + 8) * 4, &p));
   hs->ints = (int*)p;
   hipStream_t s = c->stream;
+  if (hs->fp8_eps > 0.0) {
+    TSVM_HIP(c, c->ws.get("hs_w8", (size_t)hs->n_pad * 4, &p));
+    hs->w8 = (float*)p;
+    TSVM_HIP(c, c->ws.get("hs_zmax", (size_t)hs->kpt * 4, &p));
+    hs->zmax = (float*)p;
+    TSVM_HIP(c, c->ws.get("hs_sc8", (size_t)hs->kpt * 4, &p));
+    hs->sc8 = (float*)p;
+    TSVM_HIP(c, c->ws.get("hs_Z8", (size_t)hs->kpt * hs->n_pad, &p));
+    hs->Z8 = p;
+    hs->r8 = hybrid_fp8_cost(hs->kpt);
+    hs->r64 = HYBRID_F64_COST;
+    hipLaunchKernelGGL(zero_u32s_kernel, dim3((unsigned)ceil_div(hs->kpt, 256)), dim3(256), 0, s,
+                       (unsigned int*)hs->zmax, hs->kpt);
+    hipLaunchKernelGGL(colmax_kernel, dim3(1024, (unsigned)(hs->kpt / 64)), dim3(256), 0, s, Z, n,
+                       kp, hs->kpt, (unsigned int*)hs->zmax);
+  }
   dim3 g((unsigned)(hs->n_pad / 32), (unsigned)(hs->kpt / 32));
   hipLaunchKernelGGL(make_zbt_kernel, g, dim3(256), 0, s, Z, n, kp, hs->kpt, hs->n_pad,
                      (__bf16*)hs->ZbT);
@@ -535,19 +808,25 @@ tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double bu
   const int64_t rpb = round_up(ceil_div(hs->n_pad, HS_BLOCKS), 256);
   const int nb = (int)ceil_div(hs->n_pad, rpb);
   hipLaunchKernelGGL(hs_hist_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb, part);
-  hipLaunchKernelGGL(hs_cutoff_kernel, dim3(1), dim3(HS_CUT_GROUPS * HS_BINS), 0, s, (const int*)part, nb, budget,
-                     cut, slots);
+  hipLaunchKernelGGL(hs_cutoff_kernel, dim3(1), dim3(HS_CUT_GROUPS * HS_BINS), 0, s,
+                     (const int*)part, nb, budget, hs->fp8_eps, hs->r8, hs->r64, cut, slots);
   hipLaunchKernelGGL(hs_count_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb,
                      (const int*)cut, cnt);
   hipLaunchKernelGGL(hs_scan_kernel, dim3(1), dim3(HS_BLOCKS), 0, s, (const int*)cnt, nb, off, slots, nh);
+  unsigned int* w8max = (unsigned int*)(cut + 3);  // cut[0] cut, cut[1] FP64 rows, cut[2] cut8
+  if (hs->w8) hipLaunchKernelGGL(zero_u32s_kernel, dim3(1), dim3(64), 0, s, w8max, 1);
   hipLaunchKernelGGL(hs_scatter_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, hs->n_pad, rpb,
-                     (const int*)cut, (const int*)off, hs->rows, hs->wb);
+                     (const int*)cut, (const int*)off, hs->rows, hs->wb, hs->w8, w8max);
+  if (hs->w8)
+    hipLaunchKernelGGL(fp8_scales_kernel, dim3((unsigned)ceil_div(hs->kpt, 256)), dim3(256), 0, s,
+                       (const float*)hs->zmax, hs->kpt, (const int*)cut,
+                       (const unsigned int*)w8max, hs->sc8);
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
 }
 
 tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const double* w,
-                             int64_t nh, double* M, hipEvent_t* ev_bf16) {
+                             int64_t nh, double* M, hipEvent_t* ev_bf16, int64_t n8) {
   hipStream_t s = c->stream;
   const int64_t n = hs->n;
   const int kp = hs->kp, kpt = hs->kpt;
@@ -559,9 +838,10 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
   hipLaunchKernelGGL(add_identity_kernel, dim3((unsigned)ceil_div(kp, 256)), dim3(256), 0, s, M,
                      (int64_t)kp);
   if (ev_bf16) TSVM_HIP(c, hipEventRecord(ev_bf16[0], s));
+  const bool fp8 = n8 > 0 && hs->Z8 && kpt >= 256;
   if (nh < n) {
     // splits: as many per XCD as fit its CUs with every tile of a split resident at once
-    // (the 128 kernel runs two blocks per CU, the 256 kernel one)
+    // (the 128 kernel runs two blocks per CU, the 256 kernels one)
     const bool big = kpt >= 256;
     const int ntile = kpt / (big ? 256 : 128);
     const int nlow = ntile * (ntile + 1) / 2;
@@ -572,21 +852,37 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
     const int64_t sps = ceil_div(nst, nsplit);
     void* part;
     TSVM_HIP(c, c->ws.get("hs_part", (size_t)nsplit * kpt * kpt * 4, &part));
-    if (big)
-      hipLaunchKernelGGL(syrk_bf16_256_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
-                         (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile,
-                         nsplit, sps, (float*)part);
-    else
-      hipLaunchKernelGGL(syrk_bf16_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
-                         (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile,
-                         nsplit, sps, (float*)part);
-    hipLaunchKernelGGL(syrk_accum_kernel, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)), dim3(256),
-                       0, s, (const float*)part, nsplit, kpt, kp, M);
+    if (fp8) {
+      // the FP8 pass over the FP8 rows (the others have w8 = 0) in place of the BF16 pass
+      hipLaunchKernelGGL(pack_fp8_kernel, dim3((unsigned)nst, (unsigned)(kpt / 256)), dim3(256), 0, s,
+                         (const __bf16*)hs->ZbT, kpt, (const float*)hs->w8,
+                         (const float*)hs->sc8, (unsigned char*)hs->Z8);
+      hipLaunchKernelGGL(syrk_fp8_256_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
+                         (const unsigned char*)hs->Z8, hs->n_pad, kpt, ntile, nsplit, sps,
+                         (float*)part);
+      hipLaunchKernelGGL(syrk_accum_kernel, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)),
+                         dim3(256), 0, s, (const float*)part, nsplit, kpt, kp, M,
+                         (const float*)hs->sc8);
+    } else {
+      if (big)
+        hipLaunchKernelGGL(syrk_bf16_256_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
+                           (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile,
+                           nsplit, sps, (float*)part);
+      else
+        hipLaunchKernelGGL(syrk_bf16_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
+                           (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile,
+                           nsplit, sps, (float*)part);
+      hipLaunchKernelGGL(syrk_accum_kernel, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)),
+                         dim3(256), 0, s, (const float*)part, nsplit, kpt, kp, M,
+                         (const float*)nullptr);
+    }
   }
   if (ev_bf16) TSVM_HIP(c, hipEventRecord(ev_bf16[1], s));
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
 }
+
+double hybrid_fp8_eps() { return HYBRID_FP8_EPS; }
 
 double hybrid_syrk_flops(const HybridSyrk* hs) {
   // rows streamed by the BF16 kernel times the SYRK count k (k + 1)
@@ -597,7 +893,7 @@ tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, cons
                               int precision, double budget, double* M, tsvm_syrk_info* info) {
   if (n < 0 || k <= 0 || k > 4096 || !Z || !w || !M)
     return fail(c, TSVM_ERR_ARG, "weighted_syrk: bad arguments (n=%ld k=%ld)", (long)n, (long)k);
-  if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16)
+  if (precision != TSVM_PREC_FP32 && precision != TSVM_PREC_BF16 && precision != TSVM_PREC_FP8)
     return fail(c, TSVM_ERR_ARG, "weighted_syrk: unknown precision %d", precision);
   hipStream_t s = c->stream;
   tsvm_syrk_info inf;
@@ -605,32 +901,38 @@ tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, cons
   inf.bf16_bound = 0.0;
   inf.t_bf16 = 0.0;
   inf.bf16_tflops = 0.0;
+  inf.n_fp8_rows = 0;
+  inf.fp8_bound = 0.0;
   if (precision == TSVM_PREC_FP32 || n == 0) {
     TSVM_CHECK(gemm_tn_f64(c, Z, k, k, Z, k, k, n, w, true, M));
     hipLaunchKernelGGL(add_identity_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, s, M, k);
     TSVM_HIP(c, hipGetLastError());
   } else {
     HybridSyrk hs;
-    TSVM_CHECK(hybrid_syrk_setup(c, Z, n, (int)k, &hs));
+    TSVM_CHECK(hybrid_syrk_setup(c, Z, n, (int)k, &hs,
+                                 precision == TSVM_PREC_FP8 ? HYBRID_FP8_EPS : 0.0));
     void* p;
-    TSVM_HIP(c, c->ws.get("hs_slots", 16, &p));
+    TSVM_HIP(c, c->ws.get("hs_slots", 32, &p));
     double* slots = (double*)p;
     TSVM_CHECK(hybrid_syrk_split(c, &hs, w, budget > 0 ? budget : 4.0, slots));
-    double hsl[2];
-    TSVM_HIP(c, hipMemcpyAsync(hsl, slots, 16, hipMemcpyDeviceToHost, s));
+    double hsl[4] = {0, 0, 0, 0};
+    TSVM_HIP(c, hipMemcpyAsync(hsl, slots, 32, hipMemcpyDeviceToHost, s));
     TSVM_HIP(c, hipStreamSynchronize(s));
     Event ev0, ev1;
     TSVM_HIP(c, ev0.create());
     TSVM_HIP(c, ev1.create());
     hipEvent_t ev[2] = {ev0, ev1};
     const int64_t nh = (int64_t)hsl[1];
-    tsvm_status st = hybrid_syrk_form(c, &hs, Z, w, nh, M, ev);
+    const int64_t n8 = hs.fp8_eps > 0.0 ? (int64_t)hsl[3] : 0;
+    tsvm_status st = hybrid_syrk_form(c, &hs, Z, w, nh, M, ev, n8);
     if (st == TSVM_OK) {
       TSVM_HIP(c, hipStreamSynchronize(s));
       float ms = 0.f;
       TSVM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
       inf.n_fp64_rows = nh;
       inf.bf16_bound = hsl[0];
+      inf.n_fp8_rows = n8;
+      inf.fp8_bound = hsl[2];
       if (nh < n) {
         inf.t_bf16 = ms * 1e-3;
         if (ms > 0) inf.bf16_tflops = hybrid_syrk_flops(&hs) / (ms * 1e-3) * 1e-12;

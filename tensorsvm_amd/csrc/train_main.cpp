@@ -5,7 +5,7 @@
 // Kept options: -k rank, -c C, -g gamma, -t 0|2, -s 0|1, -N n, -scale256, -analysis,
 // -test file, -tensorcore (= -precision bf16), positional training file and model file
 // (default: training file + ".model").  Defaults as the reference's globals (66-69):
-// K=50, C=1, g=0, t=0.  Added: -precision fp32|bf16|split, -stop ref|gap, -tol x, -refine n,
+// K=50, C=1, g=0, t=0.  Added: -precision fp32|bf16|split|fp8, -stop ref|gap, -tol x, -refine n,
 // -maxit n, -q (quiet), -json file (machine-readable summary of the run).
 #include <stdio.h>
 #include <stdlib.h>
@@ -36,8 +36,9 @@ void help_msg() {
       "-test file : held-out LIBSVM file; prints the prediction accuracy\n"
       "-scale256 : divide every feature value by 256\n"
       "-analysis : compute fnorm(K-U*U'), the error of the low rank approximation (RBF)\n"
-      "-precision fp32|bf16|split : arithmetic of the kernel products (default fp32;\n"
-      "    split: bf16 with the factor's sketch in two fp16 terms)\n"
+      "-precision fp32|bf16|split|fp8 : arithmetic of the kernel products (default fp32;\n"
+      "    split: bf16 with the factor's sketch in two fp16 terms; fp8: bf16 with an fp8\n"
+      "    tier in the solver's capacitance)\n"
       "-tensorcore : same as -precision bf16\n"
       "-stop ref|gap : IPM stop rule (ref: mu,|r_x|,|r_y| < 1e-7; gap: relative gap <= -tol)\n"
       "-tol x : tolerance of -stop gap (default 1e-5)\n"
@@ -97,6 +98,7 @@ int main(int argc, char* argv[]) {
       if (p == "fp32") prm.precision = TSVM_PREC_FP32;
       else if (p == "bf16") prm.precision = TSVM_PREC_BF16;
       else if (p == "split") prm.precision = TSVM_PREC_FP16X2;
+      else if (p == "fp8") prm.precision = TSVM_PREC_FP8;
       else {
         fprintf(stderr, "unknown precision %s\n", p.c_str());
         exit(1);
@@ -233,7 +235,7 @@ int main(int argc, char* argv[]) {
               "\"pobj\": %.12e, \"dobj\": %.12e, \"gap_rel\": %.6e, \"b\": %.12e, \"nsv\": %ld, "
               "\"nfree\": %ld, \"accuracy\": %.6f}\n",
               (long)n, (long)d, (long)prm.k, prm.C, prm.gamma,
-              prm.precision == TSVM_PREC_BF16 ? "bf16" : (prm.precision == TSVM_PREC_FP16X2 ? "split" : "fp32"),
+              prm.precision == TSVM_PREC_BF16 ? "bf16" : (prm.precision == TSVM_PREC_FP16X2 ? "split" : (prm.precision == TSVM_PREC_FP8 ? "fp8" : "fp32")),
               t_train, info.lra.t_total,
               info.ipm.t_total, info.ipm.iters, info.ipm.reason, info.ipm.pobj, info.ipm.dobj,
               info.ipm.gap_rel, info.b, (long)info.nsv, (long)info.nfree, acc);

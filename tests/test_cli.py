@@ -24,7 +24,7 @@ def test_usage_without_training_file():
 
 
 @pytest.mark.parametrize("args,msg", [
-    (["-precision", "fp8", "x"], "unknown precision fp8"),
+    (["-precision", "fp4", "x"], "unknown precision fp4"),
     (["-stop", "gapp", "x"], "unknown stop rule gapp"),
     (["-init", "zero", "x"], "unknown initial point zero"),
     (["-t", "1", "x"], "unimplemented -t 1"),

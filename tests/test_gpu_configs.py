@@ -145,3 +145,22 @@ def test_c4_bf16_ipm_k1024_matches_oracle(ctx, c4_case):
     assert ii.syrk_bf16_iters > 0
     assert abs(ii.pobj - io.pobj) <= 1e-6 * (1 + abs(io.pobj))
     assert np.abs(al - Xo).max() <= 1e-4 * C
+
+
+def test_c4_fp8_ipm_k1024_matches_oracle(ctx, c4_case):
+    """c4 shape in the FP8 form: the iterations whose small-t rows fit the e4m3 budget (and
+    whose remaining bulk is small) form the capacitance from the FP8 pass plus FP64 rows;
+    the refined Newton steps reach the oracle's FP64 optimum to the BF16 form's bounds."""
+    import tensorsvm_amd as T
+    X, y, om, F, w = c4_case
+    U, info = ctx.lra(X, y, 1024, 0.05, T.PREC_BF16, omega=om)
+    C = 32.0
+    al, xi, s, ii = ctx.ipm(U, y, C, precision=T.PREC_FP8, refine=2, stop=T.capi.STOP_GAP, tol=1e-7)
+    Xo, Xio, So, io = O.mpc(U.astype(np.float64), y.astype(np.float64), C, stop="gap", tol=1e-7)
+    parity_log("c4_ipm_fp8", dalpha_over_C=np.abs(al - Xo).max() / C,
+               dpobj_rel=abs(ii.pobj - io.pobj) / (1 + abs(io.pobj)), iters_gpu=ii.iters,
+               iters_oracle=io.iters, bf16_row_frac=ii.bf16_row_frac, fp8_row_frac=ii.fp8_row_frac)
+    assert ii.reason == 0 and io.reason == 0
+    assert ii.fp8_row_frac > 0.0  # the FP8 pass ran in some iterations
+    assert abs(ii.pobj - io.pobj) <= 1e-6 * (1 + abs(io.pobj))
+    assert np.abs(al - Xo).max() <= 1e-4 * C

@@ -75,3 +75,76 @@ def test_hybrid_deterministic(ctx):
     G2, i2 = _run(ctx, Z, w, T.PREC_BF16)
     assert i1.n_fp64_rows == i2.n_fp64_rows
     assert np.array_equal(G1, G2)
+
+
+def _fp8_case(n, k, seed, heavy=0.03):
+    """Late-solve weights: most rows tiny (non-support vectors, t ~ 1e-8..1e-4), a few
+    percent heavy (free support vectors, t ~ 1..100): the FP8 pass's regime."""
+    rng = np.random.default_rng(seed)
+    Z = (rng.standard_normal((n, k)) / np.sqrt(k)).astype(np.float32)
+    w = 10.0 ** rng.uniform(-8.0, -4.0, n)
+    hv = rng.random(n) < heavy
+    w[hv] = 10.0 ** rng.uniform(0.0, 2.0, int(hv.sum()))
+    Zd = Z.astype(np.float64)
+    M = np.eye(k) + Zd.T @ (w[:, None] * Zd)
+    return Z, w, M
+
+
+@pytest.mark.parametrize("n,k", [(100003, 1024), (40000, 1024), (30011, 900)])
+def test_fp8_pass_within_bound(ctx, n, k):
+    """TSVM_PREC_FP8 with late-solve weights at widths where the FP8 pass pays (kpt = 1024):
+    the small-t rows go through the e4m3 pass (per-column power-of-two scales), the rest
+    through FP64, none through BF16; the error of M in the 2-norm is within the bound the
+    split reports."""
+    import tensorsvm_amd as T
+    from conftest import parity_log
+    Z, w, M = _fp8_case(n, k, 13 * n + k)
+    G, info = _run(ctx, Z, w, T.PREC_FP8)
+    assert info.n_fp8_rows >= 0.9 * n, info.as_dict()
+    assert info.n_fp8_rows + info.n_fp64_rows == n
+    assert info.bf16_bound == 0.0
+    assert 0.0 < info.fp8_bound <= 2.0 ** -5
+    err = np.linalg.norm(G - M, 2)
+    G16, i16 = _run(ctx, Z, w, T.PREC_BF16)
+    err16 = np.linalg.norm(G16 - M, 2)
+    parity_log(f"syrk_fp8_{n}_{k}", err_fp8=err, bound_fp8=info.fp8_bound, err_bf16=err16,
+               bound_bf16=2.0 ** -9 * i16.bf16_bound, fp8_rows=int(info.n_fp8_rows),
+               fp64_rows=int(info.n_fp64_rows))
+    assert err <= info.fp8_bound + 1e-12 * np.abs(M).max(), (err, info.as_dict())
+    assert np.array_equal(G, G.T)
+    # the FP8 pass really ran: its error is far above the FP64 part's rounding
+    assert err > 1e-9
+
+
+def test_fp8_form_falls_back_to_bf16(ctx):
+    """The FP8 form keeps the BF16 split, bit-identical to the BF16 form, where the FP8
+    pass would not pay: weights spread over ten decades (too many rows between the FP8
+    and FP64 tiers), any weights at kpt <= 512 (packing outweighs the faster SYRK), and
+    k <= 128 (no FP8 pass at all)."""
+    import tensorsvm_amd as T
+    for n, k, late in [(100003, 1024, False), (100003, 512, True), (20000, 300, True),
+                       (30000, 100, True)]:
+        Z, w, _ = (_fp8_case if late else _case)(n, k, 7 * n + k)
+        G8, i8 = _run(ctx, Z, w, T.PREC_FP8)
+        G16, i16 = _run(ctx, Z, w, T.PREC_BF16)
+        assert i8.n_fp8_rows == 0 and i8.fp8_bound == 0.0
+        assert i8.n_fp64_rows == i16.n_fp64_rows
+        assert np.array_equal(G8, G16)
+
+
+def test_fp8_pass_all_rows(ctx):
+    """Every row tiny: all rows in the FP8 pass, no FP64 rows; the relative error of
+    Z^T diag(w) Z sits at the e4m3 level (a few 2^-4 per term, averaged over rows)."""
+    import tensorsvm_amd as T
+    n, k = 50000, 1024
+    rng = np.random.default_rng(21)
+    Z = (rng.standard_normal((n, k)) / np.sqrt(k)).astype(np.float32)
+    w = 10.0 ** rng.uniform(-7.0, -6.0, n)
+    Zd = Z.astype(np.float64)
+    S = Zd.T @ (w[:, None] * Zd)
+    G, info = _run(ctx, Z, w, T.PREC_FP8)
+    assert info.n_fp8_rows == n and info.n_fp64_rows == 0
+    E = G - np.eye(k) - S
+    rel = np.linalg.norm(E, 2) / np.linalg.norm(S, 2)
+    assert np.linalg.norm(E, 2) <= info.fp8_bound
+    assert 1e-5 < rel < 2.0 ** -4, rel
