@@ -364,12 +364,14 @@ def main():
         "fp32": fp32,
         "speedup_bf16_vs_fp32": (round(fp32["train_s"] / value, 2) if fp32 else None),
         "omega_gen_s": round(t_omega, 2),
-        "zdz": {"form": "hybrid capacitance: bulk rows on BF16 MFMA (error-budgeted split), "
-                        "high-weight rows on FP64 MFMA",
+        "zdz": {"form": "hybrid capacitance (error-budgeted split by w_i |z_i|^2): a dense BF16 "
+                        "(or FP8) MFMA pass over the small-weight rows, compensated hi+lo bf16 "
+                        "MFMA rows in the middle band, FP64 MFMA rows at the top",
                 "bf16_tflops": round(bf16_info["ipm"]["syrk_bf16_tflops"], 1),
                 "bf16_frac_of_peak": round(bf16_info["ipm"]["syrk_bf16_tflops"] / PEAK_BF16, 4),
                 "bf16_row_frac": round(bf16_info["ipm"]["bf16_row_frac"], 4),
                 "fp8_row_frac": round(bf16_info["ipm"]["fp8_row_frac"], 4),
+                "x2_row_frac": round(bf16_info["ipm"]["x2_row_frac"], 4),
                 "bf16_iters": bf16_info["ipm"]["syrk_bf16_iters"],
                 "t_capacitance_s": round(bf16_info["ipm"]["t_syrk"], 3),
                 "t_bf16_part_s": round(bf16_info["ipm"]["t_syrk_bf16"], 3),

@@ -163,6 +163,8 @@ typedef struct {
   double bf16_tflops;  /* n k (k+1) / t_bf16 */
   int64_t n_fp8_rows;  /* TSVM_PREC_FP8: rows in the FP8 tier */
   double fp8_bound;    /* TSVM_PREC_FP8: error bound of the FP8 tier */
+  int64_t n_x2_rows;   /* rows in the compensated (hi + lo bf16) tier */
+  double x2_bound;     /* bound on sqrt(sum t_i^2) over them */
 } tsvm_syrk_info;
 tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int64_t k,
                                    const double *w, int precision, double budget, double *M,
@@ -264,6 +266,7 @@ typedef struct {
   double bf16_row_frac;    /* rows in the BF16 part, averaged over iterations */
   int syrk_bf16_iters;     /* iterations with a BF16 part */
   double fp8_row_frac;     /* TSVM_PREC_FP8: rows in the FP8 tier, averaged over iterations */
+  double x2_row_frac;      /* rows in the compensated tier, averaged over iterations */
 } tsvm_ipm_info;
 void tsvm_ipm_default_params(tsvm_ipm_params *p);
 /* Mehrotra predictor-corrector for the SVM dual with Q = Z Z^T, every O(n) and O(nk)

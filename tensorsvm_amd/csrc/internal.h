@@ -94,10 +94,11 @@ tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
                         std::vector<double>& G, int* passes);
 
 // ---- weighted Gram M = I + Z^T diag(w) Z of the capacitance (syrk.hip) -----------
-// Hybrid form: bulk rows (small w_i |z_i|^2) on BF16 MFMA, the rest on FP64 MFMA.
-// With the FP8 tier, an iteration whose smallest-t rows fit the FP8 budget and whose
-// remaining bulk is small replaces the BF16 pass by an FP8 pass over those rows and sends
-// the rest to the FP64 part (the cheaper of the two by a per-row cost model).
+// Hybrid form: rows split by t_i = w_i |z_i|^2 into tiers, each under an error budget:
+// a dense pass over every row (BF16, or FP8 with the FP8 tier, for the smallest t),
+// compensated hi + lo bf16 rows (kpt >= 256, gathered through an index list), and FP64
+// rows (the largest t, through an index list).  Which dense pass runs, if any, is chosen
+// per iteration by a per-row cost model.
 struct HybridSyrk {
   int64_t n = 0, n_pad = 0;
   int kp = 0, kpt = 0;
@@ -105,6 +106,8 @@ struct HybridSyrk {
   double* nz = nullptr;    // |z_i|^2
   float* wb = nullptr;     // FP32 weights of the bulk rows (0 for FP64 rows), n_pad
   int* rows = nullptr;     // indices of the FP64 rows
+  int* rows2 = nullptr;    // indices of the compensated rows
+  bool x2 = false;         // compensated tier available (kpt >= 256)
   int* ints = nullptr;     // split scratch (histograms, counts, offsets, cut, count)
   // FP8 tier (fp8_eps > 0): the smallest-t rows on the FP8 MFMA with per-column scales
   double fp8_eps = 0.0;    // error budget of the tier (0: no FP8 tier)
@@ -113,27 +116,37 @@ struct HybridSyrk {
   float* sc8 = nullptr;    // per-column power-of-two scales of the iteration (kpt)
   void* Z8 = nullptr;      // e4m3 of sc8_c sqrt(w_i) z_ic (n_pad x kpt, chunked as ZbT)
   double r8 = 0.0;         // cost of the FP8 pass (packing included) / the BF16 pass
+  double r2 = 0.0;         // cost of one compensated row / one BF16-pass row
   double r64 = 0.0;        // cost of one FP64 row / one BF16-pass row
 };
+// rows per tier of one split: FP64, FP8 pass, compensated, BF16 pass
+struct HybridCounts {
+  int64_t nh = 0, n8 = 0, n2 = 0, nb = 0;
+};
+constexpr int HYBRID_NSLOT = 6;  // device doubles written by hybrid_syrk_split
 // Workspace and the once-per-solve operands (ZbT, nz) for Z (n x kp FP32, device).
 // fp8_eps > 0 (kp > 128 only) adds the FP8 tier with that error budget.
 tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridSyrk* hs,
                               double fp8_eps = 0.0);
 // error budget of the FP8 tier of TSVM_PREC_FP8
 double hybrid_fp8_eps();
-// Splits the rows for weights w (device): slots[0] = sqrt(sum over bulk rows of t_i^2)
-// bound, slots[1] = number of FP64 rows, slots[2] = error bound of the FP8 tier,
-// slots[3] = its number of rows, 0 unless the iteration takes the FP8 pass (device
-// doubles); fills rows, wb (and w8, sc8).
+// Splits the rows for weights w (device) with the BF16 tier's budget (the compensated
+// tier's is 2^7 times it): HYBRID_NSLOT device doubles, slots[0] = bound on sqrt(sum t_i^2)
+// over the BF16-pass rows, [1] = number of FP64 rows, [2] = error bound of the FP8-pass
+// rows, [3] = their number (0 unless the iteration takes the FP8 pass), [4] = bound on
+// sqrt(sum t_i^2) over the compensated rows, [5] = their number; fills rows, rows2, wb
+// (and w8, sc8).
 tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double budget,
                               double* slots);
-// M = I + FP64 part over the nh listed rows + BF16 part over the bulk, or, when n8 > 0,
-// the FP8 part over the n8 FP8 rows instead.  ev (optional, two events) brackets the
-// BF16 / FP8 part.
+// the tier sizes from the slots (host copy)
+HybridCounts hybrid_counts(const HybridSyrk* hs, const double* slots);
+// M = I + FP64 part over the listed rows + the dense pass (FP8 when hc.n8 > 0, BF16 when
+// hc.nb > 0) + the compensated part.  ev (optional, two events) brackets the
+// low-precision parts.
 tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const double* w,
-                             int64_t nh, double* M, hipEvent_t* ev, int64_t n8 = 0);
-// flops of one BF16 part, SYRK convention (rows streamed x k (k + 1))
-double hybrid_syrk_flops(const HybridSyrk* hs);
+                             const HybridCounts& hc, double* M, hipEvent_t* ev);
+// flops of the low-precision parts, SYRK convention (rows formed x k (k + 1))
+double hybrid_syrk_flops(const HybridSyrk* hs, const HybridCounts& hc);
 // tsvm_fnorm_res_dev (fnorm.hip): |K_Y|_F, |K_Y - U U^T|_F; U has row stride ldu
 tsvm_status fnorm_res_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d,
                           double gamma, const float* U, int64_t ldu, int64_t k, double* fnorm,

@@ -1229,6 +1229,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   TSVM_HIP(c, evb1.create());
   hipEvent_t evb[2] = {evb0, evb1};
   double t_syrk = 0.0, t_bf16 = 0.0, row_frac_sum = 0.0, fp8_frac_sum = 0.0;
+  double x2_frac_sum = 0.0, flops_low = 0.0, pending_flops = 0.0;
   int n_syrk = 0, n_bf16 = 0;
   bool pending_syrk = false, pending_bf16 = false;
   double hsc[IPM_NSLOT];
@@ -1267,6 +1268,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
       if (pending_bf16) {
         TSVM_HIP(c, hipEventElapsedTime(&ms, evb[0], evb[1]));
         t_bf16 += ms * 1e-3;
+        flops_low += pending_flops;
         n_bf16++;
       }
       pending_syrk = false;
@@ -1331,12 +1333,14 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     // ---- capacitance M = I + Z^T D^-1 Z and its Cholesky (1660-1714)
     TSVM_HIP(c, hipEventRecord(ev0, st));
     if (bf16) {
-      const int64_t nh = (int64_t)hsc[SL_HSN];
-      const int64_t n8 = fp8 ? (int64_t)hsc[SL_HS8N] : 0;
-      TSVM_CHECK(hybrid_syrk_form(c, &hs, Z, winv, nh, M, evb, n8));
-      pending_bf16 = nh < n;
-      row_frac_sum += 1.0 - (double)nh / (double)n;
-      fp8_frac_sum += (double)n8 / (double)n;
+      static_assert(SL_HS2N == SL_HSB + HYBRID_NSLOT - 1, "hybrid slots are contiguous");
+      const HybridCounts hc = hybrid_counts(&hs, hsc + SL_HSB);
+      TSVM_CHECK(hybrid_syrk_form(c, &hs, Z, winv, hc, M, evb));
+      pending_bf16 = hc.nh < n;
+      row_frac_sum += 1.0 - (double)hc.nh / (double)n;
+      fp8_frac_sum += (double)hc.n8 / (double)n;
+      x2_frac_sum += (double)hc.n2 / (double)n;
+      pending_flops = hybrid_syrk_flops(&hs, hc);
     } else {
       TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, n, winv, true, M));
       hipLaunchKernelGGL(k_add_identity, dim3((unsigned)ceil_div(kp, 256)), dim3(256), 0, st, M, (int64_t)kp);
@@ -1388,9 +1392,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   if (t_syrk > 0) inf.syrk_tflops = (double)n * k * (k + 1) * n_syrk / t_syrk * 1e-12;
   inf.t_syrk_bf16 = t_bf16;
   inf.syrk_bf16_iters = n_bf16;
-  if (t_bf16 > 0) inf.syrk_bf16_tflops = hybrid_syrk_flops(&hs) * n_bf16 / t_bf16 * 1e-12;
+  if (t_bf16 > 0) inf.syrk_bf16_tflops = flops_low / t_bf16 * 1e-12;
   if (n_syrk > 0) inf.bf16_row_frac = row_frac_sum / n_syrk;
   if (n_syrk > 0) inf.fp8_row_frac = fp8_frac_sum / n_syrk;
+  if (n_syrk > 0) inf.x2_row_frac = x2_frac_sum / n_syrk;
   if (info) *info = inf;
   return TSVM_OK;
 }

@@ -37,6 +37,8 @@ enum IpmSlot {
   SL_HSN,        // hybrid capacitance: number of FP64 rows (must follow SL_HSB)
   SL_HS8B,       // hybrid capacitance: error bound of the FP8 rows (must follow SL_HSN)
   SL_HS8N,       // hybrid capacitance: number of FP8 rows (must follow SL_HS8B)
+  SL_HS2B,       // hybrid capacitance: bound of the compensated rows (must follow SL_HS8N)
+  SL_HS2N,       // hybrid capacitance: number of compensated rows (must follow SL_HS2B)
   IPM_NSLOT = 32
 };
 

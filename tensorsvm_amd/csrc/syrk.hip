@@ -23,6 +23,8 @@
 // hundreds of rows a full column length apart (a TLB miss per load); nz = |z_i|^2
 // (FP64); per iteration wb (FP32 weights of the bulk rows, 0 for the FP64 rows) and rows
 // (indices of the FP64 rows, in order).
+#include <cstring>
+
 #include "common.h"
 #include "internal.h"
 
@@ -38,6 +40,9 @@ constexpr double HYBRID_FP8_EPS = 1.0 / 32.0;
 // index list costs about 20 BF16-pass rows
 __host__ inline double hybrid_fp8_cost(int kpt) { return 0.52 + 0.35 * 1024.0 / kpt; }
 constexpr double HYBRID_F64_COST = 20.0;
+// one compensated row: three BF16 SYRK rows plus its gather and hi/lo packing (FP32 row
+// read, two bf16 rows written: about 1.5 BF16-pass rows at kpt = 512, falling as 1 / kpt)
+__host__ inline double hybrid_x2_cost(int kpt) { return 3.0 + 0.75 * 1024.0 / kpt; }
 constexpr int HS_BINS = 128;     // bins of t_i by binary exponent: bin = floor(log2 t) + 64
 constexpr int SY_BK = 64;        // rows (the GEMM's K) per SYRK stage
 constexpr int SY_LS = SY_BK + 8;  // LDS row stride (bf16) of a staged panel
@@ -98,22 +103,30 @@ __global__ __launch_bounds__(256) void hs_hist_kernel(const double* __restrict__
   for (int b = threadIdx.x; b < HS_BINS; b += blockDim.x) part[blockIdx.x * HS_BINS + b] = h[b];
 }
 
+// Classes of a row by its bin of t_i (cuts c8 <= c1 <= c2, written by hs_cutoff_kernel
+// into cut[HC_C8], cut[HC_C1], cut[HC_C2]): bin < c8 the FP8 pass, c8 <= bin < c1 the
+// BF16 pass, c1 <= bin < c2 the compensated (hi + lo bf16) rows, bin >= c2 FP64.
+enum { HC_C2 = 0, HC_NH = 1, HC_C8 = 2, HC_W8MAX = 3, HC_C1 = 4, HC_N2 = 5, HC_INTS = 8 };
+
 // one block of 8 HS_BINS threads: total counts per bin (8 groups of partials, summed in
-// a fixed order), then the cutoff bin: the bulk is every bin below `cut`, the largest
-// prefix whose bound sqrt(sum cnt_b 4^(b-63)) stays within the budget.  out: cut[0] =
-// cut, cut[2] = cut8 (0: no FP8 pass), slots[0] = bound of the bulk, slots[2] = bound of
-// the FP8 rows, slots[3] = their number.
-// FP8 tier (eps8 > 0): the smallest-t bins while 2^-3 sqrt(sum t^2) (three e4m3 roundings
-// per term, 2^-4 each) plus the flush bound n8 t_max 2^-17 (an entry below the subnormal
-// step of its column's scale, at most 2^-16 of the largest, becomes 0) stays within eps8.
-// The iteration takes the FP8 pass (over bins < cut8, the rest in FP64) when the per-row
-// cost model says it is cheaper than the BF16 pass: r8 n + r64 nb < n, nb the bulk rows
-// that would move to FP64 (costs in units of one BF16-pass row).  Otherwise the split is
-// exactly the BF16 form's.
+// a fixed order), then the cuts.  Each tier keeps a bound on its error:
+// * BF16 pass: sqrt(sum t^2) over its rows <= budget (error about 2^-9 of that);
+// * compensated rows: sqrt(sum t^2) <= budget 2^7 (error about 2^-16 of that: the same
+//   absolute error as the BF16 pass);
+// * FP8 pass (eps8 > 0): 2^-3 sqrt(sum t^2) (three e4m3 roundings per term, 2^-4 each)
+//   plus the flush bound n8 t_max 2^-17 (an entry below the subnormal step of its
+//   column's scale, at most 2^-16 of the largest, becomes 0) <= eps8.
+// Three layouts are priced with the per-row costs (units of one BF16-pass row: the dense
+// passes stream every row, r2 per compensated row, r64 per FP64 row) and the cheapest
+// taken: (A) the BF16 pass over the smallest-t rows, then compensated, then FP64;
+// (B, eps8 > 0) the FP8 pass, then compensated, then FP64; (C) no dense pass: compensated
+// from the smallest t up, then FP64.  x2 = false: no compensated rows.  slots: [0] bound
+// of the BF16 rows, [2] error bound of the FP8 rows, [3] their number, [4] bound of the
+// compensated rows ([1], [5]: the FP64 and compensated counts, from hs_scan_kernel).
 constexpr int HS_CUT_GROUPS = 8;
 __global__ __launch_bounds__(HS_CUT_GROUPS * HS_BINS) void hs_cutoff_kernel(
-    const int* __restrict__ part, int nblocks, double budget, double eps8, double r8, double r64,
-    int* __restrict__ cut_out, double* __restrict__ slots) {
+    const int* __restrict__ part, int nblocks, double budget, double eps8, bool x2, double r8,
+    double r2, double r64, int* __restrict__ cut_out, double* __restrict__ slots) {
   __shared__ long long grp[HS_CUT_GROUPS][HS_BINS];
   __shared__ long long cnt[HS_BINS];
   const int b = threadIdx.x % HS_BINS, g = threadIdx.x / HS_BINS;
@@ -129,72 +142,117 @@ __global__ __launch_bounds__(HS_CUT_GROUPS * HS_BINS) void hs_cutoff_kernel(
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  double cum = 0.0;
-  int cut = HS_BINS;
-  for (int e = 0; e < HS_BINS; e++) {
-    if (cnt[e] == 0) continue;
-    const double add = (double)cnt[e] * ldexp(1.0, 2 * (e - 63));
-    if (sqrt(cum + add) > budget) {
-      cut = e;
-      break;
+  auto sq = [](int e) { return ldexp(1.0, 2 * (e - 63)); };  // bound of t^2 in bin e
+  // largest cut >= e0 with sqrt(sum over bins [e0, cut) of cnt 4^(e-63)) <= bud
+  auto cut_from = [&](int e0, double bud, double* cum_out) {
+    double cum = 0.0;
+    int cut = HS_BINS;
+    for (int e = e0; e < HS_BINS; e++) {
+      if (cnt[e] == 0) continue;
+      const double add = (double)cnt[e] * sq(e);
+      if (sqrt(cum + add) > bud) {
+        cut = e;
+        break;
+      }
+      cum += add;
     }
-    cum += add;
+    *cum_out = cum;
+    return cut;
+  };
+  auto rows_in = [&](int e0, int e1) {
+    double r = 0.0;
+    for (int e = e0; e < e1; e++) r += (double)cnt[e];
+    return r;
+  };
+  const double ntot = rows_in(0, HS_BINS);
+  const double bud2 = x2 ? budget * 128.0 : 0.0;
+  // (A)
+  double cumA, cum2A;
+  const int c1A = cut_from(0, budget, &cumA);
+  const int c2A = x2 ? cut_from(c1A, bud2, &cum2A) : c1A;
+  if (!x2) cum2A = 0.0;
+  const double costA = ntot + r2 * rows_in(c1A, c2A) + r64 * rows_in(c2A, HS_BINS);
+  int c8 = 0, c1 = c1A, c2 = c2A;
+  double cum1 = cumA, cum2 = cum2A, cost = costA;
+  // (C)
+  if (x2) {
+    double cum2C;
+    const int c2C = cut_from(0, bud2, &cum2C);
+    const double costC = r2 * rows_in(0, c2C) + r64 * rows_in(c2C, HS_BINS);
+    if (costC < cost) {
+      c8 = 0, c1 = 0, c2 = c2C, cum1 = 0.0, cum2 = cum2C, cost = costC;
+    }
   }
-  int cut8 = 0;
+  // (B)
   double cum8 = 0.0, n8 = 0.0;
   if (eps8 > 0.0) {
-    for (int e = 0; e < cut; e++) {
+    int cut8 = 0;
+    for (int e = 0; e < c1A; e++) {
       if (cnt[e] == 0) continue;  // cut8 stays one above the last non-empty bin taken
-      const double add = (double)cnt[e] * ldexp(1.0, 2 * (e - 63));
+      const double add = (double)cnt[e] * sq(e);
       const double err = ldexp(sqrt(cum8 + add), -3) + (n8 + cnt[e]) * ldexp(1.0, e - 63 - 17);
       if (err > eps8) break;
       cum8 += add;
       n8 += cnt[e];
       cut8 = e + 1;
     }
-    double ntot = 0.0, nb = 0.0;
-    for (int e = 0; e < HS_BINS; e++) ntot += (double)cnt[e];
-    for (int e = cut8; e < cut; e++) nb += (double)cnt[e];
-    if (!(n8 > 0.0 && r8 * ntot + r64 * nb < ntot)) {
-      cut8 = 0;
-      cum8 = 0.0;
-      n8 = 0.0;
+    if (n8 > 0.0) {
+      double cum2B = 0.0;
+      const int c2B = x2 ? cut_from(cut8, bud2, &cum2B) : cut8;
+      const double costB = r8 * ntot + r2 * rows_in(cut8, c2B) + r64 * rows_in(c2B, HS_BINS);
+      if (costB < cost) {
+        c8 = cut8, c1 = cut8, c2 = c2B, cum1 = 0.0, cum2 = cum2B, cost = costB;
+      } else {
+        n8 = 0.0;
+      }
     }
   }
-  if (n8 > 0.0) {  // FP8 pass: no BF16 rows, FP64 from cut8 up
-    cut = cut8;
-    cum = 0.0;
+  if (c8 == 0) {
+    n8 = 0.0;
+    cum8 = 0.0;
   }
-  cut_out[0] = cut;
-  cut_out[2] = cut8;
-  slots[0] = sqrt(cum);
-  slots[2] = n8 > 0.0 ? ldexp(sqrt(cum8), -3) + n8 * ldexp(1.0, cut8 - 1 - 63 - 17) : 0.0;
+  cut_out[HC_C2] = c2;
+  cut_out[HC_C8] = c8;
+  cut_out[HC_C1] = c1;
+  slots[0] = sqrt(cum1);
+  slots[2] = n8 > 0.0 ? ldexp(sqrt(cum8), -3) + n8 * ldexp(1.0, c8 - 1 - 63 - 17) : 0.0;
   slots[3] = n8;
+  slots[4] = sqrt(cum2);
 }
 
-// per-block counts of the FP64 rows (bin >= cut)
+// per-block counts of the FP64 rows (bin >= c2) and of the compensated rows
+// (c1 <= bin < c2)
 __global__ __launch_bounds__(256) void hs_count_kernel(const double* __restrict__ w,
                                                        const double* __restrict__ nz, int64_t n,
                                                        int64_t rpb, const int* __restrict__ cut,
-                                                       int* __restrict__ cnt) {
-  __shared__ int tot;
-  if (threadIdx.x == 0) tot = 0;
+                                                       int* __restrict__ cnt,
+                                                       int* __restrict__ cnt2) {
+  __shared__ int tot[2];
+  if (threadIdx.x < 2) tot[threadIdx.x] = 0;
   __syncthreads();
-  const int cb = *cut;
+  const int c1 = cut[HC_C1], c2 = cut[HC_C2];
   const int64_t r0 = blockIdx.x * rpb, r1 = r0 + rpb < n ? r0 + rpb : n;
-  int c = 0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) c += hs_bin(w[i] * nz[i]) >= cb;
-  atomicAdd(&tot, c);
+  int ch = 0, cm = 0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+    const int bin = hs_bin(w[i] * nz[i]);
+    ch += bin >= c2;
+    cm += bin >= c1 && bin < c2;
+  }
+  atomicAdd(&tot[0], ch);
+  atomicAdd(&tot[1], cm);
   __syncthreads();
-  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+  if (threadIdx.x == 0) {
+    cnt[blockIdx.x] = tot[0];
+    cnt2[blockIdx.x] = tot[1];
+  }
 }
 
 // one block of HS_BLOCKS threads: exclusive scan of the block counts (Hillis-Steele in
-// LDS); slots[1] = number of FP64 rows
+// LDS); *slot = *total = the sum
 __global__ __launch_bounds__(HS_BLOCKS) void hs_scan_kernel(const int* __restrict__ cnt,
                                                             int nblocks, int* __restrict__ off,
-                                                            double* __restrict__ slots,
-                                                            int* __restrict__ nh) {
+                                                            double* __restrict__ slot,
+                                                            int* __restrict__ total) {
   __shared__ int v[2][HS_BLOCKS];
   const int t = threadIdx.x;
   v[0][t] = t < nblocks ? cnt[t] : 0;
@@ -208,44 +266,47 @@ __global__ __launch_bounds__(HS_BLOCKS) void hs_scan_kernel(const int* __restric
   // v[cur] holds inclusive sums
   if (t < nblocks) off[t] = v[cur][t] - (t < nblocks ? cnt[t] : 0);
   if (t == HS_BLOCKS - 1) {
-    slots[1] = (double)v[cur][t];
-    *nh = v[cur][t];
+    *slot = (double)v[cur][t];
+    *total = v[cur][t];
   }
 }
 
-// rows of the FP64 part in increasing order; wb = FP32 weight of the bulk rows, 0 for
-// the FP64 rows and the padding
-// FP8 rows (bin < cut[2], when w8 is given): wb = 0, w8 = sqrt(w) (FP32) and
-// w8max (bits of a non-negative float) = their maximum
+// rows of the FP64 part and of the compensated part, each in increasing order; wb = FP32
+// weight of the BF16-pass rows, 0 elsewhere and on the padding; w8 (when given) =
+// sqrt(w) (FP32) on the FP8-pass rows, 0 elsewhere, and w8max (bits of a non-negative
+// float) their maximum
 __global__ __launch_bounds__(256) void hs_scatter_kernel(const double* __restrict__ w,
                                                          const double* __restrict__ nz, int64_t n,
                                                          int64_t n_pad, int64_t rpb,
                                                          const int* __restrict__ cut,
                                                          const int* __restrict__ off,
+                                                         const int* __restrict__ off2,
                                                          int* __restrict__ rows,
+                                                         int* __restrict__ rows2,
                                                          float* __restrict__ wb,
                                                          float* __restrict__ w8,
                                                          unsigned int* __restrict__ w8max) {
-  __shared__ int wsum[4];
-  const int cb = cut[0];
-  const int c8 = w8 ? cut[2] : 0;
+  __shared__ int wsum[2][4];
+  const int c2 = cut[HC_C2], c1 = cut[HC_C1];
+  const int c8 = w8 ? cut[HC_C8] : 0;
   float mx = 0.0f;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t r0 = blockIdx.x * rpb;
   int64_t r1 = r0 + rpb < n_pad ? r0 + rpb : n_pad;
-  int base = off[blockIdx.x];
+  int base = off[blockIdx.x], base2 = off2[blockIdx.x];
   for (int64_t c0 = r0; c0 < r1; c0 += 256) {
     const int64_t i = c0 + threadIdx.x;
-    bool hi = false;
+    bool hi = false, mid = false;
     float wf = 0.0f, w8f = 0.0f;
     if (i < r1 && i < n) {
       const double wi = w[i];
       const int bin = hs_bin(wi * nz[i]);
-      hi = bin >= cb;
+      hi = bin >= c2;
+      mid = bin >= c1 && bin < c2;
       if (bin < c8) {  // wi >= 0 here: negative and NaN weights are in the last bin
         w8f = (float)sqrt(wi);
         mx = fmaxf(mx, w8f);
-      } else if (!hi) {
+      } else if (bin < c1) {
         wf = (float)wi;
       }
     }
@@ -253,15 +314,22 @@ __global__ __launch_bounds__(256) void hs_scatter_kernel(const double* __restric
       wb[i] = wf;
       if (w8) w8[i] = w8f;
     }
-    const unsigned long long m = __ballot(hi);
-    const int below = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) wsum[wave] = __popcll(m);
+    const unsigned long long m = __ballot(hi), m2 = __ballot(mid);
+    const unsigned long long below_mask = (1ull << lane) - 1ull;
+    if (lane == 0) {
+      wsum[0][wave] = __popcll(m);
+      wsum[1][wave] = __popcll(m2);
+    }
     __syncthreads();
-    int pre = 0;
-    for (int k = 0; k < wave; k++) pre += wsum[k];
-    const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    if (hi) rows[base + pre + below] = (int)i;
-    base += tot;
+    int pre = 0, pre2 = 0;
+    for (int k = 0; k < wave; k++) {
+      pre += wsum[0][k];
+      pre2 += wsum[1][k];
+    }
+    if (hi) rows[base + pre + __popcll(m & below_mask)] = (int)i;
+    if (mid) rows2[base2 + pre2 + __popcll(m2 & below_mask)] = (int)i;
+    base += wsum[0][0] + wsum[0][1] + wsum[0][2] + wsum[0][3];
+    base2 += wsum[1][0] + wsum[1][1] + wsum[1][2] + wsum[1][3];
     __syncthreads();
   }
   if (w8) {
@@ -427,10 +495,12 @@ __global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict
 constexpr int SB_K = SY_BK;      // the chunk of the ZbT layout
 constexpr int SB_LS = SB_K + 8;  // LDS row stride (bf16): conflict-free 16-byte reads
 
-template <bool DIAG>
-__device__ __forceinline__ void syrk256_stages(const __bf16* __restrict__ Apan,
-                                               const __bf16* __restrict__ Bpan,
-                                               const float* __restrict__ wb, int kpt, int64_t s0,
+// APAN(st) / BPAN(st): the A / B panel (256 columns) of the operand chunk of stage st;
+// SCALE: the A panel is scaled by the row weights wb (stage st covers rows
+// (s0 + st) SB_K ..), else both panels are staged as loaded
+template <bool DIAG, bool SCALE, class APAN, class BPAN>
+__device__ __forceinline__ void syrk256_stages(APAN apan, BPAN bpan,
+                                               const float* __restrict__ wb, int64_t s0,
                                                int64_t nst, __bf16 (*sm)[256 * SB_LS],
                                                f32x16_t (&acc)[4][4], int tid, int wa, int wbo,
                                                int l32, int h) {
@@ -440,21 +510,27 @@ __device__ __forceinline__ void syrk256_stages(const __bf16* __restrict__ Apan,
   u32x4_t v[NP][8];
   f32x4_t wv[2];
   auto gload = [&](int64_t st) {
-    const int64_t base = ((s0 + st) * kpt + srow) * SB_K + scol;
+    const int64_t base = (int64_t)srow * SB_K + scol;
 #pragma unroll
-    for (int p = 0; p < NP; p++)
+    for (int p = 0; p < NP; p++) {
+      const __bf16* pan = p ? bpan(st) : apan(st);
 #pragma unroll
-      for (int q = 0; q < 8; q++)
-        v[p][q] = *(const u32x4_t*)((p ? Bpan : Apan) + base + (int64_t)32 * q * SB_K);
-    wv[0] = *(const f32x4_t*)(wb + (s0 + st) * SB_K + scol);
-    wv[1] = *(const f32x4_t*)(wb + (s0 + st) * SB_K + scol + 4);
+      for (int q = 0; q < 8; q++) v[p][q] = *(const u32x4_t*)(pan + base + (int64_t)32 * q * SB_K);
+    }
+    if (SCALE) {
+      wv[0] = *(const f32x4_t*)(wb + (s0 + st) * SB_K + scol);
+      wv[1] = *(const f32x4_t*)(wb + (s0 + st) * SB_K + scol + 4);
+    }
   };
   // half hf of the staging: panel rows (tid >> 3) + 32 q for q in [4 hf, 4 hf + 4);
-  // A scaled, B as loaded
+  // A scaled (SCALE), B as loaded
   auto sstore = [&](int buf, int hf) {
 #pragma unroll
     for (int q = 4 * hf; q < 4 * hf + 4; q++) {
-      *(bf16x8_t*)&sm[2 * buf][(srow + 32 * q) * SB_LS + scol] = scale8(v[0][q], wv);
+      if (SCALE)
+        *(bf16x8_t*)&sm[2 * buf][(srow + 32 * q) * SB_LS + scol] = scale8(v[0][q], wv);
+      else
+        *(u32x4_t*)&sm[2 * buf][(srow + 32 * q) * SB_LS + scol] = v[0][q];
       *(u32x4_t*)&sm[2 * buf + 1][(srow + 32 * q) * SB_LS + scol] = v[NP - 1][q];
     }
   };
@@ -553,10 +629,13 @@ __global__ __launch_bounds__(256, 1) void syrk_bf16_256_kernel(const __bf16* __r
       for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
   const __bf16* Apan = ZbT + (int64_t)ta * 256 * SB_K;
   const __bf16* Bpan = ZbT + (int64_t)tb * 256 * SB_K;
+  const int64_t cstride = (int64_t)kpt * SB_K;  // elements per chunk
+  auto apan = [&](int64_t st) { return Apan + (s0 + st) * cstride; };
+  auto bpan = [&](int64_t st) { return Bpan + (s0 + st) * cstride; };
   if (ta == tb)
-    syrk256_stages<true>(Apan, Bpan, wb, kpt, s0, nst, sm, acc, tid, wa, wbo, l32, h);
+    syrk256_stages<true, true>(apan, bpan, wb, s0, nst, sm, acc, tid, wa, wbo, l32, h);
   else
-    syrk256_stages<false>(Apan, Bpan, wb, kpt, s0, nst, sm, acc, tid, wa, wbo, l32, h);
+    syrk256_stages<false, true>(apan, bpan, wb, s0, nst, sm, acc, tid, wa, wbo, l32, h);
   if (ta == tb && wbo > wa) return;  // strictly above the diagonal
   // C layout of the 32x32 MFMA: register q holds row 8(q/4) + 4h + q%4, column l32
   float* P = part + (int64_t)split * kpt * kpt;
@@ -741,6 +820,111 @@ __global__ __launch_bounds__(256, 1) void syrk_fp8_256_kernel(const unsigned cha
       }
 }
 
+// Compensated rows: y_i = sqrt(w_i) z_i in FP64, hi = bf16(y), lo = bf16(y - hi), so that
+// y_a y_b = hi_a hi_b + hi_a lo_b + lo_a hi_b up to about 3 2^-18 |y_a| |y_b|.  H and L
+// hold hi and lo of the listed rows rows2[0..m2) (compacted, zero padded to m2p rows),
+// chunked like ZbT.  Block: 64 compacted rows x 64 columns through an LDS transpose
+// (rows read 256 bytes at a time, chunk columns written 128 bytes at a time).
+__global__ __launch_bounds__(256) void pack_x2_kernel(const float* __restrict__ Z, int kp,
+                                                      int kpt, const double* __restrict__ w,
+                                                      const int* __restrict__ rows2, int64_t m2,
+                                                      __bf16* __restrict__ H,
+                                                      __bf16* __restrict__ L) {
+  __shared__ double tile[64][65];
+  __shared__ double sw[64];
+  __shared__ int src[64];
+  const int64_t q = blockIdx.x;
+  const int c0 = blockIdx.y * 64;
+  const int t = threadIdx.x;
+  if (t < 64) {
+    const int64_t r = q * 64 + t;
+    const int i = r < m2 ? rows2[r] : -1;
+    src[t] = i;
+    sw[t] = i >= 0 ? sqrt(w[i]) : 0.0;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int p = 0; p < 16; p++) {
+    const int e = p * 256 + t, r = e >> 6, cc = e & 63;
+    const int i = src[r];
+    double v = 0.0;
+    if (i >= 0 && c0 + cc < kp) v = sw[r] * (double)Z[(int64_t)i * kp + c0 + cc];
+    tile[r][cc] = v;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int p = 0; p < 16; p++) {
+    const int e = p * 256 + t, cc = e >> 6, r = e & 63;
+    const double y = tile[r][cc];
+    const __bf16 hi = (__bf16)(float)y;
+    const __bf16 lo = (__bf16)(float)(y - (double)(float)hi);
+    const int64_t o = (q * kpt + c0 + cc) * SY_BK + r;
+    H[o] = hi;
+    L[o] = lo;
+  }
+}
+
+// Compensated SYRK (kpt a multiple of 256): part[split] (lower 256 x 256 tiles) = the
+// split's share of sum over the compacted rows of hi hi^T + hi lo^T + lo hi^T, as one
+// product over the K-concatenation [hi | hi | lo] [hi | lo | hi]^T: stage g of 3 S
+// (S = m2p / 64 chunks) takes its A panel from chunk g % S of H (g < 2 S) or L, its B
+// panel from H (g < S or g >= 2 S) or L.  The sum is symmetric in (a, b), so only lower
+// tiles are formed; the tiling is syrk_bf16_256_kernel's, both panels always staged.
+__global__ __launch_bounds__(256, 1) void syrk_x2_256_kernel(const __bf16* __restrict__ H,
+                                                             const __bf16* __restrict__ L,
+                                                             int64_t S, int kpt, int ntile,
+                                                             int nsplit, int64_t stages_per_split,
+                                                             float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) __bf16 sm[4][256 * SB_LS];  // [buf * 2 + panel]
+  const int nlow = ntile * (ntile + 1) / 2;
+  const int bid = blockIdx.x;
+  const int xcd = bid % 8, grp = bid / 8;
+  const int tile = grp % nlow;
+  const int split = (grp / nlow) * 8 + xcd;
+  if (split >= nsplit) return;
+  int ta = 0;
+  while ((ta + 1) * (ta + 2) / 2 <= tile) ta++;
+  const int tb = tile - ta * (ta + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, h = lane >> 5;
+  const int wa = (wave >> 1) * 128, wbo = (wave & 1) * 128;
+  const int64_t s0 = (int64_t)split * stages_per_split;
+  const int64_t nst_all = 3 * S;
+  const int64_t s1 = s0 + stages_per_split < nst_all ? s0 + stages_per_split : nst_all;
+  const int64_t nst = s1 > s0 ? s1 - s0 : 0;
+  f32x16_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[i][j][q] = 0.0f;
+  const int64_t cstride = (int64_t)kpt * SB_K;
+  const int64_t aoff = (int64_t)ta * 256 * SB_K, boff = (int64_t)tb * 256 * SB_K;
+  auto apan = [&](int64_t st) {
+    const int64_t g = s0 + st, c = g % S;
+    return (g < 2 * S ? H : L) + c * cstride + aoff;
+  };
+  auto bpan = [&](int64_t st) {
+    const int64_t g = s0 + st, c = g % S;
+    return (g >= S && g < 2 * S ? L : H) + c * cstride + boff;
+  };
+  syrk256_stages<false, false>(apan, bpan, nullptr, s0, nst, sm, acc, tid, wa, wbo, l32, h);
+  if (ta == tb && wbo > wa) return;  // strictly above the diagonal
+  // C layout of the 32x32 MFMA: register q holds row 8(q/4) + 4h + q%4, column l32
+  float* P = part + (int64_t)split * kpt * kpt;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int a = ta * 256 + wa + 32 * i + 8 * (q / 4) + 4 * h + (q % 4);
+        const int bcol = tb * 256 + wbo + 32 * j + l32;
+        P[(int64_t)a * kpt + bcol] = acc[i][j][q];
+      }
+}
+
 __global__ void add_identity_kernel(double* __restrict__ M, int64_t k) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < k) M[i * k + i] += 1.0;
@@ -759,7 +943,11 @@ tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridS
   hs->kp = kp;
   hs->kpt = kp <= 128 ? 128 : (int)round_up(kp, 256);
   hs->fp8_eps = hs->kpt >= 256 ? fp8_eps : 0.0;
+  hs->x2 = hs->kpt >= 256;
   hs->n_pad = round_up(n > 0 ? n : 1, SY_BK);
+  hs->r8 = hybrid_fp8_cost(hs->kpt);
+  hs->r2 = hybrid_x2_cost(hs->kpt);
+  hs->r64 = HYBRID_F64_COST;
   void* p;
   TSVM_HIP(c, c->ws.get("hs_ZbT", (size_t)hs->kpt * hs->n_pad * 2, &p));
   hs->ZbT = p;
@@ -769,7 +957,9 @@ tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridS
   hs->wb = (float*)p;
   TSVM_HIP(c, c->ws.get("hs_rows", (size_t)(n > 0 ? n : 1) * 4, &p));
   hs->rows = (int*)p;
-  TSVM_HIP(c, c->ws.get("hs_ints", (size_t)(HS_BLOCKS * (HS_BINS + 2) + 8) * 4, &p));
+  TSVM_HIP(c, c->ws.get("hs_rows2", (size_t)(n > 0 ? n : 1) * 4, &p));
+  hs->rows2 = (int*)p;
+  TSVM_HIP(c, c->ws.get("hs_ints", (size_t)(HS_BLOCKS * (HS_BINS + 4) + HC_INTS) * 4, &p));
   hs->ints = (int*)p;
   hipStream_t s = c->stream;
   if (hs->fp8_eps > 0.0) {
@@ -781,8 +971,6 @@ tsvm_status hybrid_syrk_setup(Ctx* c, const float* Z, int64_t n, int kp, HybridS
     hs->sc8 = (float*)p;
     TSVM_HIP(c, c->ws.get("hs_Z8", (size_t)hs->kpt * hs->n_pad, &p));
     hs->Z8 = p;
-    hs->r8 = hybrid_fp8_cost(hs->kpt);
-    hs->r64 = HYBRID_F64_COST;
     hipLaunchKernelGGL(zero_u32s_kernel, dim3((unsigned)ceil_div(hs->kpt, 256)), dim3(256), 0, s,
                        (unsigned int*)hs->zmax, hs->kpt);
     hipLaunchKernelGGL(colmax_kernel, dim3(1024, (unsigned)(hs->kpt / 64)), dim3(256), 0, s, Z, n,
@@ -803,20 +991,26 @@ tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double bu
   int* part = hs->ints;                       // HS_BLOCKS x HS_BINS
   int* cnt = part + HS_BLOCKS * HS_BINS;      // HS_BLOCKS
   int* off = cnt + HS_BLOCKS;                 // HS_BLOCKS
-  int* cut = off + HS_BLOCKS;                 // 1
-  int* nh = cut + 1;                          // 1
+  int* cnt2 = off + HS_BLOCKS;                // HS_BLOCKS
+  int* off2 = cnt2 + HS_BLOCKS;               // HS_BLOCKS
+  int* cut = off2 + HS_BLOCKS;                // HC_INTS
   const int64_t rpb = round_up(ceil_div(hs->n_pad, HS_BLOCKS), 256);
   const int nb = (int)ceil_div(hs->n_pad, rpb);
   hipLaunchKernelGGL(hs_hist_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb, part);
   hipLaunchKernelGGL(hs_cutoff_kernel, dim3(1), dim3(HS_CUT_GROUPS * HS_BINS), 0, s,
-                     (const int*)part, nb, budget, hs->fp8_eps, hs->r8, hs->r64, cut, slots);
+                     (const int*)part, nb, budget, hs->fp8_eps, hs->x2, hs->r8, hs->r2, hs->r64,
+                     cut, slots);
   hipLaunchKernelGGL(hs_count_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb,
-                     (const int*)cut, cnt);
-  hipLaunchKernelGGL(hs_scan_kernel, dim3(1), dim3(HS_BLOCKS), 0, s, (const int*)cnt, nb, off, slots, nh);
-  unsigned int* w8max = (unsigned int*)(cut + 3);  // cut[0] cut, cut[1] FP64 rows, cut[2] cut8
+                     (const int*)cut, cnt, cnt2);
+  hipLaunchKernelGGL(hs_scan_kernel, dim3(1), dim3(HS_BLOCKS), 0, s, (const int*)cnt, nb, off,
+                     slots + 1, cut + HC_NH);
+  hipLaunchKernelGGL(hs_scan_kernel, dim3(1), dim3(HS_BLOCKS), 0, s, (const int*)cnt2, nb, off2,
+                     slots + 5, cut + HC_N2);
+  unsigned int* w8max = (unsigned int*)(cut + HC_W8MAX);
   if (hs->w8) hipLaunchKernelGGL(zero_u32s_kernel, dim3(1), dim3(64), 0, s, w8max, 1);
   hipLaunchKernelGGL(hs_scatter_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, hs->n_pad, rpb,
-                     (const int*)cut, (const int*)off, hs->rows, hs->wb, hs->w8, w8max);
+                     (const int*)cut, (const int*)off, (const int*)off2, hs->rows, hs->rows2,
+                     hs->wb, hs->w8, w8max);
   if (hs->w8)
     hipLaunchKernelGGL(fp8_scales_kernel, dim3((unsigned)ceil_div(hs->kpt, 256)), dim3(256), 0, s,
                        (const float*)hs->zmax, hs->kpt, (const int*)cut,
@@ -825,46 +1019,66 @@ tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double bu
   return TSVM_OK;
 }
 
+HybridCounts hybrid_counts(const HybridSyrk* hs, const double* slots) {
+  HybridCounts hc;
+  hc.nh = (int64_t)slots[1];
+  hc.n8 = hs->fp8_eps > 0.0 ? (int64_t)slots[3] : 0;
+  hc.n2 = hs->x2 ? (int64_t)slots[5] : 0;
+  hc.nb = hs->n - hc.nh - hc.n8 - hc.n2;
+  return hc;
+}
+
+// splits of a dense or compensated pass over nst stages: as many per XCD as fit its CUs
+// with every tile of a split resident at once (the 128 kernel runs two blocks per CU, the
+// 256 kernels one)
+static void pass_splits(const Ctx* c, int kpt, int64_t nst, int* ntile, int* nlow, int* nsplit,
+                        int64_t* sps) {
+  const bool big = kpt >= 256;
+  *ntile = kpt / (big ? 256 : 128);
+  *nlow = *ntile * (*ntile + 1) / 2;
+  const int slots_xcd = (c->cu_count / 8) * (big ? 1 : 2);
+  int ns = 8 * (slots_xcd / *nlow > 1 ? slots_xcd / *nlow : 1);
+  if (ns > round_up(nst, 8)) ns = (int)round_up(nst, 8);
+  *nsplit = ns;
+  *sps = ceil_div(nst, ns);
+}
+
 tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const double* w,
-                             int64_t nh, double* M, hipEvent_t* ev_bf16, int64_t n8) {
+                             const HybridCounts& hc, double* M, hipEvent_t* ev_low) {
   hipStream_t s = c->stream;
   const int64_t n = hs->n;
   const int kp = hs->kp, kpt = hs->kpt;
-  // FP64 rows (all rows when every row is FP64: no index list)
-  if (nh >= n)
+  // FP64 rows (all rows when every row is FP64: no index list); this GEMM also writes
+  // M's first value (zero for no rows), which the other parts add to
+  if (hc.nh >= n)
     TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, n, w, true, M));
   else
-    TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, nh, w, true, M, hs->rows));
+    TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, hc.nh, w, true, M, hs->rows));
   hipLaunchKernelGGL(add_identity_kernel, dim3((unsigned)ceil_div(kp, 256)), dim3(256), 0, s, M,
                      (int64_t)kp);
-  if (ev_bf16) TSVM_HIP(c, hipEventRecord(ev_bf16[0], s));
-  const bool fp8 = n8 > 0 && hs->Z8 && kpt >= 256;
-  if (nh < n) {
-    // splits: as many per XCD as fit its CUs with every tile of a split resident at once
-    // (the 128 kernel runs two blocks per CU, the 256 kernels one)
-    const bool big = kpt >= 256;
-    const int ntile = kpt / (big ? 256 : 128);
-    const int nlow = ntile * (ntile + 1) / 2;
+  if (ev_low) TSVM_HIP(c, hipEventRecord(ev_low[0], s));
+  const dim3 accg((unsigned)ceil_div((int64_t)kp * kp, 256));
+  int ntile, nlow, nsplit;
+  int64_t sps;
+  if (hc.n8 > 0 || hc.nb > 0) {
+    // the dense pass over every row: FP8 (FP8 rows, w8 = 0 elsewhere) or BF16 (wb = 0
+    // outside the BF16 rows)
+    const bool fp8 = hc.n8 > 0 && hs->Z8 && kpt >= 256;
     const int64_t nst = hs->n_pad / SY_BK;
-    const int slots_xcd = (c->cu_count / 8) * (big ? 1 : 2);
-    int nsplit = 8 * (slots_xcd / nlow > 1 ? slots_xcd / nlow : 1);
-    if (nsplit > round_up(nst, 8)) nsplit = (int)round_up(nst, 8);
-    const int64_t sps = ceil_div(nst, nsplit);
+    pass_splits(c, kpt, nst, &ntile, &nlow, &nsplit, &sps);
     void* part;
     TSVM_HIP(c, c->ws.get("hs_part", (size_t)nsplit * kpt * kpt * 4, &part));
     if (fp8) {
-      // the FP8 pass over the FP8 rows (the others have w8 = 0) in place of the BF16 pass
       hipLaunchKernelGGL(pack_fp8_kernel, dim3((unsigned)nst, (unsigned)(kpt / 256)), dim3(256), 0, s,
                          (const __bf16*)hs->ZbT, kpt, (const float*)hs->w8,
                          (const float*)hs->sc8, (unsigned char*)hs->Z8);
       hipLaunchKernelGGL(syrk_fp8_256_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
                          (const unsigned char*)hs->Z8, hs->n_pad, kpt, ntile, nsplit, sps,
                          (float*)part);
-      hipLaunchKernelGGL(syrk_accum_kernel, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)),
-                         dim3(256), 0, s, (const float*)part, nsplit, kpt, kp, M,
-                         (const float*)hs->sc8);
+      hipLaunchKernelGGL(syrk_accum_kernel, accg, dim3(256), 0, s, (const float*)part, nsplit,
+                         kpt, kp, M, (const float*)hs->sc8);
     } else {
-      if (big)
+      if (kpt >= 256)
         hipLaunchKernelGGL(syrk_bf16_256_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
                            (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile,
                            nsplit, sps, (float*)part);
@@ -872,21 +1086,39 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
         hipLaunchKernelGGL(syrk_bf16_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
                            (const __bf16*)hs->ZbT, hs->n_pad, kpt, (const float*)hs->wb, ntile,
                            nsplit, sps, (float*)part);
-      hipLaunchKernelGGL(syrk_accum_kernel, dim3((unsigned)ceil_div((int64_t)kp * kp, 256)),
-                         dim3(256), 0, s, (const float*)part, nsplit, kpt, kp, M,
-                         (const float*)nullptr);
+      hipLaunchKernelGGL(syrk_accum_kernel, accg, dim3(256), 0, s, (const float*)part, nsplit,
+                         kpt, kp, M, (const float*)nullptr);
     }
   }
-  if (ev_bf16) TSVM_HIP(c, hipEventRecord(ev_bf16[1], s));
+  if (hc.n2 > 0 && hs->x2) {
+    // the compensated rows, compacted: capacity grown in steps of an eighth of the rows
+    const int64_t m2p = round_up(hc.n2, SY_BK);
+    const int64_t cap = round_up(m2p, round_up(ceil_div(hs->n_pad, 8), SY_BK));
+    void *ph, *pl;
+    TSVM_HIP(c, c->ws.get("hs_x2H", (size_t)cap * kpt * 2, &ph));
+    TSVM_HIP(c, c->ws.get("hs_x2L", (size_t)cap * kpt * 2, &pl));
+    const int64_t S = m2p / SY_BK;
+    hipLaunchKernelGGL(pack_x2_kernel, dim3((unsigned)S, (unsigned)(kpt / 64)), dim3(256), 0, s, Z,
+                       kp, kpt, w, (const int*)hs->rows2, hc.n2, (__bf16*)ph, (__bf16*)pl);
+    pass_splits(c, kpt, 3 * S, &ntile, &nlow, &nsplit, &sps);
+    void* part;
+    TSVM_HIP(c, c->ws.get("hs_part2", (size_t)nsplit * kpt * kpt * 4, &part));
+    hipLaunchKernelGGL(syrk_x2_256_kernel, dim3((unsigned)(nlow * nsplit)), dim3(256), 0, s,
+                       (const __bf16*)ph, (const __bf16*)pl, S, kpt, ntile, nsplit, sps,
+                       (float*)part);
+    hipLaunchKernelGGL(syrk_accum_kernel, accg, dim3(256), 0, s, (const float*)part, nsplit, kpt,
+                       kp, M, (const float*)nullptr);
+  }
+  if (ev_low) TSVM_HIP(c, hipEventRecord(ev_low[1], s));
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
 }
 
 double hybrid_fp8_eps() { return HYBRID_FP8_EPS; }
 
-double hybrid_syrk_flops(const HybridSyrk* hs) {
-  // rows streamed by the BF16 kernel times the SYRK count k (k + 1)
-  return (double)hs->n_pad * hs->kp * (hs->kp + 1.0);
+double hybrid_syrk_flops(const HybridSyrk* hs, const HybridCounts& hc) {
+  // rows formed in low precision (each once) times the SYRK count k (k + 1)
+  return (double)(hs->n - hc.nh) * hs->kp * (hs->kp + 1.0);
 }
 
 tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, const double* w,
@@ -897,12 +1129,8 @@ tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, cons
     return fail(c, TSVM_ERR_ARG, "weighted_syrk: unknown precision %d", precision);
   hipStream_t s = c->stream;
   tsvm_syrk_info inf;
+  memset(&inf, 0, sizeof(inf));
   inf.n_fp64_rows = n;
-  inf.bf16_bound = 0.0;
-  inf.t_bf16 = 0.0;
-  inf.bf16_tflops = 0.0;
-  inf.n_fp8_rows = 0;
-  inf.fp8_bound = 0.0;
   if (precision == TSVM_PREC_FP32 || n == 0) {
     TSVM_CHECK(gemm_tn_f64(c, Z, k, k, Z, k, k, n, w, true, M));
     hipLaunchKernelGGL(add_identity_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, s, M, k);
@@ -912,33 +1140,31 @@ tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, cons
     TSVM_CHECK(hybrid_syrk_setup(c, Z, n, (int)k, &hs,
                                  precision == TSVM_PREC_FP8 ? HYBRID_FP8_EPS : 0.0));
     void* p;
-    TSVM_HIP(c, c->ws.get("hs_slots", 32, &p));
+    TSVM_HIP(c, c->ws.get("hs_slots", HYBRID_NSLOT * 8, &p));
     double* slots = (double*)p;
     TSVM_CHECK(hybrid_syrk_split(c, &hs, w, budget > 0 ? budget : 4.0, slots));
-    double hsl[4] = {0, 0, 0, 0};
-    TSVM_HIP(c, hipMemcpyAsync(hsl, slots, 32, hipMemcpyDeviceToHost, s));
+    double hsl[HYBRID_NSLOT] = {0};
+    TSVM_HIP(c, hipMemcpyAsync(hsl, slots, sizeof(hsl), hipMemcpyDeviceToHost, s));
     TSVM_HIP(c, hipStreamSynchronize(s));
     Event ev0, ev1;
     TSVM_HIP(c, ev0.create());
     TSVM_HIP(c, ev1.create());
     hipEvent_t ev[2] = {ev0, ev1};
-    const int64_t nh = (int64_t)hsl[1];
-    const int64_t n8 = hs.fp8_eps > 0.0 ? (int64_t)hsl[3] : 0;
-    tsvm_status st = hybrid_syrk_form(c, &hs, Z, w, nh, M, ev, n8);
-    if (st == TSVM_OK) {
-      TSVM_HIP(c, hipStreamSynchronize(s));
-      float ms = 0.f;
-      TSVM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
-      inf.n_fp64_rows = nh;
-      inf.bf16_bound = hsl[0];
-      inf.n_fp8_rows = n8;
-      inf.fp8_bound = hsl[2];
-      if (nh < n) {
-        inf.t_bf16 = ms * 1e-3;
-        if (ms > 0) inf.bf16_tflops = hybrid_syrk_flops(&hs) / (ms * 1e-3) * 1e-12;
-      }
+    const HybridCounts hc = hybrid_counts(&hs, hsl);
+    TSVM_CHECK(hybrid_syrk_form(c, &hs, Z, w, hc, M, ev));
+    TSVM_HIP(c, hipStreamSynchronize(s));
+    float ms = 0.f;
+    TSVM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    inf.n_fp64_rows = hc.nh;
+    inf.bf16_bound = hsl[0];
+    inf.n_fp8_rows = hc.n8;
+    inf.fp8_bound = hsl[2];
+    inf.n_x2_rows = hc.n2;
+    inf.x2_bound = hsl[4];
+    if (hc.nh < n) {
+      inf.t_bf16 = ms * 1e-3;
+      if (ms > 0) inf.bf16_tflops = hybrid_syrk_flops(&hs, hc) / (ms * 1e-3) * 1e-12;
     }
-    if (st != TSVM_OK) return st;
   }
   TSVM_HIP(c, hipStreamSynchronize(s));
   if (info) *info = inf;
