@@ -320,13 +320,15 @@ def main():
         log(f"fp64 reference train {dt64:.1f}s, accuracy {acc64:.4f}")
 
     # roofline of the dominant kernel.  BF16 form: both products of the factor run the
-    # same kernel (fp16 product), so its average launch is the mean of the two; the
-    # FP32 form likewise runs kprod_f32 twice.
+    # same kernel (fp16 product), so its average product is the mean of the two; the
+    # FP32 form likewise runs kprod_f32 twice.  The wide kernel runs a product as
+    # ceil(tiles / CUs) launches of one tile per CU (256-row x 256-column tiles): the
+    # figures are per product, the sum of its launches (HIP events on the library stream).
     flops = 2.0 * ntr * ntr * (d + k)
     t1, t2 = bf16_info["lra"]["t_product1"], bf16_info["lra"]["t_product2"]
     tr = KPROD_TRAFFIC
     if low:
-        peak, kname = PEAK_BF16, ("kprod_wide<1,8,64,EM_F16,8> (fused RBF product, 256-column "
+        peak, kname = PEAK_BF16, ("kprod_wide<8,64,EM_F16,8> (fused RBF product, 256-column "
                                   "tiles, K_Y Omega and K_Y W)")
     else:
         peak, kname = PEAK_FP32, "kprod_f32 (fused RBF product, K_Y Omega and K_Y W)"
@@ -337,8 +339,9 @@ def main():
                 "frac": round(achieved / peak, 4),
                 "traffic": (tr["fetch_bytes"] + tr["write_bytes"]
                             if tr["fetch_bytes"] is not None and same_config else None),
-                "kernel": kname, "flops_per_launch": flops, "launch_s": round(t_kp, 4),
-                "launches": {"K_Y Omega": round(t1, 4), "K_Y W": round(t2, 4)},
+                "kernel": kname, "flops_per_product": flops, "product_s": round(t_kp, 4),
+                "launches_per_product": (-(-(-(-ntr // 256) * -(-k // 256)) // 256) if low else 1),
+                "products": {"K_Y Omega": round(t1, 4), "K_Y W": round(t2, 4)},
                 "traffic_source": tr["source"] if same_config else None}
 
     grad = None if args.no_grad else grad_form(ctx, torch)

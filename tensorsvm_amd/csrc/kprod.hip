@@ -546,65 +546,75 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
 
 // -------------------------------------------------------------------------------------
 // Wide-tile low-precision kernel (right-hand sides wider than 128 columns).  WAVES waves
-// per block, one block per CU; wave w owns RB 32-row blocks and NT 32-column tiles of the
-// output (RB NT 16 accumulator registers).  Used as RB = 1, NT = 8, WAVES = 8: 128
-// accumulators, two waves per SIMD, a distance tile and its exp2 formed once per 256
-// columns (4 distance MFMAs per 16 product MFMAs; the 128-column kernel above needs 8 per
-// 16).  Wider tiles do not fit: at one wave per SIMD every MFMA result is allocated to the
-// 256 accumulation registers, which 256 product accumulators plus a distance tile exceed.
-// The wave pipelines one chunk deep: while the product MFMAs of chunk c run on E_c, it
-// issues the distance MFMAs of chunk c + 1 and turns their result into E_{c+1}.  Operands
-// come in MFMA fragment order (Qf, Af above): a chunk is one contiguous run in global
-// memory and in LDS, staging is a linear copy and every fragment read is a 16-byte read at
+// per block, one block per CU; wave w owns 32 rows and NT 32-column tiles of the output
+// (NT 16 accumulator registers).  Used as NT = 8, WAVES = 8: 128 accumulators, two waves
+// per SIMD, a distance tile and its exp2 formed once per 256 columns (4 distance MFMAs per
+// 16 product MFMAs; the 128-column kernel above needs 8 per 16).  Wider tiles do not fit:
+// at one wave per SIMD every MFMA result is allocated to the 256 accumulation registers,
+// which 256 product accumulators plus a distance tile exceed.  Operands come in MFMA
+// fragment order (Qf, Af above): a chunk of 32 j is one contiguous run in global memory
+// and in LDS, staging is a linear copy and every fragment read is a 16-byte read at
 // lane * 16 (no padding, no bank conflicts).  LDS ring of three chunk buffers: iteration c
-// reads A'_c and Q_{c+1} and writes chunk c + 2; one barrier per chunk.  The outputs are
-// bit-identical to the 128-column kernel's (same operands, same accumulation order).
+// reads A'_c and Q_{c+1} and writes chunk c + 2; one barrier per chunk.
+// The loop body is laid out by hand in groups fenced with sched_barrier, one product MFMA
+// per group (left to itself the scheduler serialised every product MFMA behind its own
+// B-fragment read and put the distance / exp2 work after the products):
+//  * B fragments (A'_c) come through a ring of four registers, read four MFMAs ahead; the
+//    first four of chunk c + 1 are read before the barrier that ends chunk c (its buffer
+//    was written two chunks earlier), so no product MFMA waits on a fresh LDS read;
+//  * the Q fragments of chunk c + 1 are read after the barrier and its distance MFMAs sit
+//    between the first product MFMAs, the exp2 / conversion of the next E tile between the
+//    last ones.
+// The block handles tiles tile0 + blockIdx.x, + gridDim.x, ... below ntiles; the host
+// launches one round of grid tiles at a time, so that the blocks of an XCD, which stream
+// the same chunks of A' and Q, start each tile together and share them in L2 (in one
+// persistent launch they drift apart over the rounds: at 1.6M x 1.6M x 512 the HBM fetch
+// grew 30-fold and the clock fell from 1.9 to 1.6 GHz).  The outputs are bit-identical to
+// the 128-column kernel's (same operands, same accumulation order).
 // -------------------------------------------------------------------------------------
-template <int RB, int NT, int KD, int EM, int WAVES>
+template <int NT, int KD, int EM, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __restrict__ P,
-                                                     const __bf16* __restrict__ Qf,
-                                                     const void* __restrict__ Afv,
-                                                     const float* __restrict__ y1,
-                                                     float* __restrict__ W, int64_t ldw,
-                                                     int64_t m, int64_t r, int64_t n_chunks,
-                                                     int64_t r_tiles, const float* __restrict__ sc,
-                                                     int64_t nxb, int64_t ntiles) {
+                                                      const __bf16* __restrict__ Qf,
+                                                      const void* __restrict__ Afv,
+                                                      const float* __restrict__ y1,
+                                                      float* __restrict__ W, int64_t ldw,
+                                                      int64_t m, int64_t r, int64_t n_chunks,
+                                                      int64_t r_tiles, const float* __restrict__ sc,
+                                                      int64_t nxb, int64_t ntiles, int64_t tile0) {
   using ET = typename std::conditional<EM == EM_BF16, __bf16, _Float16>::type;
   using ET8 = typename std::conditional<EM == EM_BF16, bf16x8_t, f16x8_t>::type;
   constexpr int KS = KD / 16;
-  constexpr int QB = 64 * KD;          // bytes of a chunk's Qf run
-  constexpr int AB = 2048 * NT;        // bytes of a chunk's Af run for NT column tiles
-  constexpr int CB = QB + AB;          // bytes per LDS chunk buffer
+  constexpr int NG = 2 * NT;            // product MFMAs (groups) per chunk
+  constexpr int QB = 64 * KD;
+  constexpr int AB = 2048 * NT;
+  constexpr int CB = QB + AB;
   constexpr int THR = 64 * WAVES;
-  constexpr int NV = (CB / 16 + THR - 1) / THR;  // 16-byte staging vectors per thread
-  constexpr int CBP = NV * THR * 16;   // buffer stride: every thread stores NV vectors
-  constexpr int BR = 32 * RB * WAVES;  // rows per block
+  constexpr int NV = (CB / 16 + THR - 1) / THR;
+  constexpr int CBP = NV * THR * 16;
+  constexpr int BR = 32 * WAVES;
+  constexpr int D0 = 2;                 // group of the first distance MFMA
+  constexpr int DS = D0 + 2 * KS <= NG - 4 ? 2 : 1;  // groups between distance MFMAs
+  constexpr int X0 = D0 + DS * KS;      // first group with exp2 work
+  constexpr int XPG = (16 + (NG - X0) - 1) / (NG - X0);  // exp2 per group
+  static_assert(KS >= 1 && X0 < NG, "distance MFMAs must fit before the exp2 groups");
   __shared__ __attribute__((aligned(16))) unsigned char lds[3 * CBP];
   const unsigned char* __restrict__ Af = (const unsigned char*)Afv;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int l32 = lane & 31, h = lane >> 5;
 
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t rowbase = (tile % nxb) * BR + wave * (32 * RB);
+  for (int64_t tile = tile0 + blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t rowbase = (tile % nxb) * BR + wave * 32;
     const int64_t ct0 = (tile / nxb) * NT;
-    // B operand of the distance MFMA: this wave's P rows, resident for the whole j loop
-    bf16x8_t pb[RB][KS];
+    bf16x8_t pb[KS];
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++)
+    for (int ks = 0; ks < KS; ks++)
+      pb[ks] = *(const bf16x8_t*)(P + (rowbase + l32) * KD + ks * 16 + 8 * h);
+    f32x16_t acc[NT];
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++)
-        pb[rb][ks] = *(const bf16x8_t*)(P + (rowbase + 32 * rb + l32) * KD + ks * 16 + 8 * h);
-    f32x16_t acc[RB][NT];
+    for (int ct = 0; ct < NT; ct++)
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-      for (int ct = 0; ct < NT; ct++)
-#pragma unroll
-        for (int q = 0; q < 16; q++) acc[rb][ct][q] = 0.0f;
-
-    // branch-free staging of chunk t: vector idx < QB/16 from Qf, the rest from Af (an
-    // out-of-range index re-reads the chunk's last vector and skips its store)
+      for (int q = 0; q < 16; q++) acc[ct][q] = 0.0f;
     u32x4_t stg[NV];
     auto gload = [&](int64_t t) {
       const u32x4_t* gq = (const u32x4_t*)((const unsigned char*)Qf + t * QB);
@@ -618,36 +628,12 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
         stg[v] = *(idx < QB / 16 ? pq : pa);
       }
     };
-    // unconditional (vectors past the chunk land in the buffer's padding), so that the
-    // stores stay in the loop body's one basic block
     auto sstore = [&](int buf) {
 #pragma unroll
       for (int v = 0; v < NV; v++) *(u32x4_t*)(lds + buf * CBP + (tid + v * THR) * 16) = stg[v];
     };
-    // distance tile S^T (32 j x 32 i) of the chunk in buffer buf, per row block
-    auto dist = [&](int buf, f32x16_t* sd) {
-      bf16x8_t qf[KS];
-#pragma unroll
-      for (int ks = 0; ks < KS; ks++) qf[ks] = *(const bf16x8_t*)(lds + buf * CBP + ks * 1024 + lane * 16);
-#pragma unroll
-      for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-        for (int q = 0; q < 16; q++) sd[rb][q] = 0.0f;
-#pragma unroll
-      for (int ks = 0; ks < KS; ks++)
-#pragma unroll
-        for (int rb = 0; rb < RB; rb++)
-          sd[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qf[ks], pb[rb][ks], sd[rb], 0, 0, 0);
-    };
-    // E = exp2(S^T) in the product's A-operand order: registers 8 kap .. 8 kap + 7
-    auto expo = [&](const f32x16_t* sd, ET8 (*e)[2]) {
-#pragma unroll
-      for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-        for (int kap = 0; kap < 2; kap++)
-#pragma unroll
-          for (int tt = 0; tt < 8; tt++) e[rb][kap][tt] = (ET)__builtin_amdgcn_exp2f(sd[rb][8 * kap + tt]);
-    };
+    auto bread = [&](int buf, int g) { return *(const ET8*)(lds + buf * CBP + QB + g * 1024 + lane * 16); };
+    auto qread = [&](int buf, int ks) { return *(const bf16x8_t*)(lds + buf * CBP + ks * 1024 + lane * 16); };
 
     const int64_t nc = n_chunks;
     gload(0);
@@ -655,79 +641,75 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
     gload(nc > 1 ? 1 : 0);
     sstore(1);
     __syncthreads();
-    ET8 e[RB][2];
+    ET8 e[2];
     {
-      f32x16_t s0[RB];
-      dist(0, s0);
-      expo(s0, e);
+      f32x16_t s0;
+#pragma unroll
+      for (int q = 0; q < 16; q++) s0[q] = 0.0f;
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) s0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qread(0, ks), pb[ks], s0, 0, 0, 0);
+#pragma unroll
+      for (int kap = 0; kap < 2; kap++)
+#pragma unroll
+        for (int tt = 0; tt < 8; tt++) e[kap][tt] = (ET)__builtin_amdgcn_exp2f(s0[8 * kap + tt]);
     }
-    int bc = 0;  // buffer of chunk c
+    ET8 b[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) b[i] = bread(0, i);
+    int bc = 0;
     for (int64_t c = 0; c < nc; c++) {
-      const int bn = bc == 2 ? 0 : bc + 1;  // chunk c + 1
-      const int bw = bn == 2 ? 0 : bn + 1;  // chunk c + 2
+      const int bn = bc == 2 ? 0 : bc + 1;
+      const int bw = bn == 2 ? 0 : bn + 1;
       gload(c + 2 < nc ? c + 2 : nc - 1);
-      f32x16_t sn[RB];
-      dist(bn, sn);  // the last iteration forms a spare tile from a copy of chunk nc - 1
-      // O += E_c . A'_c
+      bf16x8_t qf[KS];
 #pragma unroll
-      for (int idx = 0; idx < 2 * NT; idx++) {
-        const int ct = idx >> 1, kap = idx & 1;
-        const ET8 bv = *(const ET8*)(lds + bc * CBP + QB + idx * 1024 + lane * 16);
+      for (int ks = 0; ks < KS; ks++) qf[ks] = qread(bn, ks);
+      f32x16_t sd;
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-          if constexpr (EM == EM_BF16)
-            acc[rb][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[rb][kap], bv, acc[rb][ct], 0, 0, 0);
-          else
-            acc[rb][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[rb][kap], bv, acc[rb][ct], 0, 0, 0);
-        }
-      }
-      expo(sn, e);
-      sstore(bw);
-      // Issue order requested for the body: the staging loads, the Q fragments and four
-      // B fragments, the distance MFMAs (each with a B-fragment read), the product MFMAs
-      // (with the remaining B-fragment reads, and from the 7th on with three VALU
-      // instructions of the next tile's exp2 / fp16 conversion each), the stores.  What the
-      // compiler keeps of it is bounded by the 256 registers of a wave at two waves per
-      // SIMD; measured at 1.6M x 1.6M x 512 (fp16) against the body without the request:
-      // 2.85-2.97 s against 2.92-2.95 s (128-column kernel: 3.05 s).
-      constexpr int PF = 4;
-      __builtin_amdgcn_sched_group_barrier(0x020, NV, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, KS, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, PF, 0);
-#pragma unroll
-      for (int i = 0; i < KS * RB; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-      }
-#pragma unroll
-      for (int i = 0; i < 2 * NT * RB; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 2 * NT - KS * RB - PF) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        if (i >= 6) __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
-      }
-      __builtin_amdgcn_sched_group_barrier(0x200, NV, 0);
+      for (int q = 0; q < 16; q++) sd[q] = 0.0f;
+      ET8 en[2];
       __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int g = 0; g < NG; g++) {
+        const int ct = g >> 1, kap = g & 1;
+        if constexpr (EM == EM_BF16)
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[kap], b[g & 3], acc[ct], 0, 0, 0);
+        else
+          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[kap], b[g & 3], acc[ct], 0, 0, 0);
+        if (g + 4 < NG) b[g & 3] = bread(bc, g + 4);
+        if (g >= D0 && g < X0 && (g - D0) % DS == 0)
+          sd = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qf[(g - D0) / DS], pb[(g - D0) / DS], sd, 0, 0, 0);
+        if (g >= X0) {
+#pragma unroll
+          for (int u = 0; u < XPG; u++) {
+            const int q = (g - X0) * XPG + u;
+            if (q < 16) en[q >> 3][q & 7] = (ET)__builtin_amdgcn_exp2f(sd[q]);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      e[0] = en[0];
+      e[1] = en[1];
+      // the first B fragments of chunk c + 1 (buffer bn, complete since the last barrier)
+#pragma unroll
+      for (int i = 0; i < 4; i++) b[i] = bread(bn, i);
+      sstore(bw);
       __syncthreads();
       bc = bn;
     }
-    // epilogue: W[i, c] = y1_i O[i, c] (times the fp16 forms' inverse scale, exact);
-    // accumulator register q holds row 8 (q / 4) + 4 h + q % 4, column l32
     const float unscale = EM == EM_BF16 ? 1.0f : sc[1];
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
+    for (int q = 0; q < 16; q++) {
+      const int64_t i = rowbase + 8 * (q / 4) + 4 * h + (q % 4);
+      if (i >= m) continue;
+      const float yi = y1[i] * unscale;
 #pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int64_t i = rowbase + 32 * rb + 8 * (q / 4) + 4 * h + (q % 4);
-        if (i >= m) continue;
-        const float yi = y1[i] * unscale;
-#pragma unroll
-        for (int ct = 0; ct < NT; ct++) {
-          const int64_t cc = (ct0 + ct) * 32 + l32;
-          if (cc < r) W[i * ldw + cc] = yi * acc[rb][ct][q];
-        }
+      for (int ct = 0; ct < NT; ct++) {
+        const int64_t cc = (ct0 + ct) * 32 + l32;
+        if (cc < r) W[i * ldw + cc] = yi * acc[ct][q];
       }
     }
-    __syncthreads();  // the next tile's prologue overwrites buffers still being read
+    __syncthreads();
   }
 }
 
@@ -1093,19 +1075,23 @@ struct KpBuffers {
 // Widest operand (KD) of the wide-tile kernel
 #define KPW_MAX_KD 128
 
-template <int RB, int NT, int KD, int WAVES>
+template <int NT, int KD, int WAVES>
 static void launch_wide(int em, const __bf16* P, const __bf16* Qf, const void* Af,
                         const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
                         int64_t n_chunks, int64_t r_tiles, int64_t nxb, const float* sc,
                         int cu_count, hipStream_t s) {
   const int64_t ntiles = nxb * (r_tiles / NT);
-  dim3 grid((unsigned)(ntiles < cu_count ? ntiles : cu_count));
-  if (em == EM_BF16)
-    hipLaunchKernelGGL((kprod_wide<RB, NT, KD, EM_BF16, WAVES>), grid, dim3(64 * WAVES), 0, s, P,
-                       Qf, Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, ntiles);
-  else
-    hipLaunchKernelGGL((kprod_wide<RB, NT, KD, EM_F16, WAVES>), grid, dim3(64 * WAVES), 0, s, P,
-                       Qf, Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, ntiles);
+  // one launch per round of cu_count tiles (one block per CU)
+  for (int64_t t0 = 0; t0 < ntiles; t0 += cu_count) {
+    const int64_t t1 = t0 + cu_count < ntiles ? t0 + cu_count : ntiles;
+    const dim3 grid((unsigned)(t1 - t0));
+    if (em == EM_BF16)
+      hipLaunchKernelGGL((kprod_wide<NT, KD, EM_BF16, WAVES>), grid, dim3(64 * WAVES), 0, s, P, Qf,
+                         Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, t1, t0);
+    else
+      hipLaunchKernelGGL((kprod_wide<NT, KD, EM_F16, WAVES>), grid, dim3(64 * WAVES), 0, s, P, Qf,
+                         Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, t1, t0);
+  }
 }
 
 // The one-term low-precision forms with more than 128 right-hand-side columns.
@@ -1148,8 +1134,8 @@ static tsvm_status product_wide(Ctx* c, KpBuffers& bufs, int em, int KD, const f
   const __bf16 *Pb = (const __bf16*)P, *Qb = (const __bf16*)Qf;
 #define KPW_CASE(kd)                                                                            \
   case kd:                                                                                      \
-    launch_wide<1, NT, kd, WAVES>(em, Pb, Qb, Af, y1, W, ldw, m, r, n_chunks, r_tiles, nxb, sc, \
-                                  c->cu_count, s);                                              \
+    launch_wide<NT, kd, WAVES>(em, Pb, Qb, Af, y1, W, ldw, m, r, n_chunks, r_tiles, nxb, sc,    \
+                               c->cu_count, s);                                                 \
     break;
   switch (KD) {
     KPW_CASE(16)
