@@ -140,6 +140,16 @@ typedef struct {
 tsvm_status tsvm_lra(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
                      int64_t k, double gamma, int precision, const float *omega, float *U,
                      tsvm_lra_info *info);
+/* The reference's own types: double LRA(float *Z, int ldz, double *U, int ldu, long n,
+ * long k) (tensorsvm-train.cu:111, 1941-2249, called at 960) writes U as double with
+ * leading dimension ldu and returns the largest eigenvalue kept (w[k-1]); gamma and the
+ * labels, globals there, are arguments here.  X: n rows of d features, leading dimension
+ * ldx >= d; U: n x k FP64 with ldu >= k (host).  U holds the FP32 factor widened (the
+ * reference's U = W X is an FP32 GEMM, 2231-2233, widened the same way).  *l1 (may be
+ * NULL) = the largest eigenvalue kept. */
+tsvm_status tsvm_lra_f64(tsvm_ctx *ctx, const float *X, int64_t ldx, const float *y, int64_t n,
+                         int64_t d, int64_t k, double gamma, int precision, const float *omega,
+                         double *U, int64_t ldu, double *l1, tsvm_lra_info *info);
 /* Device-pointer form: X, y, omega (n x k, consumed: used as workspace) and U are
  * device pointers; synchronised on return. */
 tsvm_status tsvm_lra_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
@@ -277,6 +287,15 @@ void tsvm_ipm_default_params(tsvm_ipm_params *p);
  * a: labels (+-1).  alpha (= X), xi, s: n FP64 outputs (s may be NULL). */
 tsvm_status tsvm_ipm(tsvm_ctx *ctx, const float *Z, const float *a, int64_t n, int64_t k,
                      const tsvm_ipm_params *prm, double *alpha, double *xi, double *s,
+                     tsvm_ipm_info *info);
+/* The reference's own types: void mpc(double *Z, double *a, double C, double *X,
+ * double *Xi, int n, int k) (tensorsvm-train.cu:100, 1507-1785).  Z: n x k FP64 with
+ * leading dimension ldz >= k, a: n FP64 labels, X (= alpha) and Xi: n FP64 outputs (host).
+ * Z is narrowed to the solver's FP32 factor, which is exact for a factor from
+ * tsvm_lra_f64 or the reference's LRA (FP32 values widened) and a 2^-24 relative rounding
+ * otherwise.  prm: NULL for the defaults (tsvm_ipm_default_params); C overrides prm->C. */
+tsvm_status tsvm_mpc(tsvm_ctx *ctx, const double *Z, int64_t ldz, const double *a, double C,
+                     double *X, double *Xi, int64_t n, int64_t k, const tsvm_ipm_params *prm,
                      tsvm_ipm_info *info);
 /* Device-pointer form (Z, a, alpha, xi, s on the device; s may be NULL). */
 tsvm_status tsvm_ipm_dev(tsvm_ctx *ctx, const float *Z, const float *a, int64_t n, int64_t k,

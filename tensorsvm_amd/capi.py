@@ -174,6 +174,10 @@ _EXTRA = [
                       ctypes.POINTER(IpmInfo)], None),
     ("tsvm_lra", [_P, _P, _P, _I64, _I64, _I64, _D, _INT, _P, _P, ctypes.POINTER(LraInfo)], None),
     ("tsvm_lra_dev", [_P, _P, _P, _I64, _I64, _I64, _D, _INT, _P, _P, ctypes.POINTER(LraInfo)], None),
+    ("tsvm_lra_f64", [_P, _P, _I64, _P, _I64, _I64, _I64, _D, _INT, _P, _P, _I64, ctypes.POINTER(_D),
+                      ctypes.POINTER(LraInfo)], None),
+    ("tsvm_mpc", [_P, _P, _I64, _P, _D, _P, _P, _I64, _I64, ctypes.POINTER(IpmParams),
+                  ctypes.POINTER(IpmInfo)], None),
     ("tsvm_weighted_syrk_dev", [_P, _P, _I64, _I64, _P, _INT, _D, _P, ctypes.POINTER(SyrkInfo)], None),
     ("tsvm_read_model", [ctypes.c_char_p, ctypes.POINTER(_P)], None),
     ("tsvm_model_free", [_P], 0),
@@ -356,6 +360,35 @@ class Context:
         self._check(self.lib.tsvm_lra(self.h, _ptr(X), _ptr(y), n, d, k, gamma, precision,
                                       _ptr(om), _ptr(U), ctypes.byref(info)), "tsvm_lra")
         return U, info
+
+    def lra_f64(self, X, y, k, gamma, precision=PREC_FP32, omega=None, ldu=None):
+        """tsvm_lra_f64 (the reference's LRA types): returns (U (n x ldu float64, the first k
+        columns written), l1, info)."""
+        X, y = _f32(X), _f32(y)
+        n, d = X.shape
+        ldu = k if ldu is None else ldu
+        om = None if omega is None else _f32(omega).reshape(n, k)
+        U = np.zeros((n, ldu))
+        l1 = _D()
+        info = LraInfo()
+        self._check(self.lib.tsvm_lra_f64(self.h, _ptr(X), d, _ptr(y), n, d, k, gamma, precision,
+                                          _ptr(om), _ptr(U), ldu, ctypes.byref(l1),
+                                          ctypes.byref(info)), "tsvm_lra_f64")
+        return U, l1.value, info
+
+    def mpc(self, Z, a, C, k=None, prm=None):
+        """tsvm_mpc (the reference's mpc types): Z (n x ldz float64, the first k columns
+        used), a (n float64), prm an IpmParams or None; returns (X, Xi, info)."""
+        Z = np.ascontiguousarray(Z, dtype=np.float64)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n, ldz = Z.shape
+        k = ldz if k is None else k
+        X, Xi = np.empty(n), np.empty(n)
+        info = IpmInfo()
+        self._check(self.lib.tsvm_mpc(self.h, _ptr(Z), ldz, _ptr(a), C, _ptr(X), _ptr(Xi), n, k,
+                                      ctypes.byref(prm) if prm is not None else None,
+                                      ctypes.byref(info)), "tsvm_mpc")
+        return X, Xi, info
 
     def lra_dev(self, X, y, n, d, k, gamma, precision, omega, U):
         info = LraInfo()

@@ -1451,4 +1451,22 @@ tsvm_status tsvm_ipm(tsvm_ctx* ctx, const float* Z, const float* a, int64_t n, i
   return TSVM_OK;
 }
 
+tsvm_status tsvm_mpc(tsvm_ctx* ctx, const double* Z, int64_t ldz, const double* a, double C,
+                     double* X, double* Xi, int64_t n, int64_t k, const tsvm_ipm_params* prm,
+                     tsvm_ipm_info* info) {
+  if (!ctx || !Z || !a || !X || !Xi || n <= 0 || k <= 0 || ldz < k) return TSVM_ERR_ARG;
+  tsvm_ipm_params p;
+  if (prm)
+    p = *prm;
+  else
+    tsvm_ipm_default_params(&p);
+  p.C = C;
+  std::vector<float> Zf(n * k), af(n);
+  for (int64_t i = 0; i < n; i++) {
+    for (int64_t j = 0; j < k; j++) Zf[i * k + j] = (float)Z[i * ldz + j];
+    af[i] = (float)a[i];
+  }
+  return tsvm_ipm(ctx, Zf.data(), af.data(), n, k, &p, X, Xi, nullptr, info);
+}
+
 }  // extern "C"

@@ -337,4 +337,27 @@ tsvm_status tsvm_lra(tsvm_ctx* ctx, const float* X, const float* y, int64_t n, i
   return TSVM_OK;
 }
 
+tsvm_status tsvm_lra_f64(tsvm_ctx* ctx, const float* X, int64_t ldx, const float* y, int64_t n,
+                         int64_t d, int64_t k, double gamma, int precision, const float* omega,
+                         double* U, int64_t ldu, double* l1, tsvm_lra_info* info) {
+  if (!ctx || !X || !y || !U || n <= 0 || d <= 0 || k <= 0 || ldx < d || ldu < k)
+    return TSVM_ERR_ARG;
+  std::vector<float> packed;
+  const float* Xp = X;
+  if (ldx != d) {
+    packed.resize(n * d);
+    for (int64_t i = 0; i < n; i++)
+      std::copy(X + i * ldx, X + i * ldx + d, packed.begin() + i * d);
+    Xp = packed.data();
+  }
+  std::vector<float> Uf(n * k);
+  tsvm_lra_info inf;
+  TSVM_CHECK(tsvm_lra(ctx, Xp, y, n, d, k, gamma, precision, omega, Uf.data(), &inf));
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = 0; j < k; j++) U[i * ldu + j] = (double)Uf[i * k + j];
+  if (l1) *l1 = inf.eig_max;
+  if (info) *info = inf;
+  return TSVM_OK;
+}
+
 }  // extern "C"

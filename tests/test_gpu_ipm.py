@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from datagen import gaussian_mixture
 
 pytestmark = pytest.mark.gpu
 
@@ -91,3 +92,23 @@ def test_ipm_feasible_init_same_optimum(ctx, n, k, C):
     assert i0.reason == 0 and i1.reason == 0
     assert abs(i0.pobj - i1.pobj) <= 1e-6 * (1 + abs(i0.pobj))
     assert np.abs(al0 - al1).max() <= 1e-3 * C
+
+
+def test_reference_typed_entry_points(ctx):
+    """tsvm_lra_f64 / tsvm_mpc take the reference's types (double U with a leading
+    dimension, LRA's return value; double Z and labels for mpc, tensorsvm-train.cu:100,
+    111): the same factor widened exactly, the largest eigenvalue as l1, and the same
+    solve as the FP32-factor entry points, bit for bit."""
+    import tensorsvm_amd as T
+    X, y = gaussian_mixture(3000, 6, 41)
+    n, k, gamma, C = 3000, 40, 0.1, 4.0
+    om = O.gaussrand(n * k)
+    U32, i32 = ctx.lra(X, y, k, gamma, T.PREC_FP32, omega=om)
+    U64, l1, i64 = ctx.lra_f64(X, y, k, gamma, T.PREC_FP32, omega=om, ldu=k + 3)
+    assert np.array_equal(U64[:, :k], U32.astype(np.float64))
+    assert not U64[:, k:].any()  # beyond k columns of each row: untouched
+    assert l1 == i32.eig_max == i64.eig_max
+    al, xi, _, ii = ctx.ipm(U32, y, C)
+    X64, Xi64, mi = ctx.mpc(U64, y.astype(np.float64), C, k=k)
+    assert mi.iters == ii.iters and mi.reason == ii.reason
+    assert np.array_equal(X64, al) and np.array_equal(Xi64, xi)
