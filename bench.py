@@ -101,6 +101,7 @@ def grad_form(ctx, torch, n=200_000, nt=50_000, d=32, k=256, C=1.0, seed=1003):
     return {"workload": f"scaled-variance mixture n={n} train + {nt} held-out, d={d}, k={k}, FP32",
             "t_train_s": round(t_train, 3), "t_grad_s": round(gi.t_total, 3),
             "t_backward_s": round(gi.t_backward, 4), "backward_tflops": round(tf, 2),
+            "t_backward_heldout_s": round(gi.t_backward_heldout, 4),
             "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": PEAK_FP32,
                          "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32, 4)},
             "dual": D, "heldout_loss": L,

@@ -193,8 +193,9 @@ tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int
  *             (1 - z)^2 / 2 (0 < z < 1), 0 (z >= 1)
  * g_dual, g_loss (d + 1 each, host; component 0 = d/dlog gamma) may be NULL.  The
  * backward pass recomputes kernel tiles in the fused FP32 product over derivative
- * right-hand sides; nothing of size n^2 is stored.  The held-out decision values behind
- * L come from the FP64 product, so that differences of L resolve its derivative.  precision: the forward pass's two
+ * right-hand sides; nothing of size n^2 is stored.  The held-out part (the decision values
+ * behind L and the columns of its derivative) runs on the FP64 product: its per-feature
+ * sums cancel to a small remainder, and differences of L must resolve its derivative.  precision: the forward pass's two
  * products of the factor (K_Y Omega, K_Y W) in TSVM_PREC_FP32, or in the fp16 product
  * with TSVM_PREC_BF16 (about 8x faster at large n; the backward stays FP32).  Device: X
  * (n x d), y, alpha (FP64), omega (n x k, not modified), Xt (nt x d), yt; host:
@@ -202,11 +203,13 @@ tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int
 typedef struct {
   double t_forward;       /* s: scaled features, factor and dual value */
   double t_backward;      /* s in the backward products and reductions (HIP events) */
-  double backward_flops;  /* 2 n^2 (d + 3 (2d+1)) + 2 nt n (d + 2d+1) */
-  double backward_tflops;
+  double backward_flops;  /* the dual's FP32 backward product, 2 n^2 (d + 3 (2d+1)) */
+  double backward_tflops; /* backward_flops / the time of that product and its reductions */
   double t_total;
   int64_t realk;
   int orth_passes;
+  double t_backward_heldout; /* s of t_backward in the held-out part (FP64 product, 2 nt n
+                                (d + 2d+1) flops, and its reduction) */
 } tsvm_grad_info;
 tsvm_status tsvm_grad_dev(tsvm_ctx *ctx, const float *X, const float *y, int64_t n, int64_t d,
                           const double *alpha, double b, int64_t k, double log_gamma,

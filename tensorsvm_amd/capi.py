@@ -119,7 +119,8 @@ class SyrkInfo(ctypes.Structure):
 
 class GradInfo(ctypes.Structure):
     _fields_ = [("t_forward", _D), ("t_backward", _D), ("backward_flops", _D),
-                ("backward_tflops", _D), ("t_total", _D), ("realk", _I64), ("orth_passes", _INT)]
+                ("backward_tflops", _D), ("t_total", _D), ("realk", _I64), ("orth_passes", _INT),
+                ("t_backward_heldout", _D)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
