@@ -823,14 +823,17 @@ __global__ __launch_bounds__(256, 1) void syrk_fp8_256_kernel(const unsigned cha
 // Compensated rows: y_i = sqrt(w_i) z_i in FP64, hi = bf16(y), lo = bf16(y - hi), so that
 // y_a y_b = hi_a hi_b + hi_a lo_b + lo_a hi_b up to about 3 2^-18 |y_a| |y_b|.  H and L
 // hold hi and lo of the listed rows rows2[0..m2) (compacted, zero padded to m2p rows),
-// chunked like ZbT.  Block: 64 compacted rows x 64 columns through an LDS transpose
-// (rows read 256 bytes at a time, chunk columns written 128 bytes at a time).
+// chunked like ZbT.  Block: 64 compacted rows x 64 columns; rows read as float4 (16 lanes
+// per 256-byte row segment), hi / lo transposed through LDS, chunk columns written as
+// 16-byte pieces (8 rows each).
+constexpr int PX_LS = 72;  // LDS row stride (bf16) of the transposed tiles
 __global__ __launch_bounds__(256) void pack_x2_kernel(const float* __restrict__ Z, int kp,
                                                       int kpt, const double* __restrict__ w,
                                                       const int* __restrict__ rows2, int64_t m2,
                                                       __bf16* __restrict__ H,
                                                       __bf16* __restrict__ L) {
-  __shared__ double tile[64][65];
+  __shared__ __attribute__((aligned(16))) __bf16 th[64 * PX_LS];
+  __shared__ __attribute__((aligned(16))) __bf16 tl[64 * PX_LS];
   __shared__ double sw[64];
   __shared__ int src[64];
   const int64_t q = blockIdx.x;
@@ -843,24 +846,36 @@ __global__ __launch_bounds__(256) void pack_x2_kernel(const float* __restrict__ 
     sw[t] = i >= 0 ? sqrt(w[i]) : 0.0;
   }
   __syncthreads();
-#pragma unroll 4
-  for (int p = 0; p < 16; p++) {
-    const int e = p * 256 + t, r = e >> 6, cc = e & 63;
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    const int idx = p * 256 + t, r = idx >> 4, c4 = (idx & 15) * 4;
     const int i = src[r];
-    double v = 0.0;
-    if (i >= 0 && c0 + cc < kp) v = sw[r] * (double)Z[(int64_t)i * kp + c0 + cc];
-    tile[r][cc] = v;
+    float z[4] = {0.f, 0.f, 0.f, 0.f};
+    if (i >= 0) {
+      const float* zr = Z + (int64_t)i * kp + c0 + c4;
+      if (c0 + c4 + 4 <= kp && (kp & 3) == 0) {
+        const f32x4_t v = *(const f32x4_t*)zr;
+        z[0] = v[0], z[1] = v[1], z[2] = v[2], z[3] = v[3];
+      } else {
+        for (int u = 0; u < 4; u++)
+          if (c0 + c4 + u < kp) z[u] = zr[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double y = sw[r] * (double)z[u];
+      const __bf16 hi = (__bf16)(float)y;
+      th[(c4 + u) * PX_LS + r] = hi;
+      tl[(c4 + u) * PX_LS + r] = (__bf16)(float)(y - (double)(float)hi);
+    }
   }
   __syncthreads();
-#pragma unroll 4
-  for (int p = 0; p < 16; p++) {
-    const int e = p * 256 + t, cc = e >> 6, r = e & 63;
-    const double y = tile[r][cc];
-    const __bf16 hi = (__bf16)(float)y;
-    const __bf16 lo = (__bf16)(float)(y - (double)(float)hi);
-    const int64_t o = (q * kpt + c0 + cc) * SY_BK + r;
-    H[o] = hi;
-    L[o] = lo;
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    const int idx = p * 256 + t, cc = idx >> 3, r8 = (idx & 7) * 8;
+    const int64_t o = (q * kpt + c0 + cc) * SY_BK + r8;
+    *(u32x4_t*)(H + o) = *(const u32x4_t*)&th[cc * PX_LS + r8];
+    *(u32x4_t*)(L + o) = *(const u32x4_t*)&tl[cc * PX_LS + r8];
   }
 }
 

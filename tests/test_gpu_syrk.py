@@ -165,7 +165,7 @@ def test_fp8_pass_all_rows(ctx):
     assert 1e-5 < rel < 2.0 ** -4, rel
 
 
-@pytest.mark.parametrize("n,k", [(20000, 512), (12345, 300)])
+@pytest.mark.parametrize("n,k", [(20000, 512), (12345, 300), (9001, 299)])
 def test_compensated_tier_all_rows(ctx, n, k):
     """Weights with t_i ~ 1..10 everywhere: too large for the BF16 pass to take more than a
     few rows, small enough for the compensated tier to take them all, so no dense pass runs
