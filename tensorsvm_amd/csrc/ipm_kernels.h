@@ -39,9 +39,10 @@ enum IpmSlot {
   SL_HS8N,       // hybrid capacitance: number of FP8 rows (must follow SL_HS8B)
   SL_HS2B,       // hybrid capacitance: bound of the compensated rows (must follow SL_HS8N)
   SL_HS2N,       // hybrid capacitance: number of compensated rows (must follow SL_HS2B)
-  IPM_NSLOT = 32
+  IPM_NSLOT      // number of slots (32: one 256-byte copy to the host per iteration)
 };
 
+static_assert(IPM_NSLOT == 32, "the scalar array is one 256-byte copy per iteration");
 enum RedOp { RED_SUM = 0, RED_MAX = 1, RED_MIN = 2 };
 
 // T (p x kp) = Z^T V with V p vectors of length n (row q at V + q*ldv); Z n x kp FP32.
