@@ -330,8 +330,8 @@ def main():
     t1, t2 = bf16_info["lra"]["t_product1"], bf16_info["lra"]["t_product2"]
     tr = KPROD_TRAFFIC
     if low:
-        peak, kname = PEAK_BF16, ("kprod_wide<8,64,EM_F16,8> (fused RBF product, 256-column "
-                                  "tiles, K_Y Omega and K_Y W)")
+        peak, kname = PEAK_BF16, ("kprod_wide<8,64,EM_F16,4,2> (fused RBF product, 256-row x "
+                                  "256-column tiles, 64 x 256 per wave, K_Y Omega and K_Y W)")
     else:
         peak, kname = PEAK_FP32, "kprod_f32 (fused RBF product, K_Y Omega and K_Y W)"
     t_kp = 0.5 * (t1 + t2)

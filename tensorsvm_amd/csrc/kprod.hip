@@ -544,27 +544,65 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
   }  // tile
 }
 
+// The distance MFMA of the wide-tile kernel with its result in architectural VGPRs.  The
+// kernel's 256 product accumulators fill the accumulation registers; the compiler would
+// give this MFMA an accumulation register too and move a product tile out and back around
+// it (96 register moves per 32-j chunk).  Its inputs come from LDS reads and from registers
+// loaded once per tile (never from a VALU write just before), its result is read by v_exp
+// only after several later MFMAs (the loop body's groups are fenced with sched_barrier), and
+// back-to-back accumulation into the same registers needs no wait states, as for the
+// compiler's own MFMA chains.
+template <bool LAST = false>
+__device__ __forceinline__ f32x16_t mfma_bf16_vgpr_first(bf16x8_t a, bf16x8_t b) {
+  f32x16_t d;
+  if constexpr (LAST)
+    asm volatile("v_mfma_f32_32x32x16_bf16 %0, %1, %2, 0\n\ts_nop 7\n\ts_nop 1"
+                 : "=&v"(d) : "v"(a), "v"(b));
+  else
+    asm volatile("v_mfma_f32_32x32x16_bf16 %0, %1, %2, 0" : "=&v"(d) : "v"(a), "v"(b));
+  return d;
+}
+__device__ __forceinline__ void mfma_bf16_vgpr_acc(f32x16_t& d, bf16x8_t a, bf16x8_t b) {
+  asm volatile("v_mfma_f32_32x32x16_bf16 %0, %1, %2, %0" : "+v"(d) : "v"(a), "v"(b));
+}
+// the last MFMA of a chain whose result VALU instructions read next: ten wait states
+// after it (an 8-pass XDL result read by VALU; the MFMA pipe is busy meanwhile anyway)
+__device__ __forceinline__ void mfma_bf16_vgpr_last(f32x16_t& d, bf16x8_t a, bf16x8_t b) {
+  asm volatile("v_mfma_f32_32x32x16_bf16 %0, %1, %2, %0\n\ts_nop 7\n\ts_nop 1"
+               : "+v"(d) : "v"(a), "v"(b));
+}
+
 // -------------------------------------------------------------------------------------
 // Wide-tile low-precision kernel (right-hand sides wider than 128 columns).  WAVES waves
-// per block, one block per CU; wave w owns 32 rows and NT 32-column tiles of the output
-// (NT 16 accumulator registers).  Used as NT = 8, WAVES = 8: 128 accumulators, two waves
-// per SIMD, a distance tile and its exp2 formed once per 256 columns (4 distance MFMAs per
-// 16 product MFMAs; the 128-column kernel above needs 8 per 16).  Wider tiles do not fit:
-// at one wave per SIMD every MFMA result is allocated to the 256 accumulation registers,
-// which 256 product accumulators plus a distance tile exceed.  Operands come in MFMA
-// fragment order (Qf, Af above): a chunk of 32 j is one contiguous run in global memory
-// and in LDS, staging is a linear copy and every fragment read is a 16-byte read at
-// lane * 16 (no padding, no bank conflicts).  LDS ring of three chunk buffers: iteration c
-// reads A'_c and Q_{c+1} and writes chunk c + 2; one barrier per chunk.
-// The loop body is laid out by hand in groups fenced with sched_barrier, one product MFMA
-// per group (left to itself the scheduler serialised every product MFMA behind its own
-// B-fragment read and put the distance / exp2 work after the products):
-//  * B fragments (A'_c) come through a ring of four registers, read four MFMAs ahead; the
+// per block, one block per CU; wave w owns RB 32-row blocks and NT 32-column tiles of the
+// output (RB NT 16 accumulator registers).  Used as NT = 8, WAVES = 4, RB = 2: one wave per
+// SIMD holding 64 rows x 256 columns in all 256 accumulation registers, so that every B
+// fragment read from LDS feeds two MFMAs (with 32 rows per wave each product MFMA needed
+// its own 1 KB fragment), and a distance tile and its exp2 are formed once per 256
+// columns (8 distance MFMAs per 32 product MFMAs).  The distance MFMAs write
+// architectural registers (mfma_bf16_vgpr_*): with the product tiles filling the
+// accumulation registers the compiler would otherwise shuttle a product tile out and back
+// around each of them.  Operands come in MFMA fragment order (Qf, Af above): a chunk of
+// 32 j is one contiguous run in global memory and in LDS, staging is a linear copy and
+// every fragment read is a 16-byte read at lane * 16 (no padding, no bank conflicts).
+// LDS ring of three chunk buffers: iteration c reads A'_c and Q_{c+1} and writes chunk
+// c + 2; one barrier per chunk.
+// The loop body is laid out by hand in groups fenced with sched_barrier, the two row
+// blocks' product MFMAs for one B fragment per group:
+//  * the groups take K step 0 of the eight column tiles, then K step 1, so the two updates
+//    of an accumulator within a chunk are eight groups apart (back to back, each waited
+//    for the other);
+//  * B fragments (A'_c) come through a ring of four registers, read four groups ahead; the
 //    first four of chunk c + 1 are read before the barrier that ends chunk c (its buffer
 //    was written two chunks earlier), so no product MFMA waits on a fresh LDS read;
-//  * the Q fragments of chunk c + 1 are read after the barrier and its distance MFMAs sit
-//    between the first product MFMAs, the exp2 / conversion of the next E tile between the
-//    last ones.
+//  * the Q fragments of chunk c + 1 are read after the barrier, its distance MFMAs sit in
+//    the first groups and the exp2 / conversion of the next E tile (v_exp_f32 issues at a
+//    quarter of the VALU rate) is spread over the rest.
+// Measured at 400k x 400k x 512 on MI355X: the MFMA pipe is busy 58% of the cycles at
+// 1.91 GHz.  The chip's power limit bounds what a better schedule can win: with the global
+// loads and LDS stores taken out (a timing copy, wrong results) the pipe is busy 75% but
+// the clock falls to 1.72 GHz (17% more MFMA work per second), and a register-only loop of
+// fp16 MFMAs on random operands runs at 1.64 PF/s, 66% of the 2.5 PF/s dense peak.
 // The block handles tiles tile0 + blockIdx.x, + gridDim.x, ... below ntiles; the host
 // launches one round of grid tiles at a time, so that the blocks of an XCD, which stream
 // the same chunks of A' and Q, start each tile together and share them in L2 (in one
@@ -572,7 +610,7 @@ __global__ __launch_bounds__(THR, OCC) void kprod_bf16(const __bf16* __restrict_
 // grew 30-fold and the clock fell from 1.9 to 1.6 GHz).  The outputs are bit-identical to
 // the 128-column kernel's (same operands, same accumulation order).
 // -------------------------------------------------------------------------------------
-template <int NT, int KD, int EM, int WAVES>
+template <int NT, int KD, int EM, int WAVES, int RB>
 __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __restrict__ P,
                                                       const __bf16* __restrict__ Qf,
                                                       const void* __restrict__ Afv,
@@ -584,18 +622,21 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
   using ET = typename std::conditional<EM == EM_BF16, __bf16, _Float16>::type;
   using ET8 = typename std::conditional<EM == EM_BF16, bf16x8_t, f16x8_t>::type;
   constexpr int KS = KD / 16;
-  constexpr int NG = 2 * NT;            // product MFMAs (groups) per chunk
+  constexpr int NG = 2 * NT;            // B fragments (groups) per chunk
+  constexpr int NE = RB * 16;           // exp2 per lane per chunk
   constexpr int QB = 64 * KD;
   constexpr int AB = 2048 * NT;
   constexpr int CB = QB + AB;
   constexpr int THR = 64 * WAVES;
   constexpr int NV = (CB / 16 + THR - 1) / THR;
   constexpr int CBP = NV * THR * 16;
-  constexpr int BR = 32 * WAVES;
-  constexpr int D0 = 2;                 // group of the first distance MFMA
-  constexpr int DS = D0 + 2 * KS <= NG - 4 ? 2 : 1;  // groups between distance MFMAs
-  constexpr int X0 = D0 + DS * KS;      // first group with exp2 work
-  constexpr int XPG = (16 + (NG - X0) - 1) / (NG - X0);  // exp2 per group
+  constexpr int BR = 32 * RB * WAVES;
+  // Groups D0 .. D0 + KS - 1 carry the distance MFMAs of the next chunk (one K step of
+  // every row block per group), the groups from X0 on its exp2 work (v_exp_f32 issues at a
+  // quarter of the VALU rate: three per group of two product MFMAs keep the MFMA pipe fed)
+  constexpr int D0 = 1;
+  constexpr int X0 = D0 + KS;
+  constexpr int XPG = (NE + (NG - X0) - 1) / (NG - X0);  // exp2 per group
   static_assert(KS >= 1 && X0 < NG, "distance MFMAs must fit before the exp2 groups");
   __shared__ __attribute__((aligned(16))) unsigned char lds[3 * CBP];
   const unsigned char* __restrict__ Af = (const unsigned char*)Afv;
@@ -604,17 +645,21 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
   const int l32 = lane & 31, h = lane >> 5;
 
   for (int64_t tile = tile0 + blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t rowbase = (tile % nxb) * BR + wave * 32;
+    const int64_t rowbase = (tile % nxb) * BR + wave * 32 * RB;
     const int64_t ct0 = (tile / nxb) * NT;
-    bf16x8_t pb[KS];
+    bf16x8_t pb[RB][KS];
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++)
-      pb[ks] = *(const bf16x8_t*)(P + (rowbase + l32) * KD + ks * 16 + 8 * h);
-    f32x16_t acc[NT];
+    for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-    for (int ct = 0; ct < NT; ct++)
+      for (int ks = 0; ks < KS; ks++)
+        pb[rb][ks] = *(const bf16x8_t*)(P + (rowbase + 32 * rb + l32) * KD + ks * 16 + 8 * h);
+    f32x16_t acc[RB][NT];
 #pragma unroll
-      for (int q = 0; q < 16; q++) acc[ct][q] = 0.0f;
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+      for (int ct = 0; ct < NT; ct++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc[rb][ct][q] = 0.0f;
     u32x4_t stg[NV];
     auto gload = [&](int64_t t) {
       const u32x4_t* gq = (const u32x4_t*)((const unsigned char*)Qf + t * QB);
@@ -641,21 +686,25 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
     gload(nc > 1 ? 1 : 0);
     sstore(1);
     __syncthreads();
-    ET8 e[2];
-    {
+    ET8 e[RB][2];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
       f32x16_t s0;
 #pragma unroll
       for (int q = 0; q < 16; q++) s0[q] = 0.0f;
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++) s0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qread(0, ks), pb[ks], s0, 0, 0, 0);
+      for (int ks = 0; ks < KS; ks++)
+        s0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qread(0, ks), pb[rb][ks], s0, 0, 0, 0);
 #pragma unroll
       for (int kap = 0; kap < 2; kap++)
 #pragma unroll
-        for (int tt = 0; tt < 8; tt++) e[kap][tt] = (ET)__builtin_amdgcn_exp2f(s0[8 * kap + tt]);
+        for (int tt = 0; tt < 8; tt++) e[rb][kap][tt] = (ET)__builtin_amdgcn_exp2f(s0[8 * kap + tt]);
     }
+    // B fragment of group g: column tile g % NT, K step g / NT (fragment order: ct, kap)
+    auto bfrag = [](int g) { return (g % NT) * 2 + g / NT; };
     ET8 b[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) b[i] = bread(0, i);
+    for (int i = 0; i < 4; i++) b[i] = bread(0, bfrag(i));
     int bc = 0;
     for (int64_t c = 0; c < nc; c++) {
       const int bn = bc == 2 ? 0 : bc + 1;
@@ -664,51 +713,68 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
       bf16x8_t qf[KS];
 #pragma unroll
       for (int ks = 0; ks < KS; ks++) qf[ks] = qread(bn, ks);
-      f32x16_t sd;
-#pragma unroll
-      for (int q = 0; q < 16; q++) sd[q] = 0.0f;
-      ET8 en[2];
+      f32x16_t sd[RB];
+      ET8 en[RB][2];
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int g = 0; g < NG; g++) {
-        const int ct = g >> 1, kap = g & 1;
-        if constexpr (EM == EM_BF16)
-          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[kap], b[g & 3], acc[ct], 0, 0, 0);
-        else
-          acc[ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[kap], b[g & 3], acc[ct], 0, 0, 0);
-        if (g + 4 < NG) b[g & 3] = bread(bc, g + 4);
-        if (g >= D0 && g < X0 && (g - D0) % DS == 0)
-          sd = __builtin_amdgcn_mfma_f32_32x32x16_bf16(qf[(g - D0) / DS], pb[(g - D0) / DS], sd, 0, 0, 0);
+        // K step kap of every column tile, then the other: the two updates of one
+        // accumulator are NT groups apart (back to back they wait on each other)
+        const int ct = g % NT, kap = g / NT;
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+          if constexpr (EM == EM_BF16)
+            acc[rb][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(e[rb][kap], b[g & 3], acc[rb][ct], 0, 0, 0);
+          else
+            acc[rb][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(e[rb][kap], b[g & 3], acc[rb][ct], 0, 0, 0);
+        }
+        if (g + 4 < NG) b[g & 3] = bread(bc, bfrag(g + 4));
+        if (g >= D0 && g < X0) {
+#pragma unroll
+          for (int rb = 0; rb < RB; rb++) {
+            if (g == D0)
+              sd[rb] = mfma_bf16_vgpr_first<KS == 1>(qf[0], pb[rb][0]);
+            else if (g < X0 - 1)
+              mfma_bf16_vgpr_acc(sd[rb], qf[g - D0], pb[rb][g - D0]);
+            else
+              mfma_bf16_vgpr_last(sd[rb], qf[g - D0], pb[rb][g - D0]);
+          }
+        }
         if (g >= X0) {
 #pragma unroll
           for (int u = 0; u < XPG; u++) {
             const int q = (g - X0) * XPG + u;
-            if (q < 16) en[q >> 3][q & 7] = (ET)__builtin_amdgcn_exp2f(sd[q]);
+            if (q < NE) en[q >> 4][(q >> 3) & 1][q & 7] = (ET)__builtin_amdgcn_exp2f(sd[q >> 4][q & 15]);
           }
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      e[0] = en[0];
-      e[1] = en[1];
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++) {
+        e[rb][0] = en[rb][0];
+        e[rb][1] = en[rb][1];
+      }
       // the first B fragments of chunk c + 1 (buffer bn, complete since the last barrier)
 #pragma unroll
-      for (int i = 0; i < 4; i++) b[i] = bread(bn, i);
+      for (int i = 0; i < 4; i++) b[i] = bread(bn, bfrag(i));
       sstore(bw);
       __syncthreads();
       bc = bn;
     }
     const float unscale = EM == EM_BF16 ? 1.0f : sc[1];
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int64_t i = rowbase + 8 * (q / 4) + 4 * h + (q % 4);
-      if (i >= m) continue;
-      const float yi = y1[i] * unscale;
+    for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-      for (int ct = 0; ct < NT; ct++) {
-        const int64_t cc = (ct0 + ct) * 32 + l32;
-        if (cc < r) W[i * ldw + cc] = yi * acc[ct][q];
+      for (int q = 0; q < 16; q++) {
+        const int64_t i = rowbase + 32 * rb + 8 * (q / 4) + 4 * h + (q % 4);
+        if (i >= m) continue;
+        const float yi = y1[i] * unscale;
+#pragma unroll
+        for (int ct = 0; ct < NT; ct++) {
+          const int64_t cc = (ct0 + ct) * 32 + l32;
+          if (cc < r) W[i * ldw + cc] = yi * acc[rb][ct][q];
+        }
       }
-    }
     __syncthreads();
   }
 }
@@ -1075,7 +1141,7 @@ struct KpBuffers {
 // Widest operand (KD) of the wide-tile kernel
 #define KPW_MAX_KD 128
 
-template <int NT, int KD, int WAVES>
+template <int NT, int KD, int WAVES, int RB>
 static void launch_wide(int em, const __bf16* P, const __bf16* Qf, const void* Af,
                         const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
                         int64_t n_chunks, int64_t r_tiles, int64_t nxb, const float* sc,
@@ -1086,10 +1152,10 @@ static void launch_wide(int em, const __bf16* P, const __bf16* Qf, const void* A
     const int64_t t1 = t0 + cu_count < ntiles ? t0 + cu_count : ntiles;
     const dim3 grid((unsigned)(t1 - t0));
     if (em == EM_BF16)
-      hipLaunchKernelGGL((kprod_wide<NT, KD, EM_BF16, WAVES>), grid, dim3(64 * WAVES), 0, s, P, Qf,
+      hipLaunchKernelGGL((kprod_wide<NT, KD, EM_BF16, WAVES, RB>), grid, dim3(64 * WAVES), 0, s, P, Qf,
                          Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, t1, t0);
     else
-      hipLaunchKernelGGL((kprod_wide<NT, KD, EM_F16, WAVES>), grid, dim3(64 * WAVES), 0, s, P, Qf,
+      hipLaunchKernelGGL((kprod_wide<NT, KD, EM_F16, WAVES, RB>), grid, dim3(64 * WAVES), 0, s, P, Qf,
                          Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, t1, t0);
   }
 }
@@ -1100,7 +1166,7 @@ static tsvm_status product_wide(Ctx* c, KpBuffers& bufs, int em, int KD, const f
                                 int64_t n, int64_t d, const float* A, int64_t lda, int64_t r,
                                 double gl2, float* W, int64_t ldw) {
   hipStream_t s = c->stream;
-  constexpr int NT = 8, WAVES = 8, BR = 32 * WAVES;
+  constexpr int NT = 8, RB = 2, WAVES = 4, BR = 32 * RB * WAVES;
   const int64_t n_pad = round_up(n > 0 ? n : 1, 32), n_chunks = n_pad / 32;
   const int64_t m_pad = round_up(m, BR), r_pad = round_up(r, 32 * NT);
   const int64_t r_tiles = r_pad / 32, nxb = m_pad / BR;
@@ -1134,7 +1200,7 @@ static tsvm_status product_wide(Ctx* c, KpBuffers& bufs, int em, int KD, const f
   const __bf16 *Pb = (const __bf16*)P, *Qb = (const __bf16*)Qf;
 #define KPW_CASE(kd)                                                                            \
   case kd:                                                                                      \
-    launch_wide<NT, kd, WAVES>(em, Pb, Qb, Af, y1, W, ldw, m, r, n_chunks, r_tiles, nxb, sc,    \
+    launch_wide<NT, kd, WAVES, RB>(em, Pb, Qb, Af, y1, W, ldw, m, r, n_chunks, r_tiles, nxb, sc,    \
                                c->cu_count, s);                                                 \
     break;
   switch (KD) {
