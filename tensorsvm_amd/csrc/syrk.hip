@@ -509,15 +509,18 @@ __device__ __forceinline__ void syrk256_stages(APAN apan, BPAN bpan,
   const int srow = tid >> 3;
   u32x4_t v[NP][8];
   f32x4_t wv[2];
-  auto gload = [&](int64_t st) {
+  // half hf of stage st's loads: panel rows (tid >> 3) + 32 q for q in [4 hf, 4 hf + 4);
+  // the weights come with the second half (both halves' stores scale by them)
+  auto gload = [&](int64_t st, int hf) {
     const int64_t base = (int64_t)srow * SB_K + scol;
 #pragma unroll
     for (int p = 0; p < NP; p++) {
       const __bf16* pan = p ? bpan(st) : apan(st);
 #pragma unroll
-      for (int q = 0; q < 8; q++) v[p][q] = *(const u32x4_t*)(pan + base + (int64_t)32 * q * SB_K);
+      for (int q = 4 * hf; q < 4 * hf + 4; q++)
+        v[p][q] = *(const u32x4_t*)(pan + base + (int64_t)32 * q * SB_K);
     }
-    if (SCALE) {
+    if (SCALE && hf == 1) {
       wv[0] = *(const f32x4_t*)(wb + (s0 + st) * SB_K + scol);
       wv[1] = *(const f32x4_t*)(wb + (s0 + st) * SB_K + scol + 4);
     }
@@ -549,19 +552,23 @@ __device__ __forceinline__ void syrk256_stages(APAN apan, BPAN bpan,
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
   };
   if (nst <= 0) return;
-  gload(0);
+  gload(0, 0);
+  gload(0, 1);
   sstore(0, 0);
   sstore(0, 1);
+  gload(nst > 1 ? 1 : 0, 0);
+  gload(nst > 1 ? 1 : 0, 1);
   __syncthreads();
   bf16x8_t fa0[4], fb0[4], fa1[4], fb1[4];
   // Phases are fenced with sched_barrier.  Inside one: the next step's 8 fragment reads
   // first, then the 16 MFMAs; in the last two, the staging VALU work is spread between
-  // the MFMAs (5 per MFMA) and its LDS writes come last.
+  // the MFMAs (5 per MFMA), its LDS writes come next and then the loads of the same half of
+  // stage st + 2 into the registers just written out, so that each load has a whole stage
+  // to arrive (the last stages reload the last one: no branch in the body).
   for (int64_t st = 0; st < nst; st++) {
     const int buf = (int)(st & 1);
+    const int64_t snext = st + 2 < nst ? st + 2 : nst - 1;
     fread(fa0, fb0, buf, 0);
-    // the last stage reloads itself into the unused buffer: no branch in the body
-    gload(st + 1 < nst ? st + 1 : st);
     __builtin_amdgcn_sched_barrier(0);
     fread(fa1, fb1, buf, 1);
     mm(fa0, fb0);
@@ -575,6 +582,7 @@ __device__ __forceinline__ void syrk256_stages(APAN apan, BPAN bpan,
     __builtin_amdgcn_sched_barrier(0);
     fread(fa1, fb1, buf, 3);
     sstore(buf ^ 1, 0);  // buf ^ 1 was last read before the previous barrier
+    gload(snext, 0);
     mm(fa0, fb0);
     __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
 #pragma unroll
@@ -583,8 +591,10 @@ __device__ __forceinline__ void syrk256_stages(APAN apan, BPAN bpan,
       __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
     }
     __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);
+    __builtin_amdgcn_sched_group_barrier(0x020, 4 * NP, 0);
     __builtin_amdgcn_sched_barrier(0);
     sstore(buf ^ 1, 1);
+    gload(snext, 1);
     mm(fa1, fb1);
 #pragma unroll
     for (int m = 0; m < 16; m++) {
@@ -592,6 +602,7 @@ __device__ __forceinline__ void syrk256_stages(APAN apan, BPAN bpan,
       __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
     }
     __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);
+    __builtin_amdgcn_sched_group_barrier(0x020, 4 * NP + (SCALE ? 2 : 0), 0);
     __builtin_amdgcn_sched_barrier(0);
     __syncthreads();
   }
