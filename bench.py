@@ -156,6 +156,20 @@ def cpu_baseline(X, y, gamma, k, ntr, iters, prod_rows=50_000, ipm_rows=200_000)
                       f"{iters} iterations (the GPU run's count); {th} threads"}
 
 
+def dense_pass_figures(ipm, n, k):
+    """Per-iteration time and rates of the capacitance's dense pass (BF16 MFMA)."""
+    t, it = ipm.get("t_syrk_dense", 0.0), ipm.get("syrk_bf16_iters", 0)
+    if not t or not it:
+        return {"dense_pass_ms_per_iter": None}
+    per = t / it
+    syrk = n * k * (k + 1) / per / 1e12
+    return {"dense_pass_ms_per_iter": round(1e3 * per, 3),
+            "dense_pass_mfma_tflops": round(ipm["syrk_dense_mfma_tflops"], 1),
+            "dense_pass_mfma_frac_of_bf16_peak": round(ipm["syrk_dense_mfma_tflops"] / PEAK_BF16, 4),
+            "dense_pass_syrk_tflops": round(syrk, 1),
+            "dense_pass_syrk_frac_of_bf16_peak": round(syrk / PEAK_BF16, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,7 +398,11 @@ def main():
                 "capacitance_tflops": round(bf16_info["ipm"]["syrk_tflops"], 1),
                 "capacitance_frac_of_bf16_peak": round(bf16_info["ipm"]["syrk_tflops"] / PEAK_BF16, 4),
                 "capacitance_ms_per_iter": round(1e3 * bf16_info["ipm"]["t_syrk"]
-                                                 / max(bf16_info["ipm"]["iters"], 1), 3)},
+                                                 / max(bf16_info["ipm"]["iters"], 1), 3),
+                # the dense pass alone: every row through the lower 256 x 256 tiles
+                # (diagonal tiles whole); "mfma" counts the MFMA work it issues, "syrk"
+                # the SYRK convention n k (k + 1) over all n rows
+                **dense_pass_figures(bf16_info["ipm"], ntr, k)},
         "grad": grad,
         "grad_timed": grad_timed,
         "fp64_ref": fp64_ref,

@@ -175,6 +175,8 @@ typedef struct {
   double fp8_bound;    /* TSVM_PREC_FP8: error bound of the FP8 tier */
   int64_t n_x2_rows;   /* rows in the compensated (hi + lo bf16) tier */
   double x2_bound;     /* bound on sqrt(sum t_i^2) over them */
+  double t_dense;      /* s in the dense pass (every row; lower tiles, diagonal tiles whole) */
+  double dense_mfma_tflops; /* MFMA flops the dense pass issues / t_dense */
 } tsvm_syrk_info;
 tsvm_status tsvm_weighted_syrk_dev(tsvm_ctx *ctx, const float *Z, int64_t n, int64_t k,
                                    const double *w, int precision, double budget, double *M,
@@ -280,6 +282,8 @@ typedef struct {
   int syrk_bf16_iters;     /* iterations with a BF16 part */
   double fp8_row_frac;     /* TSVM_PREC_FP8: rows in the FP8 tier, averaged over iterations */
   double x2_row_frac;      /* rows in the compensated tier, averaged over iterations */
+  double t_syrk_dense;     /* s spent in the dense (BF16 or FP8) pass over all rows */
+  double syrk_dense_mfma_tflops; /* MFMA flops the dense pass issues / t_syrk_dense */
 } tsvm_ipm_info;
 void tsvm_ipm_default_params(tsvm_ipm_params *p);
 /* Mehrotra predictor-corrector for the SVM dual with Q = Z Z^T, every O(n) and O(nk)

@@ -103,7 +103,8 @@ class IpmInfo(ctypes.Structure):
                 ("pobj", _D), ("dobj", _D), ("gap_rel", _D), ("condD", _D), ("y", _D),
                 ("t_total", _D), ("t_syrk", _D), ("syrk_tflops", _D), ("refine_res", _D),
                 ("t_syrk_bf16", _D), ("syrk_bf16_tflops", _D), ("bf16_row_frac", _D),
-                ("syrk_bf16_iters", _INT), ("fp8_row_frac", _D), ("x2_row_frac", _D)]
+                ("syrk_bf16_iters", _INT), ("fp8_row_frac", _D), ("x2_row_frac", _D),
+                ("t_syrk_dense", _D), ("syrk_dense_mfma_tflops", _D)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -111,7 +112,8 @@ class IpmInfo(ctypes.Structure):
 
 class SyrkInfo(ctypes.Structure):
     _fields_ = [("n_fp64_rows", _I64), ("bf16_bound", _D), ("t_bf16", _D), ("bf16_tflops", _D),
-                ("n_fp8_rows", _I64), ("fp8_bound", _D), ("n_x2_rows", _I64), ("x2_bound", _D)]
+                ("n_fp8_rows", _I64), ("fp8_bound", _D), ("n_x2_rows", _I64), ("x2_bound", _D),
+                ("t_dense", _D), ("dense_mfma_tflops", _D)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -141,12 +141,14 @@ tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double bu
 // the tier sizes from the slots (host copy)
 HybridCounts hybrid_counts(const HybridSyrk* hs, const double* slots);
 // M = I + FP64 part over the listed rows + the dense pass (FP8 when hc.n8 > 0, BF16 when
-// hc.nb > 0) + the compensated part.  ev (optional, two events) brackets the
-// low-precision parts.
+// hc.nb > 0) + the compensated part.  ev (optional, three events) marks the start of the
+// low-precision parts, the end of the dense pass and the end of the compensated part.
 tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const double* w,
                              const HybridCounts& hc, double* M, hipEvent_t* ev);
 // flops of the low-precision parts, SYRK convention (rows formed x k (k + 1))
 double hybrid_syrk_flops(const HybridSyrk* hs, const HybridCounts& hc);
+// MFMA flops the dense pass issues (every row, lower tiles with whole diagonal tiles)
+double hybrid_dense_mfma_flops(const HybridSyrk* hs, const HybridCounts& hc);
 // tsvm_fnorm_res_dev (fnorm.hip): |K_Y|_F, |K_Y - U U^T|_F; U has row stride ldu
 tsvm_status fnorm_res_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d,
                           double gamma, const float* U, int64_t ldu, int64_t k, double* fnorm,

@@ -1222,14 +1222,16 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   }
   TSVM_HIP(c, hipGetLastError());
 
-  Event ev0, ev1, evb0, evb1;
+  Event ev0, ev1, evb0, evb1, evb2;
   TSVM_HIP(c, ev0.create());
   TSVM_HIP(c, ev1.create());
   TSVM_HIP(c, evb0.create());
   TSVM_HIP(c, evb1.create());
-  hipEvent_t evb[2] = {evb0, evb1};
+  TSVM_HIP(c, evb2.create());
+  hipEvent_t evb[3] = {evb0, evb1, evb2};
   double t_syrk = 0.0, t_bf16 = 0.0, row_frac_sum = 0.0, fp8_frac_sum = 0.0;
   double x2_frac_sum = 0.0, flops_low = 0.0, pending_flops = 0.0;
+  double t_dense = 0.0, flops_dense = 0.0, pending_dense = 0.0;
   int n_syrk = 0, n_bf16 = 0;
   bool pending_syrk = false, pending_bf16 = false;
   double hsc[IPM_NSLOT];
@@ -1266,10 +1268,15 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
       t_syrk += ms * 1e-3;
       n_syrk++;
       if (pending_bf16) {
-        TSVM_HIP(c, hipEventElapsedTime(&ms, evb[0], evb[1]));
+        TSVM_HIP(c, hipEventElapsedTime(&ms, evb[0], evb[2]));
         t_bf16 += ms * 1e-3;
         flops_low += pending_flops;
         n_bf16++;
+        if (pending_dense > 0) {
+          TSVM_HIP(c, hipEventElapsedTime(&ms, evb[0], evb[1]));
+          t_dense += ms * 1e-3;
+          flops_dense += pending_dense;
+        }
       }
       pending_syrk = false;
       pending_bf16 = false;
@@ -1341,6 +1348,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
       fp8_frac_sum += (double)hc.n8 / (double)n;
       x2_frac_sum += (double)hc.n2 / (double)n;
       pending_flops = hybrid_syrk_flops(&hs, hc);
+      pending_dense = hybrid_dense_mfma_flops(&hs, hc);
     } else {
       TSVM_CHECK(gemm_tn_f64(c, Z, kp, kp, Z, kp, kp, n, winv, true, M));
       hipLaunchKernelGGL(k_add_identity, dim3((unsigned)ceil_div(kp, 256)), dim3(256), 0, st, M, (int64_t)kp);
@@ -1393,6 +1401,8 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   inf.t_syrk_bf16 = t_bf16;
   inf.syrk_bf16_iters = n_bf16;
   if (t_bf16 > 0) inf.syrk_bf16_tflops = flops_low / t_bf16 * 1e-12;
+  inf.t_syrk_dense = t_dense;
+  if (t_dense > 0) inf.syrk_dense_mfma_tflops = flops_dense / t_dense * 1e-12;
   if (n_syrk > 0) inf.bf16_row_frac = row_frac_sum / n_syrk;
   if (n_syrk > 0) inf.fp8_row_frac = fp8_frac_sum / n_syrk;
   if (n_syrk > 0) inf.x2_row_frac = x2_frac_sum / n_syrk;

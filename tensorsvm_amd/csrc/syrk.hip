@@ -1116,6 +1116,7 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
                          kpt, kp, M, (const float*)nullptr);
     }
   }
+  if (ev_low) TSVM_HIP(c, hipEventRecord(ev_low[1], s));
   if (hc.n2 > 0 && hs->x2) {
     // the compensated rows, compacted: capacity grown in steps of an eighth of the rows
     const int64_t m2p = round_up(hc.n2, SY_BK);
@@ -1135,7 +1136,7 @@ tsvm_status hybrid_syrk_form(Ctx* c, HybridSyrk* hs, const float* Z, const doubl
     hipLaunchKernelGGL(syrk_accum_kernel, accg, dim3(256), 0, s, (const float*)part, nsplit, kpt,
                        kp, M, (const float*)nullptr);
   }
-  if (ev_low) TSVM_HIP(c, hipEventRecord(ev_low[1], s));
+  if (ev_low) TSVM_HIP(c, hipEventRecord(ev_low[2], s));
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
 }
@@ -1145,6 +1146,14 @@ double hybrid_fp8_eps() { return HYBRID_FP8_EPS; }
 double hybrid_syrk_flops(const HybridSyrk* hs, const HybridCounts& hc) {
   // rows formed in low precision (each once) times the SYRK count k (k + 1)
   return (double)(hs->n - hc.nh) * hs->kp * (hs->kp + 1.0);
+}
+
+double hybrid_dense_mfma_flops(const HybridSyrk* hs, const HybridCounts& hc) {
+  // the dense pass streams every row through the lower tiles (256 or 128 wide, diagonal
+  // tiles whole): the MFMA work it issues
+  if (hc.n8 <= 0 && hc.nb <= 0) return 0.0;
+  const int bt = hs->kpt >= 256 ? 256 : 128, nt = hs->kpt / bt;
+  return 2.0 * bt * bt * (nt * (nt + 1) / 2) * (double)hs->n_pad;
 }
 
 tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, const double* w,
@@ -1172,15 +1181,17 @@ tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, cons
     double hsl[HYBRID_NSLOT] = {0};
     TSVM_HIP(c, hipMemcpyAsync(hsl, slots, sizeof(hsl), hipMemcpyDeviceToHost, s));
     TSVM_HIP(c, hipStreamSynchronize(s));
-    Event ev0, ev1;
+    Event ev0, ev1, ev2;
     TSVM_HIP(c, ev0.create());
     TSVM_HIP(c, ev1.create());
-    hipEvent_t ev[2] = {ev0, ev1};
+    TSVM_HIP(c, ev2.create());
+    hipEvent_t ev[3] = {ev0, ev1, ev2};
     const HybridCounts hc = hybrid_counts(&hs, hsl);
     TSVM_CHECK(hybrid_syrk_form(c, &hs, Z, w, hc, M, ev));
     TSVM_HIP(c, hipStreamSynchronize(s));
-    float ms = 0.f;
-    TSVM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    float ms = 0.f, msd = 0.f;
+    TSVM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[2]));
+    TSVM_HIP(c, hipEventElapsedTime(&msd, ev[0], ev[1]));
     inf.n_fp64_rows = hc.nh;
     inf.bf16_bound = hsl[0];
     inf.n_fp8_rows = hc.n8;
@@ -1190,6 +1201,11 @@ tsvm_status weighted_syrk_dev(Ctx* c, const float* Z, int64_t n, int64_t k, cons
     if (hc.nh < n) {
       inf.t_bf16 = ms * 1e-3;
       if (ms > 0) inf.bf16_tflops = hybrid_syrk_flops(&hs, hc) / (ms * 1e-3) * 1e-12;
+    }
+    const double fd = hybrid_dense_mfma_flops(&hs, hc);
+    if (fd > 0) {
+      inf.t_dense = msd * 1e-3;
+      if (msd > 0) inf.dense_mfma_tflops = fd / (msd * 1e-3) * 1e-12;
     }
   }
   TSVM_HIP(c, hipStreamSynchronize(s));

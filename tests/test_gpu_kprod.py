@@ -116,15 +116,18 @@ def test_kprod_fp16x2_exact_rhs(ctx):
     assert rel.max() < 2.0 ** -12
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16"])
-def test_kprod_wide_tiles_match_narrow(ctx, prec):
+@pytest.mark.parametrize("prec,d", [("bf16", 18), ("fp16", 18), ("fp16", 2), ("fp16", 9),
+                                    ("fp16", 14), ("bf16", 25), ("fp16", 30), ("fp16", 36),
+                                    ("fp16", 41)])
+def test_kprod_wide_tiles_match_narrow(ctx, prec, d):
     """Right-hand sides wider than 128 columns run the 256-column-tile kernel; its outputs
     are bit-identical to the 128-column kernel's on each half (same distance tiles, same
     accumulation order).  For fp16 both halves get the same power-of-two scale by carrying
-    the same largest |A| entry."""
+    the same largest |A| entry.  The feature counts cover every compiled operand width of
+    the wide kernel (KD = 16 .. 128: one to eight distance K steps)."""
     import tensorsvm_amd as T
     p = T.PREC_BF16 if prec == "bf16" else T.PREC_FP16
-    m, n, d, r = 1000, 3000, 18, 256
+    m, n, r = 1000, 3000, 256
     X1, y1, X2, y2, A = _case(m, n, d, r, 0.05, 123)
     A[0, 0], A[0, 128] = 7.5, 7.5
     W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, p)
