@@ -487,24 +487,29 @@ __global__ __launch_bounds__(256) void syrk_bf16_kernel(const __bf16* __restrict
 // a 128 x 128 quarter of the tile (4 x 4 MFMA 32x32x16 tiles, 256 accumulators), so that
 // each 16-byte fragment read from LDS feeds 4 MFMAs.  Software pipeline per stage of SB_K
 // rows (4 steps of 16): the fragments of step s + 1 are read while the MFMAs of step s
-// run (one wave per SIMD leaves nothing else to hide LDS latency), the next stage's
-// global loads are issued at the top of the stage and staged into the other LDS buffer
-// (the A panel scaled by the weights) during the last two steps, one barrier per stage.
-// On a diagonal tile one panel is loaded (staged scaled and as is) and the result of the
-// wave above the diagonal is not stored.
+// run (one wave per SIMD leaves nothing else to hide LDS latency); the next stage is
+// staged into the other LDS buffer (the A panel scaled by the weights) during the last
+// two steps, half a stage per step, and each half's registers are reloaded at once with
+// the stage after it (a whole stage for the loads to arrive); one barrier per stage.
+// The result of the wave above the diagonal of a diagonal tile is not stored.
 constexpr int SB_K = SY_BK;      // the chunk of the ZbT layout
 constexpr int SB_LS = SB_K + 8;  // LDS row stride (bf16): conflict-free 16-byte reads
 
 // APAN(st) / BPAN(st): the A / B panel (256 columns) of the operand chunk of stage st;
 // SCALE: the A panel is scaled by the row weights wb (stage st covers rows
 // (s0 + st) SB_K ..), else both panels are staged as loaded
-template <bool DIAG, bool SCALE, class APAN, class BPAN>
+template <bool SCALE, class APAN, class BPAN>
 __device__ __forceinline__ void syrk256_stages(APAN apan, BPAN bpan,
                                                const float* __restrict__ wb, int64_t s0,
                                                int64_t nst, __bf16 (*sm)[256 * SB_LS],
                                                f32x16_t (&acc)[4][4], int tid, int wa, int wbo,
                                                int l32, int h) {
-  constexpr int NP = DIAG ? 1 : 2;  // panels loaded
+  // Both panels are loaded on diagonal tiles too (the same rows twice, the second from
+  // L2): the three tiles of a split then stream at one pace and share each row in their
+  // XCD's L2.  With one panel the diagonal tiles ran ahead and the pass fetched 3.0 GB
+  // from HBM at 1.6M x 512 instead of 1.65 GB (in the same time: the pass is not
+  // HBM-bound, but the traffic is waste).
+  constexpr int NP = 2;  // panels loaded
   const int scol = (tid & 7) * 8;   // staging map: panel rows (tid >> 3) + 32 q, 8 K at scol
   const int srow = tid >> 3;
   u32x4_t v[NP][8];
@@ -643,10 +648,7 @@ __global__ __launch_bounds__(256, 1) void syrk_bf16_256_kernel(const __bf16* __r
   const int64_t cstride = (int64_t)kpt * SB_K;  // elements per chunk
   auto apan = [&](int64_t st) { return Apan + (s0 + st) * cstride; };
   auto bpan = [&](int64_t st) { return Bpan + (s0 + st) * cstride; };
-  if (ta == tb)
-    syrk256_stages<true, true>(apan, bpan, wb, s0, nst, sm, acc, tid, wa, wbo, l32, h);
-  else
-    syrk256_stages<false, true>(apan, bpan, wb, s0, nst, sm, acc, tid, wa, wbo, l32, h);
+  syrk256_stages<true>(apan, bpan, wb, s0, nst, sm, acc, tid, wa, wbo, l32, h);
   if (ta == tb && wbo > wa) return;  // strictly above the diagonal
   // C layout of the 32x32 MFMA: register q holds row 8(q/4) + 4h + q%4, column l32
   float* P = part + (int64_t)split * kpt * kpt;
@@ -935,7 +937,7 @@ __global__ __launch_bounds__(256, 1) void syrk_x2_256_kernel(const __bf16* __res
     const int64_t g = s0 + st, c = g % S;
     return (g >= S && g < 2 * S ? L : H) + c * cstride + boff;
   };
-  syrk256_stages<false, false>(apan, bpan, nullptr, s0, nst, sm, acc, tid, wa, wbo, l32, h);
+  syrk256_stages<false>(apan, bpan, nullptr, s0, nst, sm, acc, tid, wa, wbo, l32, h);
   if (ta == tb && wbo > wa) return;  // strictly above the diagonal
   // C layout of the 32x32 MFMA: register q holds row 8(q/4) + 4h + q%4, column l32
   float* P = part + (int64_t)split * kpt * kpt;
