@@ -55,7 +55,7 @@ def log(*a):
 # two products, each the sum of its launches): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (KiB per dispatch) scaled by the factor that the same counters give for the bench's
 # 3.28 GB Omega copy (x2.0 and x1.0)
-KPROD_TRAFFIC = {"fetch_bytes": 459_875_333_824, "write_bytes": 3_276_800_000,
+KPROD_TRAFFIC = {"fetch_bytes": 408_445_491_968, "write_bytes": 3_366_400_000,
                  "source": "profiles/kprod_factor_c2_pmc.md"}
 
 
