@@ -1018,6 +1018,16 @@ __global__ void k_update(int64_t n, double* __restrict__ x, double* __restrict__
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[SL_Y] += al * (sc[SL_DY] + sc[SL_DYC]);
 }
 
+// Z^T x after k_update: x += 0.99 alpha (dx_aff + dx_corr), and T3a, T3 hold Z^T of the two
+// steps (skipped, as k_update, after a failed factorisation)
+__global__ void k_carry_tx(int kp, double* __restrict__ Tx, const double* __restrict__ T3a,
+                           const double* __restrict__ T3c, const double* __restrict__ sc,
+                           const int* __restrict__ info) {
+  if (*info != 0) return;
+  const double al = 0.99 * sc[SL_ALPHA];
+  for (int e = threadIdx.x; e < kp; e += blockDim.x) Tx[e] += al * (T3a[e] + T3c[e]);
+}
+
 // M += I (kp x kp) and copy lower to upper is not needed (Cholesky reads the lower part)
 __global__ void k_add_identity(double* __restrict__ M, int64_t kp) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1119,7 +1129,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   WS("ipm_solp", 2 * n * 8, solp);
   WS("ipm_solc", n * 8, solc);
   WS("ipm_sol3", n * 8, sol3);
-  WS("ipm_T", 6 * kp * 8, T);
+  WS("ipm_T", 8 * kp * 8, T);
   WS("ipm_M", (int64_t)kp * kp * 8, M);
   WS("ipm_sc", IPM_NSLOT * 8, sc);
   WS("ipm_red", EW_BLOCKS * MAXR * 8, red);
@@ -1166,7 +1176,9 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   double* Tr7 = T + 2 * kp;
   double* Tb = T + 3 * kp;
   double* Tc = T + 4 * kp;
-  double* T3 = T + 5 * kp;
+  double* T3 = T + 5 * kp;   // Z^T of the corrector's step
+  double* Tx = T + 6 * kp;   // Z^T x
+  double* T3a = T + 7 * kp;  // Z^T of the predictor's (affine) step
   // SMW solve: sol = (D + Z Z^T)^{-1} rhs with the capacitance as factored, p right-hand
   // sides (its error is corrected by the bordered refinement below, whose residual uses
   // the exact operator).  V = D^-1 rhs must be in place; Z^T V, M^-1 Z^T V, then one
@@ -1186,21 +1198,25 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   //   elimination; dx += ddx, dy += ddy.  T3 = Z^T sol3 comes in from the linear
   //   combination that formed sol3, so the residual pass reads Z once.
   const int border = prm->refine < 0 ? 1 : (prm->refine > 0 ? prm->refine : 0);
-  auto border_refine = [&](int corr) -> tsvm_status {
+  // With refinement, Z^T of each step is known from the passes that form it (T3 of the
+  // predictor and of the corrector, the last sweep's Z^T s5 included), so Z^T x is carried
+  // from iteration to iteration as Z^T x += alpha (T3a + T3) instead of read from Z again.
+  const bool carry_tx = border >= 1;
+  auto border_refine = [&](int corr, double* T3v) -> tsvm_status {
     for (int sw = 0; sw < border; sw++) {
-      const bool more = sw + 1 < border;
+      const bool zt = sw + 1 < border || carry_tx;  // Z^T s5 wanted: T3v stays Z^T sol3
       // residual, V = r5' / D and Z^T V in one pass
-      TSVM_CHECK(ipm_zpass(c, ZP_BORDER, 1, 1, Z, n, kp, T3, nullptr, winv, a, D, rhs, sol3, sc,
+      TSVM_CHECK(ipm_zpass(c, ZP_BORDER, 1, 1, Z, n, kp, T3v, nullptr, winv, a, D, rhs, sol3, sc,
                            corr, V, red, Tsol));
       TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_R2B, SL_BRES, SL_BRHS}, zblk));
       // s5 = S^-1 r5'; the correction is a combination of s5 and b = S^-1 a
       TSVM_CHECK(chol_solve_f64(c, M, kp, Tsol, 1, dinfo));
-      TSVM_CHECK(ipm_zpass(c, ZP_SMW, 1, more ? 1 : 0, Z, n, kp, Tsol, V, winv, a, nullptr,
-                           nullptr, nullptr, sc, 0, solc, red, more ? Tc : nullptr));
+      TSVM_CHECK(ipm_zpass(c, ZP_SMW, 1, zt ? 1 : 0, Z, n, kp, Tsol, V, winv, a, nullptr,
+                           nullptr, nullptr, sc, 0, solc, red, zt ? Tc : nullptr));
       TSVM_CHECK(reduce({RED_SUM}, {SL_AR7B}, zblk));
       hipLaunchKernelGGL(k_border_lin, eg, eb, 0, st, n, corr, (const double*)(solp + n),
-                         (const double*)solc, sc, sol3, kp, (const double*)(more ? Tc : nullptr),
-                         (const double*)Tb, T3);
+                         (const double*)solc, sc, sol3, kp, (const double*)(zt ? Tc : nullptr),
+                         (const double*)Tb, T3v);
     }
     return TSVM_OK;
   };
@@ -1243,13 +1259,13 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   for (iter = 0; iter < maxit; iter++) {
     // ---- residual pass (1606-1643); with wide rows it also forms the predictor's scaled
     // right-hand sides [r5, a]/D and their Z^T (into Tsol) from the same read of Z
-    TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, x, n, 1, Tc));
+    if (!carry_tx || iter == 0) TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, x, n, 1, Tx));
     if (fuse_res) {
       const ResArgs ra{x, s, xi, dx, ds, dxi, D, winv, rhs, V, C};
-      TSVM_CHECK(ipm_zpass(c, ZP_RES, 1, 2, Z, n, kp, Tc, nullptr, nullptr, a, nullptr, nullptr,
+      TSVM_CHECK(ipm_zpass(c, ZP_RES, 1, 2, Z, n, kp, Tx, nullptr, nullptr, a, nullptr, nullptr,
                            nullptr, sc, 0, nullptr, red, Tsol, &ra));
     } else {
-      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, Tc, 1, e, n));
+      TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, Tx, 1, e, n));
       hipLaunchKernelGGL(k_resid, eg, eb, 0, st, n, C, (const double*)x, (const double*)s,
                          (const double*)xi, (const double*)a, (const double*)e, (const double*)sc,
                          dx, ds, dxi, D, winv, red);
@@ -1364,8 +1380,8 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     }
     TSVM_CHECK(smw(2, solp, Tr7, {SL_AR7, SL_AB}, fuse_res));  // r7 = S^-1 r5, b = S^-1 a
     hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 0, (const double*)(solp + n), (const double*)solp,
-                       sc, sol3, kp, (const double*)(border ? Tr7 : nullptr), (const double*)Tb, T3);
-    TSVM_CHECK(border_refine(0));
+                       sc, sol3, kp, (const double*)(border ? Tr7 : nullptr), (const double*)Tb, T3a);
+    TSVM_CHECK(border_refine(0, T3a));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 0, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
                        (const double*)ads, (const double*)adxi, red);
@@ -1381,13 +1397,16 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_CHECK(smw(1, solc, border ? Tc : nullptr, {SL_AR7C}));
     hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 1, (const double*)(solp + n), (const double*)solc,
                        sc, sol3, kp, (const double*)(border ? Tc : nullptr), (const double*)Tb, T3);
-    TSVM_CHECK(border_refine(1));
+    TSVM_CHECK(border_refine(1, T3));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 1, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
                        (const double*)ads, (const double*)adxi, red);
     TSVM_CHECK(reduce({RED_MIN}, {SL_ALPHA}));
     hipLaunchKernelGGL(k_update, eg, eb, 0, st, n, x, s, xi, (const double*)dx, (const double*)ds,
                        (const double*)dxi, sc, (const int*)dinfo);
+    if (carry_tx)
+      hipLaunchKernelGGL(k_carry_tx, dim3(1), dim3(256), 0, st, kp, Tx, (const double*)T3a,
+                         (const double*)T3, (const double*)sc, (const int*)dinfo);
     TSVM_HIP(c, hipGetLastError());
   }
   TSVM_HIP(c, hipStreamSynchronize(st));
