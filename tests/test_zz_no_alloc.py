@@ -40,6 +40,23 @@ def test_noalloc_matches_workspace_product(ctx, prec):
         assert np.array_equal(a, b), (m, n, prec, np.abs(a - b).max())
 
 
+@pytest.mark.parametrize("prec", [1, 3])
+def test_noalloc_matches_workspace_product_wide(ctx, prec):
+    """Right-hand sides wider than 128 columns (the 256-column-tile kernel of the bf16 and
+    fp16 forms) through the scratch array: two calls, the second smaller."""
+    import torch
+    for (m, n, d, r, seed) in [(800, 6000, 18, 256, 3), (300, 2000, 18, 160, 4)]:
+        X1, y1, X2, y2, A = _inputs(m, n, d, r, seed + 10 * prec)
+        W_ws = torch.full((m, r), float("nan"), dtype=torch.float32, device="cuda")
+        W_na = torch.full((m, r), float("nan"), dtype=torch.float32, device="cuda")
+        ctx.kernel_product_dev(X1, y1, m, X2, y2, n, d, A, r, 0.05, prec, W_ws)
+        ctx.kernel_product_noalloc_dev(X1, y1, m, X2, y2, n, d, A, r, 0.05, prec, W_na)
+        ctx.sync()
+        a, b = W_ws.cpu().numpy(), W_na.cpu().numpy()
+        assert np.isfinite(a).all()
+        assert np.array_equal(a, b), (m, n, prec, np.abs(a - b).max())
+
+
 def test_noalloc_rejects_oversize(ctx):
     import tensorsvm_amd as T
     # 2 n r bytes of fp16 right-hand side alone exceed the 64 MiB scratch: refused before
