@@ -603,19 +603,17 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
   const int64_t chunk = ceil_div(ceil_div(n, nw), ZP_U) * ZP_U;
   const int64_t r0 = wave * chunk;
   const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
-  for (int64_t base = r0; base < r1; base += ZP_U) {
-    // rows base .. base + ZP_U - 1; rows past r1 are clamped for the loads and masked
-    f32x4_t z[ZP_U][F];
+  // Software pipeline: the rows and per-row operands of the next step are loaded before
+  // the current step's products, reductions and updates run (two register sets, the loop
+  // unrolled by two so that they alternate without copies)
+  auto load_step = [&](int64_t base, f32x4_t(&z)[ZP_U][F], double(&o)[NO]) {
 #pragma unroll
     for (int u = 0; u < ZP_U; u++) {
       const int64_t rr = base + u < n ? base + u : n - 1;
 #pragma unroll
       for (int f = 0; f < F; f++) z[u][f] = *(const f32x4_t*)(Z + rr * kp + 4 * grp[f]);
     }
-    const int64_t myrow = base + ul;
-    const bool ok = myrow < r1;
-    const int64_t rl = myrow < n ? myrow : n - 1;
-    double o[NO];
+    const int64_t rl = base + ul < n ? base + ul : n - 1;
     if constexpr (MODE == ZP_BORDER) {
       o[0] = D[rl];
       o[1] = dxv[rl];
@@ -634,6 +632,11 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
       o[2] = ra.xi[rl];
       o[3] = a[rl];
     }
+  };
+  // rows base .. base + ZP_U - 1 (rows past r1 were clamped for the loads and are masked)
+  auto step = [&](int64_t base, const f32x4_t(&z)[ZP_U][F], const double(&o)[NO]) {
+    const int64_t myrow = base + ul;
+    const bool ok = myrow < r1;
     double w, w1 = 0.0;  // this lane's product(s) after the row-wise update
     if constexpr (DOT) {
       double v[M];
@@ -731,6 +734,16 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
             for (int j = 0; j < 4; j++) acc[q][f][j] += uq * (double)z[u][f][j];
         }
     }
+  };
+  f32x4_t za[ZP_U][F], zb[ZP_U][F];
+  double oa[NO], ob[NO];
+  if (r0 < r1) load_step(r0, za, oa);
+  for (int64_t base = r0; base < r1; base += 2 * ZP_U) {
+    if (base + ZP_U < r1) load_step(base + ZP_U, zb, ob);
+    step(base, za, oa);
+    if (base + ZP_U >= r1) break;
+    if (base + 2 * ZP_U < r1) load_step(base + 2 * ZP_U, za, oa);
+    step(base + ZP_U, zb, ob);
   }
   if constexpr (Q > 0) {
     const int half = wid & 1;
