@@ -75,7 +75,7 @@ def grad_form(ctx, torch, n=200_000, nt=50_000, d=32, k=256, C=1.0, seed=1003):
     dom = om.clone()
     dal = torch.empty(n, dtype=torch.float64, device=dev)
     prm = T.capi.train_params(k=k, C=C, gamma=g0, kernel_type=2, precision=T.PREC_FP32,
-                              stop=T.capi.STOP_GAP, tol=1e-5, maxit=300, refine=1)
+                              stop=T.capi.STOP_GAP, tol=1e-5, maxit=300)
     t0 = time.perf_counter()
     info = ctx.train_dev(dX, dy, n, d, prm, dom, dal)
     torch.cuda.synchronize()
@@ -196,6 +196,9 @@ def main():
     ap.add_argument("--precision", choices=("bf16", "fp32", "fp8"), default="bf16",
                     help="form of the timed steps (the comparison run, if any, is FP32); fp8: the "
                          "BF16 form with the FP8 e4m3 pass of the capacitance")
+    ap.add_argument("--refine", type=int, default=-1,
+                    help="refinement sweeps of the Newton steps (m > 0: m for the affine step and m "
+                         "for the corrector's; -m: m for the combined step only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the trainer "
@@ -246,7 +249,7 @@ def main():
         dOm.copy_(dOm0)  # the factor construction consumes Omega
         torch.cuda.synchronize()
         prm = T.capi.train_params(k=k, C=C, gamma=gamma, kernel_type=2, precision=precision,
-                                  stop=T.capi.STOP_GAP, tol=1e-5, maxit=300, refine=1,
+                                  stop=T.capi.STOP_GAP, tol=1e-5, maxit=300, refine=args.refine,
                                   progress=int(args.progress), analysis=int(args.analysis))
         t0 = time.perf_counter()
         info = ctx.train_dev(dX, dy, ntr, d, prm, dOm, dal)

@@ -261,7 +261,9 @@ typedef struct {
   int precision;     /* TSVM_PREC_FP32: FP64 capacitance; TSVM_PREC_BF16 (or FP16X2): hybrid
                         capacitance (tsvm_weighted_syrk_dev, bulk rows on BF16 MFMA) +
                         refinement of the Newton step against the FP32 factor */
-  int refine;        /* refinement sweeps of each Newton step's bordered system, default 1 */
+  int refine;        /* refinement sweeps of the bordered Newton system: m > 0 sweeps for each of
+                        the affine and the corrector's steps; -m (default -1): m sweeps for the
+                        combined step only, the affine step left as solved */
   int verbose;       /* print the reference's iteration line every 5 iterations */
   int init;          /* 0: the reference's initial point for kernel_type; 1: the same x with
                         s, xi shifted so the initial dual residual is zero (what the linear
@@ -319,7 +321,7 @@ typedef struct {
   int stop_mode;      /* TSVM_STOP_REF (default) or TSVM_STOP_GAP */
   double tol;         /* gap tolerance for TSVM_STOP_GAP */
   int maxit;          /* IPM iterations (100) */
-  int refine;         /* refinement sweeps of each Newton step */
+  int refine;         /* refinement sweeps, as tsvm_ipm_params.refine (default -1) */
   int verbose;        /* reference-style progress lines on stdout */
   int init;           /* IPM initial point, as tsvm_ipm_params.init */
   int analysis;       /* -analysis: |K_Y|_F and |K_Y - U U^T|_F after the factor (RBF) */

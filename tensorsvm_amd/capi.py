@@ -129,7 +129,7 @@ class GradInfo(ctypes.Structure):
 
 
 def ipm_params(C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5, precision=PREC_FP32,
-               refine=1, verbose=0, init=0):
+               refine=-1, verbose=0, init=0):
     return IpmParams(C, kernel_type, maxit, stop, tol, precision, refine, verbose, init)
 
 
@@ -152,7 +152,7 @@ class TrainInfo(ctypes.Structure):
 
 
 def train_params(k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32, stop=STOP_REF,
-                 tol=1e-5, maxit=100, refine=1, verbose=0, init=0, analysis=0, progress=0):
+                 tol=1e-5, maxit=100, refine=-1, verbose=0, init=0, analysis=0, progress=0):
     return TrainParams(k, C, gamma, kernel_type, precision, stop, tol, maxit, refine, verbose, init,
                        analysis, progress)
 
@@ -400,7 +400,7 @@ class Context:
         return info
 
     def ipm_dev(self, Z, a, n, k, C, alpha, xi, s=None, maxit=100, stop=STOP_REF, tol=1e-5,
-                precision=PREC_FP32, refine=1, init=0):
+                precision=PREC_FP32, refine=-1, init=0):
         """Dual IPM on device Z (n x k FP32) and labels a; alpha, xi (and s) device FP64."""
         prm = ipm_params(C, 2, maxit, stop, tol, precision, refine, 0, init)
         info = IpmInfo()
@@ -410,7 +410,7 @@ class Context:
         return info
 
     def ipm(self, Z, a, C, kernel_type=2, maxit=100, stop=STOP_REF, tol=1e-5,
-            precision=PREC_FP32, refine=1, verbose=0, init=0):
+            precision=PREC_FP32, refine=-1, verbose=0, init=0):
         """Dual IPM on Q = Z Z^T; returns (alpha, xi, s, info)."""
         Z, a = _f32(Z), _f32(a)
         n, k = Z.shape
@@ -422,7 +422,7 @@ class Context:
         return al, xi, s, info
 
     def train(self, X, y, k=50, C=1.0, gamma=0.0, kernel_type=2, precision=PREC_FP32,
-              stop=STOP_REF, tol=1e-5, maxit=100, refine=1, verbose=0, omega=None, init=0,
+              stop=STOP_REF, tol=1e-5, maxit=100, refine=-1, verbose=0, omega=None, init=0,
               analysis=0, progress=0):
         """Train on X (n x d) with labels +-1; returns (alpha, w (linear) or None, info)."""
         X, y = _f32(X), _f32(y)

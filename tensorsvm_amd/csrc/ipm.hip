@@ -10,16 +10,21 @@
 //                     MFMA with FP32 accumulation (BF16 form); FP64 Cholesky on the GPU
 //   predictor         SMW solves for [r5, a] batched into one pass pair over Z; the
 //                     reference's third solve is a linear combination (S^-1 is linear:
-//                     S^-1 (a dy - r5) = dy S^-1 a - S^-1 r5); then a sweep of bordered
-//                     refinement: the residual of the bordered Newton system against the
-//                     exact operator D + Z Z^T (FP32 factor, FP64 accumulation), one SMW
-//                     solve, the correction again combined with S^-1 a; step bound and
-//                     mu_aff as device reductions
-//   corrector         the same with one SMW solve (S^-1 a is reused), combined step, update
+//                     S^-1 (a dy - r5) = dy S^-1 a - S^-1 r5); step bound and mu_aff as
+//                     device reductions
+//   corrector         one SMW solve (S^-1 a is reused); the corrector's step is added to
+//                     the affine one and a sweep of bordered refinement corrects the
+//                     combined step: the residual of the bordered Newton system for the
+//                     combined right-hand sides against the exact operator D + Z Z^T (FP32
+//                     factor, FP64 accumulation), one SMW solve, the correction again
+//                     combined with S^-1 a; update.  (refine > 0 instead refines the affine
+//                     and the corrector's steps each.)
 // The last pass of each SMW solve also accumulates Z^T sol, so Z^T of the combined step is
 // a combination of k-vectors, and the bordered residual pass also accumulates Z^T of the
-// next solve's right-hand side.  Per iteration that is 10 reads of Z; for 256 <= kp <= 768
-// the residual pass also forms the predictor's right-hand sides and their Z^T (9 reads).
+// next solve's right-hand side; Z^T x is carried from iteration to iteration.  Per
+// iteration that is 6 reads of Z for 256 <= kp <= 768 (the residual pass, which also forms
+// the predictor's right-hand sides and their Z^T; the predictor's finish; the corrector's
+// Z^T and finish; the refinement's residual and finish), 8 with refine > 0.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -905,27 +910,34 @@ __global__ void k_scale(int64_t n, int p, const double* __restrict__ rhs, int64_
 // r7 = S^-1 r5, b = S^-1 a are already at hand, so dx = dy b - r7 without another solve.
 // With Tr7 = Z^T r7 and Tb = Z^T b given (kp entries), block 0 also forms
 // T3 = Z^T dx = dy Tb - Tr7 for the bordered refinement.
+// corr = 2: the corrector's step is added to the affine one (adx, dy_aff, T3a = Z^T adx),
+// so sol3, the corrector's dy and T3 hold the combined step.
 __global__ void k_sol3_lin(int64_t n, int corr, const double* __restrict__ b,
                            const double* __restrict__ r7, double* __restrict__ sc,
                            double* __restrict__ sol3, int kp, const double* __restrict__ Tr7,
-                           const double* __restrict__ Tb, double* __restrict__ T3) {
+                           const double* __restrict__ Tb, double* __restrict__ T3,
+                           const double* __restrict__ adx, const double* __restrict__ T3a) {
+  const bool comb = corr == 2;
   const double r2 = corr ? 0.0 : -sc[SL_AX];
   const double dy = (r2 + sc[corr ? SL_AR7C : SL_AR7]) / sc[SL_AB];
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[corr ? SL_DYC : SL_DY] = dy;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[corr ? SL_DYC : SL_DY] = comb ? dy + sc[SL_DY] : dy;
   if (Tr7 && blockIdx.x == 0)
-    for (int e = threadIdx.x; e < kp; e += blockDim.x) T3[e] = dy * Tb[e] - Tr7[e];
-  GRID_LOOP(i, n) sol3[i] = dy * b[i] - r7[i];
+    for (int e = threadIdx.x; e < kp; e += blockDim.x)
+      T3[e] = dy * Tb[e] - Tr7[e] + (comb ? T3a[e] : 0.0);
+  GRID_LOOP(i, n) sol3[i] = dy * b[i] - r7[i] + (comb ? adx[i] : 0.0);
 }
 
 // bordered correction: (ddx, ddy) solve the same bordered system for the residual
 // (r5', rho2): ddy = (rho2 + a^T S^-1 r5') / a^T b, ddx = S^-1 (a ddy - r5') = ddy b - s5
 // with s5 = S^-1 r5' and b = S^-1 a (S^-1 linear); dx += ddx, dy += ddy.  With Ts5 =
 // Z^T s5 given, block 0 also updates T3 = Z^T dx by ddy Tb - Ts5 (for a further sweep).
+// corr: 0 the affine step, 1 the corrector's, 2 the combined step (whose r2 is the affine
+// one's, the corrector's being 0).
 __global__ void k_border_lin(int64_t n, int corr, const double* __restrict__ b,
                              const double* __restrict__ s5, double* __restrict__ sc,
                              double* __restrict__ sol3, int kp, const double* __restrict__ Ts5,
                              const double* __restrict__ Tb, double* __restrict__ T3) {
-  const double r2 = corr ? 0.0 : -sc[SL_AX];
+  const double r2 = corr == 1 ? 0.0 : -sc[SL_AX];
   const double rho2 = r2 - sc[SL_R2B];
   const double ddy = (rho2 + sc[SL_AR7B]) / sc[SL_AB];
   GRID_LOOP(i, n) sol3[i] += ddy * b[i] - s5[i];
@@ -990,11 +1002,16 @@ __global__ __launch_bounds__(256) void k_muaff(int64_t n, double C, const double
 }
 
 // corrector right-hand sides (1750-1757): keep the affine step, r3 = sigma mu - dx ds,
-// r4 = sigma mu + dx dxi, r1 = r2 = 0, rhs0 = r5 = -r3/x + r4/(C-x)
+// r4 = sigma mu + dx dxi, r1 = r2 = 0, rhs0 = r5 = -r3/x + r4/(C-x).
+// comb: also V = r5 / D (the corrector's solve), and r3, r4, rhs0 are left as the sums of
+// the affine right-hand sides (r3 = -s x, r4 = -xi (C - x), r5 in rhs0) and the
+// corrector's, the right-hand sides of the combined step.
 __global__ void k_rhs_corr(int64_t n, double C, const double* __restrict__ x,
                            double* __restrict__ dx, double* __restrict__ ds, double* __restrict__ dxi,
                            double* __restrict__ adx, double* __restrict__ ads, double* __restrict__ adxi,
-                           const double* __restrict__ sc, double* __restrict__ rhs) {
+                           const double* __restrict__ sc, double* __restrict__ rhs, int comb,
+                           const double* __restrict__ s, const double* __restrict__ xi,
+                           const double* __restrict__ winv, double* __restrict__ V) {
   const double n2 = 2.0 * (double)n;
   const double mu = sc[SL_MU] / n2, mua = sc[SL_MUAFF] / n2;
   double sig = mua / mu;
@@ -1006,18 +1023,28 @@ __global__ void k_rhs_corr(int64_t n, double C, const double* __restrict__ x,
     ads[i] = a3;
     adxi[i] = a4;
     double r3 = sm - a1 * a3, r4 = sm + a1 * a4;
+    const double xv = x[i], cx = C - xv;
+    const double r5 = -r3 / xv + r4 / cx;
+    if (comb) {
+      V[i] = r5 * winv[i];
+      r3 -= s[i] * xv;
+      r4 -= xi[i] * cx;
+      rhs[i] += r5;
+    } else {
+      rhs[i] = r5;
+    }
     ds[i] = r3;
     dxi[i] = r4;
-    double xv = x[i];
-    rhs[i] = -r3 / xv + r4 / (C - xv);
   }
 }
 
-// x, s, xi += 0.99 alpha step; y += 0.99 alpha (dy + dyc)   (1769-1777).  Skipped when the
-// capacitance Cholesky failed (the reference returns the current iterate, 1711-1714).
+// x, s, xi += 0.99 alpha step; y += 0.99 alpha (dy + dyc)   (1769-1777); comb: the
+// corrector's dy slot holds the combined step's.  Skipped when the capacitance Cholesky
+// failed (the reference returns the current iterate, 1711-1714).
 __global__ void k_update(int64_t n, double* __restrict__ x, double* __restrict__ s, double* __restrict__ xi,
                          const double* __restrict__ dx, const double* __restrict__ ds,
-                         const double* __restrict__ dxi, double* __restrict__ sc, const int* __restrict__ info) {
+                         const double* __restrict__ dxi, double* __restrict__ sc, const int* __restrict__ info,
+                         int comb) {
   if (*info != 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) sc[SL_CHOLFAIL] = 1.0;
     return;
@@ -1028,17 +1055,18 @@ __global__ void k_update(int64_t n, double* __restrict__ x, double* __restrict__
     s[i] += al * ds[i];
     xi[i] += al * dxi[i];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[SL_Y] += al * (sc[SL_DY] + sc[SL_DYC]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[SL_Y] += al * (comb ? sc[SL_DYC] : sc[SL_DY] + sc[SL_DYC]);
 }
 
 // Z^T x after k_update: x += 0.99 alpha (dx_aff + dx_corr), and T3a, T3 hold Z^T of the two
-// steps (skipped, as k_update, after a failed factorisation)
+// steps (T3a null: T3 holds Z^T of the combined step); skipped, as k_update, after a failed
+// factorisation
 __global__ void k_carry_tx(int kp, double* __restrict__ Tx, const double* __restrict__ T3a,
                            const double* __restrict__ T3c, const double* __restrict__ sc,
                            const int* __restrict__ info) {
   if (*info != 0) return;
   const double al = 0.99 * sc[SL_ALPHA];
-  for (int e = threadIdx.x; e < kp; e += blockDim.x) Tx[e] += al * (T3a[e] + T3c[e]);
+  for (int e = threadIdx.x; e < kp; e += blockDim.x) Tx[e] += al * ((T3a ? T3a[e] : 0.0) + T3c[e]);
 }
 
 // M += I (kp x kp) and copy lower to upper is not needed (Cholesky reads the lower part)
@@ -1210,7 +1238,11 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   //   rho = [-r5 - (A dx - a dy); r2 - a^T dx];  (ddx, ddy) from the same Schur
   //   elimination; dx += ddx, dy += ddy.  T3 = Z^T sol3 comes in from the linear
   //   combination that formed sol3, so the residual pass reads Z once.
-  const int border = prm->refine < 0 ? 1 : (prm->refine > 0 ? prm->refine : 0);
+  // refine = m > 0: m sweeps for the affine step and m for the corrector's; refine = -m:
+  // the affine step is left as solved (it only sets the centring sigma and the corrector's
+  // second-order term) and m sweeps refine the combined step, the one taken
+  const bool comb = prm->refine < 0;
+  const int border = comb ? -prm->refine : prm->refine;
   // With refinement, Z^T of each step is known from the passes that form it (T3 of the
   // predictor and of the corrector, the last sweep's Z^T s5 included), so Z^T x is carried
   // from iteration to iteration as Z^T x += alpha (T3a + T3) instead of read from Z again.
@@ -1393,8 +1425,9 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     }
     TSVM_CHECK(smw(2, solp, Tr7, {SL_AR7, SL_AB}, fuse_res));  // r7 = S^-1 r5, b = S^-1 a
     hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 0, (const double*)(solp + n), (const double*)solp,
-                       sc, sol3, kp, (const double*)(border ? Tr7 : nullptr), (const double*)Tb, T3a);
-    TSVM_CHECK(border_refine(0, T3a));
+                       sc, sol3, kp, (const double*)(border ? Tr7 : nullptr), (const double*)Tb, T3a,
+                       (const double*)nullptr, (const double*)nullptr);
+    if (!comb) TSVM_CHECK(border_refine(0, T3a));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 0, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
                        (const double*)ads, (const double*)adxi, red);
@@ -1405,21 +1438,27 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     TSVM_CHECK(reduce({RED_SUM}, {SL_MUAFF}));
     // ---- corrector (1750-1777)
     hipLaunchKernelGGL(k_rhs_corr, eg, eb, 0, st, n, C, (const double*)x, dx, ds, dxi, adx, ads, adxi,
-                       (const double*)sc, rhs);
-    hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, 1, (const double*)rhs, n, (const double*)winv, V, n);
+                       (const double*)sc, rhs, comb ? 1 : 0, (const double*)s, (const double*)xi,
+                       (const double*)winv, V);
+    if (!comb)
+      hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, 1, (const double*)rhs, n, (const double*)winv, V, n);
     TSVM_CHECK(smw(1, solc, border ? Tc : nullptr, {SL_AR7C}));
-    hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 1, (const double*)(solp + n), (const double*)solc,
-                       sc, sol3, kp, (const double*)(border ? Tc : nullptr), (const double*)Tb, T3);
-    TSVM_CHECK(border_refine(1, T3));
-    hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 1, (const double*)x, (const double*)s,
+    // comb: sol3, T3 and the corrector's dy become the combined step's, which the sweeps
+    // refine against the combined right-hand sides and k_step completes as one step
+    hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, comb ? 2 : 1, (const double*)(solp + n),
+                       (const double*)solc, sc, sol3, kp, (const double*)(border ? Tc : nullptr),
+                       (const double*)Tb, T3, (const double*)adx, (const double*)T3a);
+    TSVM_CHECK(border_refine(comb ? 2 : 1, T3));
+    hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, comb ? 0 : 1, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
                        (const double*)ads, (const double*)adxi, red);
     TSVM_CHECK(reduce({RED_MIN}, {SL_ALPHA}));
     hipLaunchKernelGGL(k_update, eg, eb, 0, st, n, x, s, xi, (const double*)dx, (const double*)ds,
-                       (const double*)dxi, sc, (const int*)dinfo);
+                       (const double*)dxi, sc, (const int*)dinfo, comb ? 1 : 0);
     if (carry_tx)
-      hipLaunchKernelGGL(k_carry_tx, dim3(1), dim3(256), 0, st, kp, Tx, (const double*)T3a,
-                         (const double*)T3, (const double*)sc, (const int*)dinfo);
+      hipLaunchKernelGGL(k_carry_tx, dim3(1), dim3(256), 0, st, kp, Tx,
+                         (const double*)(comb ? nullptr : T3a), (const double*)T3, (const double*)sc,
+                         (const int*)dinfo);
     TSVM_HIP(c, hipGetLastError());
   }
   TSVM_HIP(c, hipStreamSynchronize(st));
@@ -1456,7 +1495,7 @@ void tsvm_ipm_default_params(tsvm_ipm_params* p) {
   p->stop_mode = TSVM_STOP_REF;
   p->tol = 1e-5;
   p->precision = TSVM_PREC_FP32;
-  p->refine = 1;
+  p->refine = -1;
   p->verbose = 0;
   p->init = 0;
 }

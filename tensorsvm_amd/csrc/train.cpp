@@ -153,7 +153,7 @@ void tsvm_train_default_params(tsvm_train_params* p) {
   p->stop_mode = TSVM_STOP_REF;
   p->tol = 1e-5;
   p->maxit = 100;
-  p->refine = 1;
+  p->refine = -1;
   p->verbose = 0;
   p->init = 0;
   p->analysis = 0;

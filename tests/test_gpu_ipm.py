@@ -38,10 +38,11 @@ def test_ipm_fp32_vs_sklearn_fixture(ctx, name):
                                    (4000, 160, 8.0), (6000, 200, 32.0), (3000, 72, 4.0),
                                    (4000, 256, 8.0), (4000, 300, 32.0), (3000, 600, 8.0),
                                    (1200, 1000, 8.0)])
-def test_ipm_fp32_matches_oracle(ctx, n, k, C):
+@pytest.mark.parametrize("refine", [1, -1])
+def test_ipm_fp32_matches_oracle(ctx, n, k, C, refine):
     import tensorsvm_amd as T
     Z, y = _problem(n, k, n + k)
-    al, xi, s, info = ctx.ipm(Z, y, C, precision=T.PREC_FP32)
+    al, xi, s, info = ctx.ipm(Z, y, C, precision=T.PREC_FP32, refine=refine)
     Xo, Xio, So, io = O.mpc(Z.astype(np.float64), y.astype(np.float64), C)
     assert info.reason == io.reason
     assert np.abs(al - Xo).max() <= 1e-5 * C
@@ -53,13 +54,16 @@ def test_ipm_fp32_matches_oracle(ctx, n, k, C):
     assert np.all((sv_g == sv_o) | amb)
 
 
+# refine = 2: two sweeps for each of the affine and the corrector's steps; -1: one sweep
+# for the combined step only (the affine step as solved)
+@pytest.mark.parametrize("refine", [2, -1])
 @pytest.mark.parametrize("n,k,C", [(5000, 32, 32.0), (8000, 100, 8.0), (6000, 200, 32.0),
                                    (4000, 300, 32.0), (3000, 600, 8.0)])
-def test_ipm_bf16_refined_matches_oracle(ctx, n, k, C):
+def test_ipm_bf16_refined_matches_oracle(ctx, n, k, C, refine):
     import tensorsvm_amd as T
     Z, y = _problem(n, k, 7 * n + k)
-    al, xi, s, info = ctx.ipm(Z, y, C, precision=T.PREC_BF16, refine=2, stop=T.capi.STOP_GAP,
-                              tol=1e-7)
+    al, xi, s, info = ctx.ipm(Z, y, C, precision=T.PREC_BF16, refine=refine,
+                              stop=T.capi.STOP_GAP, tol=1e-7)
     Xo, Xio, So, io = O.mpc(Z.astype(np.float64), y.astype(np.float64), C, stop="gap", tol=1e-7)
     assert info.reason == 0, info.as_dict()
     # the hybrid capacitance put the bulk rows on BF16 in the later iterations
