@@ -1,10 +1,15 @@
 // FP64 Cholesky of the k x k capacitance matrix M = I + Z^T D^-1 Z and its solves, on
 // the GPU (replaces LAPACKE_dpotrf, tensorsvm-train.cu:1709, and LAPACKE_dpotrs, 1834).
 //
-// Blocked right-looking factorisation with 32-column panels; per panel three launches:
-//   chol_diag    one wave factors the 32 x 32 diagonal block in registers and inverts it
-//   chol_panel   rows below: L_i = A_i L_jj^{-T} (one thread per entry)
-//   chol_update  trailing lower triangle A -= L_p L_p^T (one block per 32 x 32 tile)
+// Blocked right-looking factorisation with 32-column panels, one launch per panel:
+//   chol_diag    (first panel only) one wave factors the 32 x 32 diagonal block in
+//                registers and inverts it
+//   chol_step    block (ti, tl) of the trailing lower triangle forms the two panel tiles it
+//                needs, L_i = A_i L_jj^{-T}, subtracts L_ti L_tl^T from its tile, and the
+//                diagonal blocks write their panel tile out; block (0, 0) then factors the
+//                next diagonal block from its updated tile, so the next launch has it
+// The factor is built in a workspace copy (the panel columns of M are read by every block
+// of a launch while the diagonal blocks write theirs); chol_finish copies it into M.
 // A failure (non-positive pivot, as dpotrf's info > 0) is recorded in a device int and
 // every later launch (and chol_solve_f64) becomes a no-op, so the host reads one flag.
 // The inverted diagonal blocks and a transposed copy of the factor are kept for the
@@ -23,131 +28,158 @@ __device__ inline double readlane_f64(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// One wave factors the 32 x 32 diagonal block in registers (lane i < 32 holds row i) and
-// inverts the factor (lane c < 32 then holds column c of the inverse); the entries
-// another lane needs come by readlane, so there is no barrier.  Rows past the matrix
-// end are identity.
-__global__ __launch_bounds__(64) void chol_diag(double* __restrict__ M, int64_t k, int64_t j0,
-                                                double* __restrict__ Dinv, int* __restrict__ info) {
-  if (*info != 0) return;
-  const int lane = threadIdx.x;
+// Factors the diagonal block at (j0, j0): A points at its first entry (row stride lda, in
+// global memory or LDS), the factor goes to Lw (k x k, row-major) and its inverse to Dinv.
+// Called by every thread of a block of at least 64 threads (it has a block barrier); the
+// first wave computes.  Lane i < 32 holds row i and the entries another lane needs come by
+// readlane, so the factorisation needs no barrier.  The inverse is formed in the same loop:
+// lane c keeps column c of L^-1 as partial sums s_i (starting from e_c); once column j of
+// L is final, x_j = s_j / L_jj and s_i -= L_ij x_j for i > j, with the L_ij the
+// factorisation has just read.  Rows past the matrix end are identity.
+__device__ __forceinline__ void chol_diag_dev(const double* A, int64_t lda, double* __restrict__ Lw,
+                                              int64_t k, int64_t j0, double* __restrict__ Dinv,
+                                              int* __restrict__ info) {
+  __shared__ int sbad;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const bool w0 = tid < 64;
   const int jb = (int)(k - j0 < CB ? k - j0 : CB);
-  const bool row = lane < jb;
-  double a[CB];
+  const bool row = w0 && lane < jb;
+  if (w0) {
+    double a[CB], sx[CB];
 #pragma unroll
-  for (int t = 0; t < CB; t++)
-    a[t] = row && t < jb ? M[(j0 + lane) * k + j0 + t] : (lane == t ? 1.0 : 0.0);
-  int bad = 0;
-#pragma unroll
-  for (int j = 0; j < CB; j++) {
-    const double d = readlane_f64(a[j], j);
-    if (!(d > 0.0)) {
-      bad = j + 1;
-      break;
+    for (int t = 0; t < CB; t++) {
+      a[t] = row && t < jb ? A[lane * lda + t] : (lane == t ? 1.0 : 0.0);
+      sx[t] = lane == t ? 1.0 : 0.0;
     }
-    // 1/sqrt(d): the hardware estimate and two Newton steps y <- y (3 - d y^2) / 2 (each
-    // doubles the correct bits), so the pivot step needs no division on its critical path
-    double rs = __builtin_amdgcn_rsq(d);
-    rs = rs * (1.5 - 0.5 * d * rs * rs);
-    rs = rs * (1.5 - 0.5 * d * rs * rs);
-    const double s = d * rs;
-    a[j] = lane == j ? s : (lane > j ? a[j] * rs : a[j]);
+    int bad = 0;
 #pragma unroll
-    for (int l = j + 1; l < CB; l++) {
-      const double llj = readlane_f64(a[j], l);
-      if (lane >= l) a[l] -= a[j] * llj;
+    for (int j = 0; j < CB; j++) {
+      const double d = readlane_f64(a[j], j);
+      if (!(d > 0.0)) {
+        bad = j + 1;
+        break;
+      }
+      // 1/sqrt(d): the hardware estimate and two Newton steps y <- y (3 - d y^2) / 2 (each
+      // doubles the correct bits), so the pivot step needs no division on its critical path;
+      // rs is also 1 / L_jj for the inverse
+      double rs = __builtin_amdgcn_rsq(d);
+      rs = rs * (1.5 - 0.5 * d * rs * rs);
+      rs = rs * (1.5 - 0.5 * d * rs * rs);
+      // Every lane updates every entry: those above a lane's diagonal (t > lane) are never
+      // read (the loop reads a[j] only on lanes l >= j, and only t <= lane is written out),
+      // so the updates need no lane masks; on lane j, a[j] rs = d rs is L_jj.
+      a[j] *= rs;
+      const double xj = sx[j] * rs;
+      sx[j] = xj;
+#pragma unroll
+      for (int l = j + 1; l < CB; l++) {
+        const double llj = readlane_f64(a[j], l);
+        a[l] -= a[j] * llj;
+        sx[l] -= llj * xj;
+      }
     }
-  }
-  if (bad) {
-    if (lane == 0) *info = (int)(j0 + bad);
-    return;
-  }
-  if (row)
+    if (lane == 0) sbad = bad;  // wave-uniform
+    if (!bad && row)
 #pragma unroll
-    for (int t = 0; t < CB; t++)
-      if (t < jb) M[(j0 + lane) * k + j0 + t] = t <= lane ? a[t] : 0.0;
-  // column c of the inverse: Li[i][c] = (e_c[i] - sum_{t < i} L[i][t] Li[t][c]) / L[i][i],
-  // with Li[t][c] = 0 above the diagonal (t < c).  Lane c keeps its column in registers
-  // (unrolled loops, no LDS round trip between the dependent steps); the factor and its
-  // reciprocal diagonal are read from LDS, every lane at the same address.
-  __shared__ double Ls[CB][CB + 1];
-  __shared__ double Rd[CB];
-  if (lane < CB) {
+      for (int t = 0; t < CB; t++)
+        if (t < jb) Lw[(j0 + lane) * k + j0 + t] = t <= lane ? a[t] : 0.0;
+    if (!bad && lane < CB)
 #pragma unroll
-    for (int t = 0; t < CB; t++) Ls[lane][t] = a[t];
-    double dl = 0.0;  // L[lane][lane]
-#pragma unroll
-    for (int t = 0; t < CB; t++)
-      if (t == lane) dl = a[t];
-    Rd[lane] = 1.0 / dl;
+      for (int i = 0; i < CB; i++) Dinv[(j0 / CB) * CB * CB + i * CB + lane] = sx[i];
   }
   __syncthreads();
-  if (lane < CB) {
-    const int c = lane;
-    double x[CB];
-#pragma unroll
-    for (int i = 0; i < CB; i++) {
-      double v = i == c ? 1.0 : 0.0;
-#pragma unroll
-      for (int t = 0; t < i; t++) v -= Ls[i][t] * x[t];
-      x[i] = i >= c ? v * Rd[i] : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < CB; i++) Dinv[(j0 / CB) * CB * CB + i * CB + c] = x[i];
-  }
+  if (sbad && tid == 0) *info = (int)(j0 + sbad);
 }
 
-// rows below the diagonal block: L_i[c] = sum_{t <= c} A_i[t] Linv[c][t]; a block takes 8
-// rows, one thread per (row, column)
-__global__ __launch_bounds__(256) void chol_panel(double* __restrict__ M, int64_t k, int64_t j0,
-                                                  const double* __restrict__ Dinv,
-                                                  const int* __restrict__ info) {
+// the first diagonal block
+__global__ __launch_bounds__(64) void chol_diag(const double* __restrict__ M, double* __restrict__ Lw,
+                                                int64_t k, double* __restrict__ Dinv,
+                                                int* __restrict__ info) {
   if (*info != 0) return;
+  chol_diag_dev(M, k, Lw, k, 0, Dinv, info);
+}
+
+// Panel j0 (its diagonal block factored: inverse in Dinv) and the trailing update.  Block
+// x covers tile (ti, tl), tl <= ti, x = ti (ti + 1) / 2 + tl, of the trailing lower
+// triangle (rows and columns from j0 + 32).  It forms the panel tiles of its row and
+// column blocks, P[r][c] = sum_{t <= c} A[r][t] Linv[c][t] (A: M's columns j0 .. j0 + 31),
+// and subtracts P_ti P_tl^T from its tile of M; the diagonal blocks write their panel tile
+// to Lw.  Block 0 keeps its updated tile, the next diagonal block, in LDS and factors it.
+__global__ __launch_bounds__(256) void chol_step(double* __restrict__ M, double* __restrict__ Lw,
+                                                 int64_t k, int64_t j0, double* __restrict__ Dinv,
+                                                 int* __restrict__ info) {
+  if (*info != 0) return;
+  const int x = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= x) ti++;
+  while (ti * (ti + 1) / 2 > x) ti--;
+  const int tl = x - ti * (ti + 1) / 2;
+  const bool dg = ti == tl;
   __shared__ double Li[CB][CB + 1];
-  __shared__ double Ar[8][CB + 1];
-  const double* D = Dinv + (j0 / CB) * CB * CB;
-  for (int e = threadIdx.x; e < CB * CB; e += 256) Li[e / CB][e % CB] = D[e];
-  const int r = threadIdx.x / CB, c = threadIdx.x % CB;
-  const int64_t i = j0 + CB + blockIdx.x * 8 + r;
-  Ar[r][c] = i < k ? M[i * k + j0 + c] : 0.0;
-  __syncthreads();
-  if (i >= k) return;
-  double v = 0.0;
-#pragma unroll
-  for (int t = 0; t < CB; t++)
-    if (t <= c) v += Ar[r][t] * Li[c][t];
-  M[i * k + j0 + c] = v;
-}
-
-__global__ __launch_bounds__(256) void chol_update(double* __restrict__ M, int64_t k, int64_t j0,
-                                                   const int* __restrict__ info) {
-  if (*info != 0) return;
-  const int ti = blockIdx.x, tl = blockIdx.y;
-  if (tl > ti) return;
   __shared__ double Pi[CB][CB + 1];
   __shared__ double Pl[CB][CB + 1];
+  __shared__ double Tn[CB][CB + 1];  // block 0: the updated tile
+  const double* D = Dinv + (j0 / CB) * CB * CB;
   const int64_t r0 = j0 + CB;
   const int64_t ib = r0 + (int64_t)ti * CB, lb = r0 + (int64_t)tl * CB;
-  for (int e = threadIdx.x; e < CB * CB; e += 256) {
-    int r = e / CB, t = e % CB;
-    Pi[r][t] = (ib + r < k) ? M[(ib + r) * k + j0 + t] : 0.0;
-    Pl[r][t] = (lb + r < k) ? M[(lb + r) * k + j0 + t] : 0.0;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < CB * CB; e += 256) {
+    const int r = e / CB, t = e % CB;
+    Li[r][t] = D[e];
+    Pi[r][t] = ib + r < k ? M[(ib + r) * k + j0 + t] : 0.0;
+    if (!dg) Pl[r][t] = lb + r < k ? M[(lb + r) * k + j0 + t] : 0.0;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < CB * CB; e += 256) {
-    int r = e / CB, q = e % CB;
-    int64_t i = ib + r, l = lb + q;
-    if (i >= k || l >= k || l > i) continue;
+  constexpr int PER = CB * CB / 256;
+  double pv[PER], qv[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int e = tid + 256 * u, r = e / CB, c = e % CB;
+    double v = 0.0, w = 0.0;
+#pragma unroll
+    for (int t = 0; t < CB; t++)
+      if (t <= c) {
+        v += Pi[r][t] * Li[c][t];
+        if (!dg) w += Pl[r][t] * Li[c][t];
+      }
+    pv[u] = v;
+    qv[u] = w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int e = tid + 256 * u, r = e / CB, c = e % CB;
+    Pi[r][c] = pv[u];
+    if (!dg) Pl[r][c] = qv[u];
+    if (dg && ib + r < k) Lw[(ib + r) * k + j0 + c] = pv[u];
+  }
+  __syncthreads();
+  const double(*Q)[CB + 1] = dg ? Pi : Pl;
+  for (int e = tid; e < CB * CB; e += 256) {
+    const int r = e / CB, q = e % CB;
+    const int64_t i = ib + r, l = lb + q;
+    if (i >= k || l >= k || l > i) {
+      if (x == 0) Tn[r][q] = 0.0;
+      continue;
+    }
     double v = 0.0;
 #pragma unroll
-    for (int t = 0; t < CB; t++) v += Pi[r][t] * Pl[q][t];
-    M[i * k + l] -= v;
+    for (int t = 0; t < CB; t++) v += Pi[r][t] * Q[q][t];
+    const double nv = M[i * k + l] - v;
+    M[i * k + l] = nv;
+    if (x == 0) Tn[r][q] = nv;
+  }
+  if (x == 0) {
+    __syncthreads();
+    chol_diag_dev(&Tn[0][0], CB + 1, Lw, k, r0, Dinv, info);
   }
 }
 
-// zero the strict upper triangle and write Lt = L^T (32 x 32 tiles through LDS)
-__global__ __launch_bounds__(256) void chol_finish(double* __restrict__ M, int64_t k,
-                                                   double* __restrict__ Lt,
+// M = the factor (strict upper triangle zero) and Lt = L^T, from the factor's workspace
+// copy (32 x 32 tiles through LDS)
+__global__ __launch_bounds__(256) void chol_finish(double* __restrict__ M, const double* __restrict__ Lw,
+                                                   int64_t k, double* __restrict__ Lt,
                                                    const int* __restrict__ info) {
   if (*info != 0) return;
   __shared__ double tile[CB][CB + 1];
@@ -157,8 +189,8 @@ __global__ __launch_bounds__(256) void chol_finish(double* __restrict__ M, int64
     const int64_t i = i0 + r, j = j0 + c;
     double v = 0.0;
     if (i < k && j < k) {
-      v = j <= i ? M[i * k + j] : 0.0;
-      if (j > i) M[i * k + j] = 0.0;
+      v = j <= i ? Lw[i * k + j] : 0.0;
+      M[i * k + j] = v;
     }
     tile[r][c] = v;
   }
@@ -172,23 +204,22 @@ __global__ __launch_bounds__(256) void chol_finish(double* __restrict__ M, int64
 __global__ void set_int(int* p, int v) { *p = v; }
 
 tsvm_status chol_f64(Ctx* c, double* M, int64_t k, int* info) {
-  void* dinv;
+  void *dinv, *lw, *lt;
   TSVM_HIP(c, c->ws.get("chol_dinv", (size_t)(ceil_div(k, CB) * CB * CB * 8), &dinv));
+  TSVM_HIP(c, c->ws.get("chol_lw", (size_t)(k * k * 8), &lw));
+  TSVM_HIP(c, c->ws.get("chol_lt", (size_t)(k * k * 8), &lt));
   hipStream_t s = c->stream;
   hipLaunchKernelGGL(set_int, dim3(1), dim3(1), 0, s, info, 0);
-  for (int64_t j0 = 0; j0 < k; j0 += CB) {
-    hipLaunchKernelGGL(chol_diag, dim3(1), dim3(64), 0, s, M, k, j0, (double*)dinv, info);
-    const int64_t rest = k - j0 - CB;
-    if (rest <= 0) break;
-    hipLaunchKernelGGL(chol_panel, dim3((unsigned)ceil_div(rest, 8)), dim3(256), 0, s, M, k, j0,
-                       (const double*)dinv, (const int*)info);
-    const unsigned T = (unsigned)ceil_div(rest, CB);
-    hipLaunchKernelGGL(chol_update, dim3(T, T), dim3(256), 0, s, M, k, j0, (const int*)info);
+  hipLaunchKernelGGL(chol_diag, dim3(1), dim3(64), 0, s, (const double*)M, (double*)lw, k,
+                     (double*)dinv, info);
+  for (int64_t j0 = 0; j0 + CB < k; j0 += CB) {
+    const int64_t T = ceil_div(k - j0 - CB, CB);
+    hipLaunchKernelGGL(chol_step, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, M,
+                       (double*)lw, k, j0, (double*)dinv, info);
   }
-  void* lt;
-  TSVM_HIP(c, c->ws.get("chol_lt", (size_t)(k * k * 8), &lt));
   const unsigned T = (unsigned)ceil_div(k, CB);
-  hipLaunchKernelGGL(chol_finish, dim3(T, T), dim3(256), 0, s, M, k, (double*)lt, (const int*)info);
+  hipLaunchKernelGGL(chol_finish, dim3(T, T), dim3(256), 0, s, M, (const double*)lw, k,
+                     (double*)lt, (const int*)info);
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
 }

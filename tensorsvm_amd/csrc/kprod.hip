@@ -1358,12 +1358,15 @@ static tsvm_status kernel_product_impl(Ctx* c, KpBuffers& bufs, const float* X1,
     const int64_t BN = NT * 32, m_pad = round_up(m, 128), r_pad = round_up(r, BN);
     // FP32 sums over segments of 8192 columns (the reference's tile, whose products it
     // accumulates with beta = 1, 2537-2555), added in FP64 in segment order, bounded by
-    // 8 GiB of partial sums: at most m r nseg floats.  Where that bound binds (m r > 2^28:
-    // 1.6M x 512 takes two segments of 800k columns) each FP32 sum spans more columns; the
-    // reference accumulates all n columns of a product in FP32.
+    // 8 GiB of partial sums: at most max(m, n) r nseg floats.  Where that bound binds
+    // (max(m, n) r > 2^28: 1.6M x 512 takes two segments of 800k columns) each FP32 sum
+    // spans more columns; the reference accumulates all n columns of a product in FP32.
+    // The bound takes the larger of m and n so that a product computed in row slabs (m < n)
+    // is segmented, and rounded, exactly as the whole one.
     const int64_t seg_len = 256;  // chunks of 32 columns
     int64_t nseg = ceil_div(n_chunks, seg_len);
-    const int64_t max_seg = (int64_t)(8.0 * (1ll << 30) / ((double)m * r * 4.0));
+    const double seg_rows = (double)(m > n ? m : n);
+    const int64_t max_seg = (int64_t)(8.0 * (1ll << 30) / (seg_rows * r * 4.0));
     if (nseg > max_seg) nseg = max_seg > 1 ? max_seg : 1;
     const int64_t seg_chunks = ceil_div(n_chunks, nseg);
     nseg = ceil_div(n_chunks, seg_chunks);

@@ -102,7 +102,7 @@ tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
 
 // W = K_Y A over all n rows.  With progress lines on, the rows go in slabs of about
 // 2e15 flops (multiples of 256 rows), one stderr line per slab; each row's result does
-// not depend on the slabbing.
+// not depend on the slabbing (the FP32 product's column segments follow n, not the slab).
 static tsvm_status product_rows(Ctx* c, const char* name, const float* X, const float* y,
                                 int64_t n, int64_t d, const float* A, int64_t k, double gamma,
                                 int prec, float* W) {
