@@ -232,7 +232,9 @@ tsvm_status chol_f64(Ctx* c, double* M, int64_t k, int* info) {
 // At the start of a step every thread issues its loads at once (one entry of Linv_bb
 // into LDS, the 32 entries of L its row or column needs, read from Lt = L^T in the
 // forward sweep so that a wave's loads are coalesced), so a step costs one round of load
-// latency.  Needs k - 32 <= 1024 (one row or column per thread).
+// latency.  Needs k - 32 <= 1024 (one row or column per thread).  The block products run
+// over all 32 entries without branches: Linv_bb is stored with exact zeros above its
+// diagonal and in the identity rows and columns of a partial last block.
 constexpr int SOLVE_THREADS = 1024;
 template <int p>
 __global__ __launch_bounds__(SOLVE_THREADS) void chol_solve_kernel(
@@ -242,7 +244,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void chol_solve_kernel(
   extern __shared__ double smem[];
   double* X = smem;              // p x k
   double* Y = smem + 4 * k;      // [4][CB] the current block
-  double* Db = Y + 4 * CB;       // [CB][CB] Linv_bb
+  double* Db = Y + 4 * CB;       // [CB][CB + 1] Linv_bb (padded rows: the forward block
+                                 // product reads a row per thread)
   const int tid = threadIdx.x;
   const int q = tid / CB, rr = tid % CB;  // the block solve: q < p, rr < jb
   for (int64_t e = tid; e < (int64_t)p * k; e += SOLVE_THREADS) X[e] = B[e];
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void chol_solve_kernel(
     const int64_t j0 = b * CB;
     const int jb = (int)(k - j0 < CB ? k - j0 : CB);
     const int64_t r = j0 + jb + tid;  // this thread's row below the block
-    Db[tid] = Dinv[b * CB * CB + tid];
+    Db[(tid / CB) * (CB + 1) + tid % CB] = Dinv[b * CB * CB + tid];
     if (r < k)
 #pragma unroll
       for (int t = 0; t < CB; t++) l[t] = t < jb ? Lt[(j0 + t) * k + r] : 0.0;
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void chol_solve_kernel(
       double s = 0.0;
 #pragma unroll
       for (int t = 0; t < CB; t++)
-        if (t <= rr) s += Db[rr * CB + t] * X[q * k + j0 + t];
+        s += Db[rr * (CB + 1) + t] * X[q * k + j0 + (t < jb ? t : 0)];
       Y[q * CB + rr] = s;
     }
     __syncthreads();
@@ -281,7 +284,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void chol_solve_kernel(
     const int64_t j0 = b * CB;
     const int jb = (int)(k - j0 < CB ? k - j0 : CB);
     const int64_t cc = tid;  // this thread's column before the block
-    Db[tid] = Dinv[b * CB * CB + tid];
+    Db[(tid / CB) * (CB + 1) + tid % CB] = Dinv[b * CB * CB + tid];
     if (cc < j0)
 #pragma unroll
       for (int t = 0; t < CB; t++) l[t] = t < jb ? L[(j0 + t) * k + cc] : 0.0;
@@ -290,7 +293,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void chol_solve_kernel(
       double s = 0.0;
 #pragma unroll
       for (int t = 0; t < CB; t++)
-        if (t >= rr && t < jb) s += Db[t * CB + rr] * X[q * k + j0 + t];
+        s += Db[t * (CB + 1) + rr] * X[q * k + j0 + (t < jb ? t : 0)];
       Y[q * CB + rr] = s;
     }
     __syncthreads();
@@ -315,7 +318,7 @@ tsvm_status chol_solve_f64(Ctx* c, const double* L, int64_t k, double* B, int64_
   void* dinv;
   TSVM_HIP(c, c->ws.get("chol_dinv", (size_t)(ceil_div(k, CB) * CB * CB * 8), &dinv));
   if (k - CB > SOLVE_THREADS) return fail(c, TSVM_ERR_ARG, "chol_solve: k=%ld too large", (long)k);
-  const size_t sm = (size_t)(4 * k + 4 * CB + CB * CB) * 8;
+  const size_t sm = (size_t)(4 * k + 4 * CB + CB * (CB + 1)) * 8;
   void* lt;
   TSVM_HIP(c, c->ws.get("chol_lt", (size_t)(k * k * 8), &lt));
   const double* di = (const double*)dinv;
