@@ -76,15 +76,22 @@ __device__ inline void block_reduce_store(double (&v)[R], const int (&op)[R], do
   }
 }
 
+// which reductions a pass left in its per-block partials, and where their results go
+// (passed by value: no host-to-device copy per reduction)
+struct RedSpec {
+  int n;
+  int op[MAXR];
+  int slot[MAXR];
+};
+
 // sc[slot[r]] = reduction over nblk blocks of red[b * R + r]  (one block, fixed order)
 __global__ __launch_bounds__(256) void reduce_slots(const double* __restrict__ red, int nblk,
-                                                    int R, const int* __restrict__ ops,
-                                                    const int* __restrict__ slots,
-                                                    double* __restrict__ sc) {
+                                                    const RedSpec rs, double* __restrict__ sc) {
   __shared__ double sh[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int R = rs.n;
   for (int r = 0; r < R; r++) {
-    const int op = ops[r];
+    const int op = rs.op[r];
     double x = op == RED_SUM ? 0.0 : (op == RED_MAX ? -INFINITY : INFINITY);
     for (int b = threadIdx.x; b < nblk; b += 256) {
       double y = red[b * R + r];
@@ -97,7 +104,7 @@ __global__ __launch_bounds__(256) void reduce_slots(const double* __restrict__ r
       double z = sh[0];
       for (int w = 1; w < 4; w++)
         z = op == RED_SUM ? z + sh[w] : (op == RED_MAX ? fmax(z, sh[w]) : fmin(z, sh[w]));
-      sc[slots[r]] = z;
+      sc[rs.slot[r]] = z;
     }
     __syncthreads();
   }
@@ -1151,7 +1158,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   };
   double *a, *e, *dx, *ds, *dxi, *adx, *ads, *adxi, *D, *winv, *rhs, *V, *solp, *solc, *T, *M,
       *sc, *red, *s, *sol3;
-  int *dinfo, *tables;
+  int* dinfo;
 #define WS(name, bytes, var)              \
   TSVM_CHECK(ws(name, (bytes), &p));      \
   var = (decltype(var))p;
@@ -1175,7 +1182,6 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   WS("ipm_sc", IPM_NSLOT * 8, sc);
   WS("ipm_red", EW_BLOCKS * MAXR * 8, red);
   WS("ipm_info", 16, dinfo);
-  WS("ipm_tables", 2 * MAXR * 4, tables);
   if (s_out) {
     s = s_out;
   } else {
@@ -1196,15 +1202,13 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, (int64_t)IPM_NSLOT, sc, 0.0);
 
   auto reduce = [&](std::vector<int> ops, std::vector<int> slots, int nblk = EW_BLOCKS) -> tsvm_status {
-    int R = (int)ops.size();
-    int buf[2 * MAXR] = {0};
-    for (int r = 0; r < R; r++) {
-      buf[r] = ops[r];
-      buf[MAXR + r] = slots[r];
+    RedSpec rs{};
+    rs.n = (int)ops.size();
+    for (int r = 0; r < rs.n; r++) {
+      rs.op[r] = ops[r];
+      rs.slot[r] = slots[r];
     }
-    TSVM_HIP(c, hipMemcpyAsync(tables, buf, sizeof(buf), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(reduce_slots, dim3(1), dim3(256), 0, st, (const double*)red, nblk, R,
-                       (const int*)tables, (const int*)(tables + MAXR), sc);
+    hipLaunchKernelGGL(reduce_slots, dim3(1), dim3(256), 0, st, (const double*)red, nblk, rs, sc);
     return TSVM_OK;
   };
   const int zblk = zpass_blocks(c);
