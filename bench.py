@@ -340,15 +340,18 @@ def main():
 
     # roofline of the dominant kernel.  BF16 form: both products of the factor run the
     # same kernel (fp16 product), so its average product is the mean of the two; the
-    # FP32 form likewise runs kprod_f32 twice.  The wide kernel runs a product as
-    # ceil(tiles / CUs) launches of one tile per CU (256-row x 256-column tiles): the
-    # figures are per product, the sum of its launches (HIP events on the library stream).
+    # FP32 form likewise runs kprod_f32 twice.  The wide kernel (256-row x 256-column
+    # tiles) runs a product with at least two rounds of tiles as one cooperative launch of
+    # one block per CU on the XCD-round schedule, a smaller one as ceil(tiles / CUs)
+    # launches of one tile per CU: the figures are per product (HIP events on the library
+    # stream around its launches).
     flops = 2.0 * ntr * ntr * (d + k)
     t1, t2 = bf16_info["lra"]["t_product1"], bf16_info["lra"]["t_product2"]
     tr = KPROD_TRAFFIC
     if low:
         peak, kname = PEAK_BF16, ("kprod_wide<8,64,EM_F16,4,2> (fused RBF product, 256-row x "
-                                  "256-column tiles, 64 x 256 per wave, K_Y Omega and K_Y W)")
+                                  "256-column tiles, 64 x 256 per wave, one cooperative launch "
+                                  "per product with tiles taken per XCD; K_Y Omega and K_Y W)")
     else:
         peak, kname = PEAK_FP32, "kprod_f32 (fused RBF product, K_Y Omega and K_Y W)"
     t_kp = 0.5 * (t1 + t2)
@@ -359,7 +362,8 @@ def main():
                 "traffic": (tr["fetch_bytes"] + tr["write_bytes"]
                             if tr["fetch_bytes"] is not None and same_config else None),
                 "kernel": kname, "flops_per_product": flops, "product_s": round(t_kp, 4),
-                "launches_per_product": (-(-(-(-ntr // 256) * -(-k // 256)) // 256) if low else 1),
+                "launches_per_product": (1 if not low or -(-ntr // 256) * -(-k // 256) >= 512
+                                         else -(-(-(-ntr // 256) * -(-k // 256)) // 256)),
                 "products": {"K_Y Omega": round(t1, 4), "K_Y W": round(t2, 4)},
                 "traffic_source": tr["source"] if same_config else None}
 

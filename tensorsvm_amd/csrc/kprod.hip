@@ -610,6 +610,53 @@ __device__ __forceinline__ void mfma_bf16_vgpr_last(f32x16_t& d, bf16x8_t a, bf1
 // grew 30-fold and the clock fell from 1.9 to 1.6 GHz).  The outputs are bit-identical to
 // the 128-column kernel's (same operands, same accumulation order).
 // -------------------------------------------------------------------------------------
+// XCD-round schedule of the persistent wide launch.  Workgroups go to the eight XCDs
+// round-robin, so block b runs on XCD b % 8; the gs = gridDim.x / 8 blocks of an XCD take
+// the gs consecutive tiles of one unit together (they stream the same column chunks
+// through their XCD's L2), and the XCD's block 0 takes the next unit from a device-wide
+// counter once all gs have finished the last one: an XCD that runs faster takes more
+// units, where one launch per round of tiles waits for the slowest XCD every round.
+// sched (zeroed before the launch): [0] next unit, [1 + x] (round << 32) | unit published
+// for XCD x, [1 + 8 + x] tiles finished on XCD x, [17] error flag.  The waits are bounded
+// (the launch is cooperative, so every block is resident and they end); a wait that runs
+// out sets the flag and the blocks leave the loop.
+constexpr int KP_XCDS = 8;
+constexpr int KP_SCHED_WORDS = 2 + 2 * KP_XCDS;
+constexpr long long KP_SPIN_MAX = 1ll << 26;
+__device__ __forceinline__ unsigned long long kp_ld(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// called by one thread per block; returns the unit of this round, or -1 after a failed wait
+__device__ long long kp_next_unit(unsigned long long* sched, int xcd, int slot, int gs,
+                                  long long round) {
+  unsigned long long* pub = sched + 1 + xcd;
+  unsigned long long* done = sched + 1 + KP_XCDS + xcd;
+  unsigned long long* err = sched + 1 + 2 * KP_XCDS;
+  long long spins = 0;
+  if (slot == 0) {
+    while (kp_ld(done) < (unsigned long long)gs * (unsigned long long)round) {
+      if (++spins > KP_SPIN_MAX) {
+        atomicOr(err, 1ull);
+        return -1;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    const unsigned long long u = atomicAdd(sched, 1ull);
+    __hip_atomic_store(pub, ((unsigned long long)(round + 1) << 32) | u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    return (long long)u;
+  }
+  for (;;) {
+    const unsigned long long v = kp_ld(pub);
+    if ((v >> 32) >= (unsigned long long)(round + 1)) return (long long)(v & 0xffffffffull);
+    if (++spins > KP_SPIN_MAX) {
+      atomicOr(err, 1ull);
+      return -1;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
 template <int NT, int KD, int EM, int WAVES, int RB>
 __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __restrict__ P,
                                                       const __bf16* __restrict__ Qf,
@@ -618,7 +665,8 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
                                                       float* __restrict__ W, int64_t ldw,
                                                       int64_t m, int64_t r, int64_t n_chunks,
                                                       int64_t r_tiles, const float* __restrict__ sc,
-                                                      int64_t nxb, int64_t ntiles, int64_t tile0) {
+                                                      int64_t nxb, int64_t ntiles, int64_t tile0,
+                                                      unsigned long long* __restrict__ sched) {
   using ET = typename std::conditional<EM == EM_BF16, __bf16, _Float16>::type;
   using ET8 = typename std::conditional<EM == EM_BF16, bf16x8_t, f16x8_t>::type;
   constexpr int KS = KD / 16;
@@ -644,7 +692,30 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
   const int lane = tid & 63, wave = tid >> 6;
   const int l32 = lane & 31, h = lane >> 5;
 
-  for (int64_t tile = tile0 + blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // sched: the XCD-round schedule (one persistent launch); null: tiles tile0 + blockIdx.x,
+  // + gridDim.x, ... (one launch per round)
+  const bool dyn = sched != nullptr;
+  const int gs = (int)(gridDim.x / KP_XCDS);
+  const int xcd = (int)(blockIdx.x % KP_XCDS), slot = (int)(blockIdx.x / KP_XCDS);
+  const int64_t nunits = dyn ? (ntiles + gs - 1) / gs : 0;
+  __shared__ long long s_unit;
+  long long round = 0;
+  for (int64_t tile = tile0 + blockIdx.x;; tile += gridDim.x) {
+    if (dyn) {
+      if (tid == 0) s_unit = kp_next_unit(sched, xcd, slot, gs, round);
+      __syncthreads();
+      const long long u = s_unit;
+      __syncthreads();  // s_unit is rewritten next round
+      if (u < 0 || u >= nunits) break;
+      round++;
+      tile = u * gs + slot;
+      if (tile >= ntiles) {  // the last unit is partial: this block's share is empty
+        if (tid == 0) atomicAdd(sched + 1 + KP_XCDS + xcd, 1ull);
+        continue;
+      }
+    } else if (tile >= ntiles) {
+      break;
+    }
     const int64_t rowbase = (tile % nxb) * BR + wave * 32 * RB;
     const int64_t ct0 = (tile / nxb) * NT;
     bf16x8_t pb[RB][KS];
@@ -776,6 +847,7 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
         }
       }
     __syncthreads();
+    if (dyn && tid == 0) atomicAdd(sched + 1 + KP_XCDS + xcd, 1ull);
   }
 }
 
@@ -1141,23 +1213,51 @@ struct KpBuffers {
 // Widest operand (KD) of the wide-tile kernel
 #define KPW_MAX_KD 128
 
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+// One cooperative launch of cu_count blocks on the XCD-round schedule (sched: device words,
+// KP_SCHED_WORDS of them) when there are at least two rounds of tiles and cu_count splits
+// into the eight XCDs; otherwise (or if the cooperative launch is refused) one launch per
+// round of cu_count tiles (one block per CU).  *dyn_used tells which ran.
 template <int NT, int KD, int WAVES, int RB>
-static void launch_wide(int em, const __bf16* P, const __bf16* Qf, const void* Af,
-                        const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
-                        int64_t n_chunks, int64_t r_tiles, int64_t nxb, const float* sc,
-                        int cu_count, hipStream_t s) {
+static hipError_t launch_wide(int em, const __bf16* P, const __bf16* Qf, const void* Af,
+                              const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
+                              int64_t n_chunks, int64_t r_tiles, int64_t nxb, const float* sc,
+                              int cu_count, unsigned long long* sched, hipStream_t s,
+                              bool* dyn_used) {
   const int64_t ntiles = nxb * (r_tiles / NT);
-  // one launch per round of cu_count tiles (one block per CU)
+  *dyn_used = false;
+  if (sched && cu_count % KP_XCDS == 0 && ntiles >= 2 * (int64_t)cu_count) {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, sched, KP_SCHED_WORDS);
+    int64_t t0 = 0;
+    unsigned long long* sp = sched;
+    void* args[] = {(void*)&P, (void*)&Qf, (void*)&Af, (void*)&y1, (void*)&W, (void*)&ldw,
+                    (void*)&m, (void*)&r, (void*)&n_chunks, (void*)&r_tiles, (void*)&sc,
+                    (void*)&nxb, (void*)&ntiles, (void*)&t0, (void*)&sp};
+    const void* fn = em == EM_BF16 ? (const void*)&kprod_wide<NT, KD, EM_BF16, WAVES, RB>
+                                   : (const void*)&kprod_wide<NT, KD, EM_F16, WAVES, RB>;
+    const hipError_t e = hipLaunchCooperativeKernel(fn, dim3((unsigned)cu_count),
+                                                    dim3(64 * WAVES), args, 0, s);
+    if (e == hipSuccess) {
+      *dyn_used = true;
+      return hipSuccess;
+    }
+    (void)hipGetLastError();  // refused: fall back to one launch per round
+  }
+  unsigned long long* none = nullptr;
   for (int64_t t0 = 0; t0 < ntiles; t0 += cu_count) {
     const int64_t t1 = t0 + cu_count < ntiles ? t0 + cu_count : ntiles;
     const dim3 grid((unsigned)(t1 - t0));
     if (em == EM_BF16)
       hipLaunchKernelGGL((kprod_wide<NT, KD, EM_BF16, WAVES, RB>), grid, dim3(64 * WAVES), 0, s, P, Qf,
-                         Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, t1, t0);
+                         Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, t1, t0, none);
     else
       hipLaunchKernelGGL((kprod_wide<NT, KD, EM_F16, WAVES, RB>), grid, dim3(64 * WAVES), 0, s, P, Qf,
-                         Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, t1, t0);
+                         Af, y1, W, ldw, m, r, n_chunks, r_tiles, sc, nxb, t1, t0, none);
   }
+  return hipGetLastError();
 }
 
 // The one-term low-precision forms with more than 128 right-hand-side columns.
@@ -1170,7 +1270,8 @@ static tsvm_status product_wide(Ctx* c, KpBuffers& bufs, int em, int KD, const f
   const int64_t n_pad = round_up(n > 0 ? n : 1, 32), n_chunks = n_pad / 32;
   const int64_t m_pad = round_up(m, BR), r_pad = round_up(r, 32 * NT);
   const int64_t r_tiles = r_pad / 32, nxb = m_pad / BR;
-  void *P, *Qf, *Af, *scv = nullptr;
+  void *P, *Qf, *Af, *scv = nullptr, *schedv = nullptr;
+  TSVM_CHECK(bufs.get(c, "kp_sched", KP_SCHED_WORDS * 8, &schedv));
   TSVM_CHECK(bufs.get(c, "kp_P", m_pad * KD * 2, &P));
   TSVM_CHECK(bufs.get(c, "kp_Q", n_pad * KD * 2, &Qf));
   TSVM_CHECK(bufs.get(c, "kp_A", n_pad * r_pad * 2, &Af));
@@ -1198,10 +1299,13 @@ static tsvm_status product_wide(Ctx* c, KpBuffers& bufs, int em, int KD, const f
   }
   TSVM_HIP(c, hipGetLastError());
   const __bf16 *Pb = (const __bf16*)P, *Qb = (const __bf16*)Qf;
+  unsigned long long* sched = (unsigned long long*)schedv;
+  bool dyn = false;
+  hipError_t le = hipSuccess;
 #define KPW_CASE(kd)                                                                            \
   case kd:                                                                                      \
-    launch_wide<NT, kd, WAVES, RB>(em, Pb, Qb, Af, y1, W, ldw, m, r, n_chunks, r_tiles, nxb, sc,    \
-                               c->cu_count, s);                                                 \
+    le = launch_wide<NT, kd, WAVES, RB>(em, Pb, Qb, Af, y1, W, ldw, m, r, n_chunks, r_tiles,    \
+                                        nxb, sc, c->cu_count, sched, s, &dyn);                  \
     break;
   switch (KD) {
     KPW_CASE(16)
@@ -1216,7 +1320,14 @@ static tsvm_status product_wide(Ctx* c, KpBuffers& bufs, int em, int KD, const f
       return fail(c, TSVM_ERR_ARG, "kernel_product(wide): KD=%d not compiled", KD);
   }
 #undef KPW_CASE
-  TSVM_HIP(c, hipGetLastError());
+  TSVM_HIP(c, le);
+  if (dyn) {
+    // the schedule's error flag (a bounded wait that ran out)
+    unsigned long long flag = 0;
+    TSVM_HIP(c, hipMemcpyAsync(&flag, sched + KP_SCHED_WORDS - 1, 8, hipMemcpyDeviceToHost, s));
+    TSVM_HIP(c, hipStreamSynchronize(s));
+    if (flag != 0) return fail(c, TSVM_ERR_HIP, "kernel_product(wide): tile schedule wait timed out");
+  }
   return TSVM_OK;
 }
 

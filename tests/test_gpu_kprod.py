@@ -137,6 +137,25 @@ def test_kprod_wide_tiles_match_narrow(ctx, prec, d):
     assert np.array_equal(W[:, 128:], W1)
 
 
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_kprod_wide_xcd_schedule_matches_narrow(ctx, prec):
+    """With at least two rounds of 256-row x 256-column tiles (here 547 row blocks x 2
+    column tiles) the wide kernel runs as one cooperative launch whose blocks take units of
+    tiles per XCD from a device-wide counter; every tile is computed as in the one launch per
+    round, so each 128-column quarter is bit-identical to the 128-column kernel's."""
+    import tensorsvm_amd as T
+    p = T.PREC_BF16 if prec == "bf16" else T.PREC_FP16
+    m, n, d, r = 140_000, 2000, 18, 512
+    X1, y1, X2, y2, A = _case(m, n, d, r, 0.05, 321)
+    for j in range(4):
+        A[0, 128 * j] = 7.5
+    W = ctx.kernel_product(X1, y1, X2, y2, A, 0.05, p)
+    for j in range(4):
+        Wj = ctx.kernel_product(X1, y1, X2, y2, np.ascontiguousarray(A[:, 128 * j:128 * (j + 1)]),
+                                0.05, p)
+        assert np.array_equal(W[:, 128 * j:128 * (j + 1)], Wj), j
+
+
 @pytest.mark.parametrize("m,n,d,r", [(300, 257, 18, 40), (1000, 1000, 18, 512), (513, 700, 54, 100),
                                      (129, 64, 32, 256), (77, 33, 5, 7), (200, 3001, 41, 130)])
 def test_kprod_fp64(ctx, m, n, d, r):
