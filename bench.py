@@ -52,10 +52,10 @@ def log(*a):
 
 
 # HBM bytes per product of the factor's product kernel at the default workload (mean of its
-# two products, each the sum of its launches): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# two products, one dispatch each): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (KiB per dispatch) scaled by the factor that the same counters give for the bench's
 # 3.28 GB Omega copy (x2.0 and x1.0)
-KPROD_TRAFFIC = {"fetch_bytes": 408_445_491_968, "write_bytes": 3_366_400_000,
+KPROD_TRAFFIC = {"fetch_bytes": 401_741_402_112, "write_bytes": 3_304_685_568,
                  "source": "profiles/kprod_factor_c2_pmc.md"}
 
 
