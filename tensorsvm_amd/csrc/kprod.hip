@@ -603,12 +603,16 @@ __device__ __forceinline__ void mfma_bf16_vgpr_last(f32x16_t& d, bf16x8_t a, bf1
 // loads and LDS stores taken out (a timing copy, wrong results) the pipe is busy 75% but
 // the clock falls to 1.72 GHz (17% more MFMA work per second), and a register-only loop of
 // fp16 MFMAs on random operands runs at 1.64 PF/s, 66% of the 2.5 PF/s dense peak.
-// The block handles tiles tile0 + blockIdx.x, + gridDim.x, ... below ntiles; the host
-// launches one round of grid tiles at a time, so that the blocks of an XCD, which stream
-// the same chunks of A' and Q, start each tile together and share them in L2 (in one
-// persistent launch they drift apart over the rounds: at 1.6M x 1.6M x 512 the HBM fetch
-// grew 30-fold and the clock fell from 1.9 to 1.6 GHz).  The outputs are bit-identical to
-// the 128-column kernel's (same operands, same accumulation order).
+// The blocks of an XCD stream the same chunks of A' and Q and must start each tile
+// together to share them in L2 (in a persistent launch where every block walked its own
+// tiles they drifted apart: at 1.6M x 1.6M x 512 the HBM fetch grew 30-fold and the clock
+// fell from 1.9 to 1.6 GHz).  With sched, one cooperative launch runs the XCD-round
+// schedule (kp_next_unit): the blocks of an XCD take a unit of consecutive tiles together
+// and the next unit once all have finished, from a counter that every XCD draws on, so a
+// faster XCD takes more units.  Without it the block handles tiles tile0 + blockIdx.x,
+// + gridDim.x, ... below ntiles and the host launches one round of grid tiles at a time.
+// The outputs are bit-identical to the 128-column kernel's (same operands, same
+// accumulation order) on either schedule.
 // -------------------------------------------------------------------------------------
 // XCD-round schedule of the persistent wide launch.  Workgroups go to the eight XCDs
 // round-robin, so block b runs on XCD b % 8; the gs = gridDim.x / 8 blocks of an XCD take
