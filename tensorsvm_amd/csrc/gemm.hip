@@ -243,8 +243,8 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   return TSVM_OK;
 }
 
-// 256 threads; block owns 64 rows x 64 columns of Y, wave w rows [16w, 16w+16) x 64 columns
-// (4 MFMA tiles).  K (= ka) is walked 4 at a time.
+// Generic form: 256 threads; block owns 64 rows x 64 columns of Y, wave w rows
+// [16w, 16w+16) x 64 columns (4 MFMA tiles).  K (= ka) is walked 4 at a time.
 __global__ __launch_bounds__(256) void gemm_nn_tall_kernel(const float* __restrict__ A,
                                                            int64_t lda, int64_t ka,
                                                            const double* __restrict__ B,
@@ -288,12 +288,89 @@ __global__ __launch_bounds__(256) void gemm_nn_tall_kernel(const float* __restri
     }
 }
 
+// ka % 16 == 0, kb % 64 == 0, A rows 16-byte aligned: 256 threads; block owns 256 rows x 64
+// columns of Y, wave w rows [64w, 64w + 64): four 16-row blocks x four 16-column MFMA
+// tiles, so each B element loaded feeds four MFMAs.  K is walked 16 at a time: lane group
+// kq takes k0 + 4 kq .. + 3 (one 16-byte load of A per row block), MFMA s of the step
+// k0 + 4 kq + s.  One wave per SIMD (the 16 accumulators take 128 registers), so the
+// operands of step k0 + 16 are loaded into a second register set while step k0's MFMAs
+// run (loop unrolled by two).  Loads are branch-free: rows past n read row n - 1 (their
+// results are not stored), the prefetch past the last step reloads it.
+constexpr int GT_RB = 4;
+__global__ __launch_bounds__(256) void gemm_nn_tall_rb_kernel(const float* __restrict__ A,
+                                                              int64_t lda, int64_t ka,
+                                                              const double* __restrict__ B,
+                                                              int64_t kb, int64_t n,
+                                                              float* __restrict__ Y, int64_t ldy) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int64_t row0 = (int64_t)blockIdx.x * (64 * GT_RB) + wave * (16 * GT_RB);
+  const int64_t col0 = (int64_t)blockIdx.y * 64;
+  const float* pa[GT_RB];
+#pragma unroll
+  for (int rb = 0; rb < GT_RB; rb++) {
+    const int64_t arow = row0 + 16 * rb + l16;
+    pa[rb] = A + (arow < n ? arow : n - 1) * lda + 4 * kq;
+  }
+  const double* pb = B + (int64_t)(4 * kq) * kb + col0 + l16;
+  f64x4_t acc[GT_RB][4];
+#pragma unroll
+  for (int rb = 0; rb < GT_RB; rb++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[rb][j] = (f64x4_t){0, 0, 0, 0};
+  auto load = [&](int64_t k0, f32x4_t(&av)[GT_RB], double(&bv)[4][4]) {
+#pragma unroll
+    for (int sx = 0; sx < 4; sx++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[sx][j] = pb[(k0 + sx) * kb + 16 * j];
+#pragma unroll
+    for (int rb = 0; rb < GT_RB; rb++) av[rb] = *(const f32x4_t*)(pa[rb] + k0);
+  };
+  // the 16 accumulators in turn: an accumulator's next update is 16 MFMAs later
+  auto step = [&](const f32x4_t(&av)[GT_RB], const double(&bv)[4][4]) {
+#pragma unroll
+    for (int sx = 0; sx < 4; sx++)
+#pragma unroll
+      for (int rb = 0; rb < GT_RB; rb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          acc[rb][j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[rb][sx], bv[sx][j],
+                                                            acc[rb][j], 0, 0, 0);
+  };
+  f32x4_t aa[GT_RB], ab[GT_RB];
+  double ba[4][4], bb[4][4];
+  load(0, aa, ba);
+  for (int64_t k0 = 0; k0 < ka; k0 += 32) {
+    load(k0 + 16 < ka ? k0 + 16 : k0, ab, bb);
+    step(aa, ba);
+    if (k0 + 16 >= ka) break;
+    load(k0 + 32 < ka ? k0 + 32 : k0 + 16, aa, ba);
+    step(ab, bb);
+  }
+#pragma unroll
+  for (int rb = 0; rb < GT_RB; rb++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        // D: row lane / 16 + 4 q of the 16-row block, column lane % 16 of tile j
+        const int64_t r = row0 + 16 * rb + kq + 4 * q, col = col0 + 16 * j + l16;
+        if (r < n) Y[r * ldy + col] = (float)acc[rb][j][q];
+      }
+}
+
 tsvm_status gemm_nn_tall(Ctx* c, const float* A, int64_t lda, int64_t ka, const double* B,
                          int64_t kb, int64_t n, float* Y, int64_t ldy) {
   if (n <= 0 || kb <= 0) return TSVM_OK;
-  dim3 grid((unsigned)ceil_div(n, 64), (unsigned)ceil_div(kb, 64));
-  hipLaunchKernelGGL(gemm_nn_tall_kernel, grid, dim3(256), 0, c->stream, A, lda, ka, B, kb, n, Y,
-                     ldy);
+  if (ka % 16 == 0 && kb % 64 == 0 && lda % 4 == 0 && ((uintptr_t)A & 15) == 0) {
+    dim3 grid((unsigned)ceil_div(n, 64 * GT_RB), (unsigned)(kb / 64));
+    hipLaunchKernelGGL(gemm_nn_tall_rb_kernel, grid, dim3(256), 0, c->stream, A, lda, ka, B, kb,
+                       n, Y, ldy);
+  } else {
+    dim3 grid((unsigned)ceil_div(n, 64), (unsigned)ceil_div(kb, 64));
+    hipLaunchKernelGGL(gemm_nn_tall_kernel, grid, dim3(256), 0, c->stream, A, lda, ka, B, kb, n,
+                       Y, ldy);
+  }
   TSVM_HIP(c, hipGetLastError());
   return TSVM_OK;
 }
