@@ -92,7 +92,11 @@ tsvm_status tsvm_kernel_product(tsvm_ctx *ctx, const float *X1, const float *y1,
                                 const float *X2, const float *y2, int64_t n, int64_t d,
                                 const float *A, int64_t r, double gamma, int precision,
                                 float *W);
-/* Same with device pointers (enqueued on the context's stream, not synchronised). */
+/* Same with device pointers (enqueued on the context's stream, not synchronised, except
+ * that a product of at least two rounds of 256-row x 256-column tiles per CU, which runs
+ * as one cooperative launch whose blocks take tiles per XCD from a device counter, waits
+ * for its end to check that schedule; under stream capture it runs as one launch per
+ * round instead). */
 tsvm_status tsvm_kernel_product_dev(tsvm_ctx *ctx, const float *X1, const float *y1,
                                     int64_t m, const float *X2, const float *y2, int64_t n,
                                     int64_t d, const float *A, int64_t r, double gamma,
@@ -109,7 +113,7 @@ tsvm_status tsvm_kernel_product_dev(tsvm_ctx *ctx, const float *X1, const float 
  * 4(m + n)(d + 2) + 4 n r (+ 4 m r per 8192-column segment when n > 8192) in FP32.
  * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two
  * contexts or streams) race and must be serialised by the caller.  Enqueued on the
- * context's stream, not synchronised. */
+ * context's stream and synchronised as tsvm_kernel_product_dev. */
 #define TSVM_NOALLOC_SCRATCH_BYTES (64ull << 20)
 tsvm_status tsvm_kernel_product_noalloc_dev(tsvm_ctx *ctx, const float *X1, const float *y1,
                                             int64_t m, const float *X2, const float *y2,

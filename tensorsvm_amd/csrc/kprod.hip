@@ -622,24 +622,24 @@ __device__ __forceinline__ void mfma_bf16_vgpr_last(f32x16_t& d, bf16x8_t a, bf1
 // units, where one launch per round of tiles waits for the slowest XCD every round.
 // sched (zeroed before the launch): [0] next unit, [1 + x] (round << 32) | unit published
 // for XCD x, [1 + 8 + x] tiles finished on XCD x, [17] error flag.  The waits are bounded
-// (the launch is cooperative, so every block is resident and they end); a wait that runs
-// out sets the flag and the blocks leave the loop.
+// (the launch is cooperative, so every block is resident and they end; the bound, in polls
+// of the counter, is set from the tile's length with a wide margin); a wait that runs out
+// sets the flag and the blocks leave the loop.
 constexpr int KP_XCDS = 8;
 constexpr int KP_SCHED_WORDS = 2 + 2 * KP_XCDS;
-constexpr long long KP_SPIN_MAX = 1ll << 26;
 __device__ __forceinline__ unsigned long long kp_ld(unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // called by one thread per block; returns the unit of this round, or -1 after a failed wait
 __device__ long long kp_next_unit(unsigned long long* sched, int xcd, int slot, int gs,
-                                  long long round) {
+                                  long long round, long long spin_max) {
   unsigned long long* pub = sched + 1 + xcd;
   unsigned long long* done = sched + 1 + KP_XCDS + xcd;
   unsigned long long* err = sched + 1 + 2 * KP_XCDS;
   long long spins = 0;
   if (slot == 0) {
     while (kp_ld(done) < (unsigned long long)gs * (unsigned long long)round) {
-      if (++spins > KP_SPIN_MAX) {
+      if (++spins > spin_max) {
         atomicOr(err, 1ull);
         return -1;
       }
@@ -653,7 +653,7 @@ __device__ long long kp_next_unit(unsigned long long* sched, int xcd, int slot, 
   for (;;) {
     const unsigned long long v = kp_ld(pub);
     if ((v >> 32) >= (unsigned long long)(round + 1)) return (long long)(v & 0xffffffffull);
-    if (++spins > KP_SPIN_MAX) {
+    if (++spins > spin_max) {
       atomicOr(err, 1ull);
       return -1;
     }
@@ -706,7 +706,9 @@ __global__ __launch_bounds__(64 * WAVES, 1) void kprod_wide(const __bf16* __rest
   long long round = 0;
   for (int64_t tile = tile0 + blockIdx.x;; tile += gridDim.x) {
     if (dyn) {
-      if (tid == 0) s_unit = kp_next_unit(sched, xcd, slot, gs, round);
+      // the wait for a unit lasts about a tile (n_chunks chunks of 32 columns, some 2000
+      // cycles each): 2^20 polls plus 2^12 per chunk
+      if (tid == 0) s_unit = kp_next_unit(sched, xcd, slot, gs, round, (1ll << 20) + (n_chunks << 12));
       __syncthreads();
       const long long u = s_unit;
       __syncthreads();  // s_unit is rewritten next round
@@ -1222,9 +1224,10 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 }
 
 // One cooperative launch of cu_count blocks on the XCD-round schedule (sched: device words,
-// KP_SCHED_WORDS of them) when there are at least two rounds of tiles and cu_count splits
-// into the eight XCDs; otherwise (or if the cooperative launch is refused) one launch per
-// round of cu_count tiles (one block per CU).  *dyn_used tells which ran.
+// KP_SCHED_WORDS of them) when there are at least two rounds of tiles, cu_count splits into
+// the eight XCDs and the stream is not being captured; otherwise (or if the cooperative
+// launch is refused) one launch per round of cu_count tiles (one block per CU).  *dyn_used
+// tells which ran (the caller then waits for the product and checks the schedule's flag).
 template <int NT, int KD, int WAVES, int RB>
 static hipError_t launch_wide(int em, const __bf16* P, const __bf16* Qf, const void* Af,
                               const float* y1, float* W, int64_t ldw, int64_t m, int64_t r,
@@ -1233,7 +1236,10 @@ static hipError_t launch_wide(int em, const __bf16* P, const __bf16* Qf, const v
                               bool* dyn_used) {
   const int64_t ntiles = nxb * (r_tiles / NT);
   *dyn_used = false;
-  if (sched && cu_count % KP_XCDS == 0 && ntiles >= 2 * (int64_t)cu_count) {
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) != hipSuccess) cap = hipStreamCaptureStatusActive;
+  if (sched && cap == hipStreamCaptureStatusNone && cu_count % KP_XCDS == 0 &&
+      ntiles >= 2 * (int64_t)cu_count) {
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, sched, KP_SCHED_WORDS);
     int64_t t0 = 0;
     unsigned long long* sp = sched;
