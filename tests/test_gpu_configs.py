@@ -129,16 +129,20 @@ def test_c4_factor_k1024(ctx, c4_case, form):
     assert np.isclose(info.eig_max, w[-1], rtol=1e-4 if form == "fp32" else 1e-3)
 
 
-def test_c4_bf16_ipm_k1024_matches_oracle(ctx, c4_case):
+# refine = 2: two sweeps for each of the affine and the corrector's steps; -1 (the default):
+# one sweep for the combined step
+@pytest.mark.parametrize("refine", [2, -1])
+def test_c4_bf16_ipm_k1024_matches_oracle(ctx, c4_case, refine):
     """c4 shape: the BF16 interior-point form (hybrid capacitance at k = 1024, refinement)
     against the oracle's FP64 IPM on the same factor."""
     import tensorsvm_amd as T
     X, y, om, F, w = c4_case
     U, info = ctx.lra(X, y, 1024, 0.05, T.PREC_BF16, omega=om)
     C = 32.0
-    al, xi, s, ii = ctx.ipm(U, y, C, precision=T.PREC_BF16, refine=2, stop=T.capi.STOP_GAP, tol=1e-7)
+    al, xi, s, ii = ctx.ipm(U, y, C, precision=T.PREC_BF16, refine=refine, stop=T.capi.STOP_GAP,
+                            tol=1e-7)
     Xo, Xio, So, io = O.mpc(U.astype(np.float64), y.astype(np.float64), C, stop="gap", tol=1e-7)
-    parity_log("c4_ipm_bf16", dalpha_over_C=np.abs(al - Xo).max() / C,
+    parity_log(f"c4_ipm_bf16_refine{refine}", dalpha_over_C=np.abs(al - Xo).max() / C,
                dpobj_rel=abs(ii.pobj - io.pobj) / (1 + abs(io.pobj)), iters_gpu=ii.iters,
                iters_oracle=io.iters, bf16_row_frac=ii.bf16_row_frac)
     assert ii.reason == 0 and io.reason == 0
