@@ -198,7 +198,8 @@ def main():
                          "BF16 form with the FP8 e4m3 pass of the capacitance")
     ap.add_argument("--refine", type=int, default=-1,
                     help="refinement sweeps of the Newton steps (m > 0: m for the affine step and m "
-                         "for the corrector's; -m: m for the combined step only)")
+                         "for the corrector's, each solved first; -m: m for the combined step, "
+                         "started from the affine one, the first adding the corrector's step)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the trainer "

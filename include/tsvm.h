@@ -266,8 +266,10 @@ typedef struct {
                         capacitance (tsvm_weighted_syrk_dev, bulk rows on BF16 MFMA) +
                         refinement of the Newton step against the FP32 factor */
   int refine;        /* refinement sweeps of the bordered Newton system: m > 0 sweeps for each of
-                        the affine and the corrector's steps; -m (default -1): m sweeps for the
-                        combined step only, the affine step left as solved */
+                        the affine and the corrector's steps (each solved first); -m (default
+                        -1): the combined step starts from the affine one and m sweeps against
+                        the combined right-hand sides follow, the first adding the corrector's
+                        step (no solve of its own) */
   int verbose;       /* print the reference's iteration line every 5 iterations */
   int init;          /* 0: the reference's initial point for kernel_type; 1: the same x with
                         s, xi shifted so the initial dual residual is zero (what the linear
@@ -280,7 +282,9 @@ typedef struct {
   double t_total;    /* s */
   double t_syrk;     /* s spent building Z^T D^-1 Z (all iterations) */
   double syrk_tflops;/* algorithmic n k (k+1) flops / t_syrk */
-  double refine_res; /* last relative refinement residual of the bordered Newton system */
+  double refine_res; /* last relative refinement residual of the bordered Newton system (with
+                        refine = -1, the residual of the affine step against the combined
+                        right-hand sides: the corrector's relative size) */
   /* low-precision form: the capacitance's bulk rows on BF16 MFMA (hybrid split) */
   double t_syrk_bf16;      /* s spent in the BF16 part (all iterations) */
   double syrk_bf16_tflops; /* n k (k+1) flops per BF16 part / its average time */

@@ -12,19 +12,19 @@
 //                     reference's third solve is a linear combination (S^-1 is linear:
 //                     S^-1 (a dy - r5) = dy S^-1 a - S^-1 r5); step bound and mu_aff as
 //                     device reductions
-//   corrector         one SMW solve (S^-1 a is reused); the corrector's step is added to
-//                     the affine one and a sweep of bordered refinement corrects the
-//                     combined step: the residual of the bordered Newton system for the
-//                     combined right-hand sides against the exact operator D + Z Z^T (FP32
-//                     factor, FP64 accumulation), one SMW solve, the correction again
-//                     combined with S^-1 a; update.  (refine > 0 instead refines the affine
-//                     and the corrector's steps each.)
+//   corrector         the combined step starts from the affine one, and a sweep of
+//                     bordered refinement against the combined right-hand sides (the
+//                     residual of the bordered Newton system against the exact operator
+//                     D + Z Z^T, FP32 factor, FP64 accumulation; one SMW solve; the
+//                     correction again combined with S^-1 a) adds the corrector's step and
+//                     corrects the affine one in one solve; update.  (refine > 0 instead
+//                     solves the corrector's system on its own and refines each step.)
 // The last pass of each SMW solve also accumulates Z^T sol, so Z^T of the combined step is
 // a combination of k-vectors, and the bordered residual pass also accumulates Z^T of the
 // next solve's right-hand side; Z^T x is carried from iteration to iteration.  Per
-// iteration that is 6 reads of Z for 256 <= kp <= 768 (the residual pass, which also forms
-// the predictor's right-hand sides and their Z^T; the predictor's finish; the corrector's
-// Z^T and finish; the refinement's residual and finish), 8 with refine > 0.
+// iteration that is 4 reads of Z for 256 <= kp <= 768 (the residual pass, which also forms
+// the predictor's right-hand sides and their Z^T; the predictor's finish; the sweep's
+// residual and finish), 8 with refine = 1.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -917,21 +917,16 @@ __global__ void k_scale(int64_t n, int p, const double* __restrict__ rhs, int64_
 // r7 = S^-1 r5, b = S^-1 a are already at hand, so dx = dy b - r7 without another solve.
 // With Tr7 = Z^T r7 and Tb = Z^T b given (kp entries), block 0 also forms
 // T3 = Z^T dx = dy Tb - Tr7 for the bordered refinement.
-// corr = 2: the corrector's step is added to the affine one (adx, dy_aff, T3a = Z^T adx),
-// so sol3, the corrector's dy and T3 hold the combined step.
 __global__ void k_sol3_lin(int64_t n, int corr, const double* __restrict__ b,
                            const double* __restrict__ r7, double* __restrict__ sc,
                            double* __restrict__ sol3, int kp, const double* __restrict__ Tr7,
-                           const double* __restrict__ Tb, double* __restrict__ T3,
-                           const double* __restrict__ adx, const double* __restrict__ T3a) {
-  const bool comb = corr == 2;
+                           const double* __restrict__ Tb, double* __restrict__ T3) {
   const double r2 = corr ? 0.0 : -sc[SL_AX];
   const double dy = (r2 + sc[corr ? SL_AR7C : SL_AR7]) / sc[SL_AB];
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[corr ? SL_DYC : SL_DY] = comb ? dy + sc[SL_DY] : dy;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[corr ? SL_DYC : SL_DY] = dy;
   if (Tr7 && blockIdx.x == 0)
-    for (int e = threadIdx.x; e < kp; e += blockDim.x)
-      T3[e] = dy * Tb[e] - Tr7[e] + (comb ? T3a[e] : 0.0);
-  GRID_LOOP(i, n) sol3[i] = dy * b[i] - r7[i] + (comb ? adx[i] : 0.0);
+    for (int e = threadIdx.x; e < kp; e += blockDim.x) T3[e] = dy * Tb[e] - Tr7[e];
+  GRID_LOOP(i, n) sol3[i] = dy * b[i] - r7[i];
 }
 
 // bordered correction: (ddx, ddy) solve the same bordered system for the residual
@@ -1010,9 +1005,9 @@ __global__ __launch_bounds__(256) void k_muaff(int64_t n, double C, const double
 
 // corrector right-hand sides (1750-1757): keep the affine step, r3 = sigma mu - dx ds,
 // r4 = sigma mu + dx dxi, r1 = r2 = 0, rhs0 = r5 = -r3/x + r4/(C-x).
-// comb: also V = r5 / D (the corrector's solve), and r3, r4, rhs0 are left as the sums of
-// the affine right-hand sides (r3 = -s x, r4 = -xi (C - x), r5 in rhs0) and the
-// corrector's, the right-hand sides of the combined step.
+// comb: r3, r4, rhs0 are left as the sums of the affine right-hand sides (r3 = -s x,
+// r4 = -xi (C - x), r5 in rhs0) and the corrector's, the right-hand sides of the combined
+// step (and V = r5 / D of the corrector's part when V is given).
 __global__ void k_rhs_corr(int64_t n, double C, const double* __restrict__ x,
                            double* __restrict__ dx, double* __restrict__ ds, double* __restrict__ dxi,
                            double* __restrict__ adx, double* __restrict__ ads, double* __restrict__ adxi,
@@ -1033,7 +1028,7 @@ __global__ void k_rhs_corr(int64_t n, double C, const double* __restrict__ x,
     const double xv = x[i], cx = C - xv;
     const double r5 = -r3 / xv + r4 / cx;
     if (comb) {
-      V[i] = r5 * winv[i];
+      if (V) V[i] = r5 * winv[i];
       r3 -= s[i] * xv;
       r4 -= xi[i] * cx;
       rhs[i] += r5;
@@ -1074,6 +1069,19 @@ __global__ void k_carry_tx(int kp, double* __restrict__ Tx, const double* __rest
   if (*info != 0) return;
   const double al = 0.99 * sc[SL_ALPHA];
   for (int e = threadIdx.x; e < kp; e += blockDim.x) Tx[e] += al * ((T3a ? T3a[e] : 0.0) + T3c[e]);
+}
+
+// the combined step starting from the affine one: sol3 = dx_aff, T3 = Z^T dx_aff and the
+// corrector's dy slot = dy_aff; the refinement sweeps against the combined right-hand sides
+// then add the corrector's part
+__global__ void k_start_affine(int64_t n, const double* __restrict__ adx, double* __restrict__ sol3,
+                               int kp, const double* __restrict__ T3a, double* __restrict__ T3,
+                               double* __restrict__ sc) {
+  GRID_LOOP(i, n) sol3[i] = adx[i];
+  if (blockIdx.x == 0) {
+    for (int e = threadIdx.x; e < kp; e += blockDim.x) T3[e] = T3a[e];
+    if (threadIdx.x == 0) sc[SL_DYC] = sc[SL_DY];
+  }
 }
 
 // M += I (kp x kp) and copy lower to upper is not needed (Cholesky reads the lower part)
@@ -1242,18 +1250,21 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
   //   rho = [-r5 - (A dx - a dy); r2 - a^T dx];  (ddx, ddy) from the same Schur
   //   elimination; dx += ddx, dy += ddy.  T3 = Z^T sol3 comes in from the linear
   //   combination that formed sol3, so the residual pass reads Z once.
-  // refine = m > 0: m sweeps for the affine step and m for the corrector's; refine = -m:
-  // the affine step is left as solved (it only sets the centring sigma and the corrector's
-  // second-order term) and m sweeps refine the combined step, the one taken
+  // refine = m > 0: m sweeps for the affine step and m for the corrector's, each solved
+  // first.  refine = -m: the combined step (the one taken) starts from the affine step as
+  // solved and m sweeps against the combined right-hand sides follow (m + 1 near the stop):
+  // the first one's residual is the corrector's right-hand side plus the affine step's own
+  // residual, so its correction is the corrector's step and the affine step's refinement
+  // together (the corrector needs no solve of its own); further sweeps refine the sum.
   const bool comb = prm->refine < 0;
   const int border = comb ? -prm->refine : prm->refine;
   // With refinement, Z^T of each step is known from the passes that form it (T3 of the
   // predictor and of the corrector, the last sweep's Z^T s5 included), so Z^T x is carried
   // from iteration to iteration as Z^T x += alpha (T3a + T3) instead of read from Z again.
   const bool carry_tx = border >= 1;
-  auto border_refine = [&](int corr, double* T3v) -> tsvm_status {
-    for (int sw = 0; sw < border; sw++) {
-      const bool zt = sw + 1 < border || carry_tx;  // Z^T s5 wanted: T3v stays Z^T sol3
+  auto border_refine = [&](int corr, double* T3v, int nsw) -> tsvm_status {
+    for (int sw = 0; sw < nsw; sw++) {
+      const bool zt = sw + 1 < nsw || carry_tx;  // Z^T s5 wanted: T3v stays Z^T sol3
       // residual, V = r5' / D and Z^T V in one pass
       TSVM_CHECK(ipm_zpass(c, ZP_BORDER, 1, 1, Z, n, kp, T3v, nullptr, winv, a, D, rhs, sol3, sc,
                            corr, V, red, Tsol));
@@ -1378,6 +1389,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
       fflush(stderr);
     }
     bool conv;
+    const bool late = prm->stop_mode == TSVM_STOP_REF ? mu < 1e-5 : gap <= 100.0 * prm->tol;
     if (prm->stop_mode == TSVM_STOP_REF) {
       conv = mu < 1e-7 && normdx < 1e-7 && normdy < 1e-7;
     } else {
@@ -1429,9 +1441,8 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     }
     TSVM_CHECK(smw(2, solp, Tr7, {SL_AR7, SL_AB}, fuse_res));  // r7 = S^-1 r5, b = S^-1 a
     hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 0, (const double*)(solp + n), (const double*)solp,
-                       sc, sol3, kp, (const double*)(border ? Tr7 : nullptr), (const double*)Tb, T3a,
-                       (const double*)nullptr, (const double*)nullptr);
-    if (!comb) TSVM_CHECK(border_refine(0, T3a));
+                       sc, sol3, kp, (const double*)(border ? Tr7 : nullptr), (const double*)Tb, T3a);
+    if (!comb) TSVM_CHECK(border_refine(0, T3a, border));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, 0, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
                        (const double*)ads, (const double*)adxi, red);
@@ -1443,16 +1454,23 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     // ---- corrector (1750-1777)
     hipLaunchKernelGGL(k_rhs_corr, eg, eb, 0, st, n, C, (const double*)x, dx, ds, dxi, adx, ads, adxi,
                        (const double*)sc, rhs, comb ? 1 : 0, (const double*)s, (const double*)xi,
-                       (const double*)winv, V);
-    if (!comb)
+                       (const double*)winv, (double*)nullptr);
+    if (comb) {  // the sweeps below solve for the corrector's part
+      hipLaunchKernelGGL(k_start_affine, eg, eb, 0, st, n, (const double*)adx, sol3, kp,
+                         (const double*)T3a, T3, sc);
+    } else {
       hipLaunchKernelGGL(k_scale, eg, eb, 0, st, n, 1, (const double*)rhs, n, (const double*)winv, V, n);
-    TSVM_CHECK(smw(1, solc, border ? Tc : nullptr, {SL_AR7C}));
-    // comb: sol3, T3 and the corrector's dy become the combined step's, which the sweeps
-    // refine against the combined right-hand sides and k_step completes as one step
-    hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, comb ? 2 : 1, (const double*)(solp + n),
-                       (const double*)solc, sc, sol3, kp, (const double*)(border ? Tc : nullptr),
-                       (const double*)Tb, T3, (const double*)adx, (const double*)T3a);
-    TSVM_CHECK(border_refine(comb ? 2 : 1, T3));
+      TSVM_CHECK(smw(1, solc, border ? Tc : nullptr, {SL_AR7C}));
+      hipLaunchKernelGGL(k_sol3_lin, eg, eb, 0, st, n, 1, (const double*)(solp + n),
+                         (const double*)solc, sc, sol3, kp, (const double*)(border ? Tc : nullptr),
+                         (const double*)Tb, T3);
+    }
+    // refine = -m: one sweep more once the stop is near (relative gap within 100 times the
+    // tolerance, or mu below 1e-5 under the reference's rule): the first sweep leaves the
+    // corrector's step with the error of one solve on the factored capacitance, which the
+    // last iterations, with D at its most ill-conditioned, need refined
+    const int nsw = comb ? border + (late ? 1 : 0) : border;
+    TSVM_CHECK(border_refine(comb ? 2 : 1, T3, nsw));
     hipLaunchKernelGGL(k_step, eg, eb, 0, st, n, C, comb ? 0 : 1, (const double*)x, (const double*)s,
                        (const double*)xi, (const double*)sol3, dx, ds, dxi, (const double*)adx,
                        (const double*)ads, (const double*)adxi, red);

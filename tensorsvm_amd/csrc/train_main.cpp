@@ -43,7 +43,8 @@ void help_msg() {
       "-stop ref|gap : IPM stop rule (ref: mu,|r_x|,|r_y| < 1e-7; gap: relative gap <= -tol)\n"
       "-tol x : tolerance of -stop gap (default 1e-5)\n"
       "-refine n : refinement sweeps of the Newton steps: n > 0 for each of the affine and the\n"
-      "    corrector's steps, -n for the combined step only (default -1)\n"
+      "    corrector's steps, -n for the combined step, started from the affine one (the first\n"
+      "    sweep adds the corrector's step; default -1)\n"
       "-maxit n : IPM iteration limit (default 100)\n"
       "-init ref|feasible : IPM initial point (feasible: zero initial dual residual)\n"
       "-q : quiet\n"
