@@ -108,8 +108,9 @@ __global__ __launch_bounds__(256) void hs_hist_kernel(const double* __restrict__
 // BF16 pass, c1 <= bin < c2 the compensated (hi + lo bf16) rows, bin >= c2 FP64.
 enum { HC_C2 = 0, HC_NH = 1, HC_C8 = 2, HC_W8MAX = 3, HC_C1 = 4, HC_N2 = 5, HC_INTS = 8 };
 
-// one block of 8 HS_BINS threads: total counts per bin (8 groups of partials, summed in
-// a fixed order), then the cuts.  Each tier keeps a bound on its error:
+// one block of 1024 threads: total counts per bin (32 groups of 32 threads, each thread
+// summing four bins of every 32nd block's partials with 16-byte loads; integer sums, so
+// the order does not matter), then the cuts.  Each tier keeps a bound on its error:
 // * BF16 pass: sqrt(sum t^2) over its rows <= budget (error about 2^-9 of that);
 // * compensated rows: sqrt(sum t^2) <= budget 2^7 (error about 2^-16 of that: the same
 //   absolute error as the BF16 pass);
@@ -123,34 +124,47 @@ enum { HC_C2 = 0, HC_NH = 1, HC_C8 = 2, HC_W8MAX = 3, HC_C1 = 4, HC_N2 = 5, HC_I
 // from the smallest t up, then FP64.  x2 = false: no compensated rows.  slots: [0] bound
 // of the BF16 rows, [2] error bound of the FP8 rows, [3] their number, [4] bound of the
 // compensated rows ([1], [5]: the FP64 and compensated counts, from hs_scan_kernel).
-constexpr int HS_CUT_GROUPS = 8;
-__global__ __launch_bounds__(HS_CUT_GROUPS * HS_BINS) void hs_cutoff_kernel(
+constexpr int HS_CUT_GROUPS = 32;
+constexpr int HS_CUT_THREADS = HS_CUT_GROUPS * HS_BINS / 4;
+static_assert(HS_BINS % 4 == 0, "16-byte loads of the bin partials");
+__global__ __launch_bounds__(HS_CUT_THREADS) void hs_cutoff_kernel(
     const int* __restrict__ part, int nblocks, double budget, double eps8, bool x2, double r8,
     double r2, double r64, int* __restrict__ cut_out, double* __restrict__ slots) {
   __shared__ long long grp[HS_CUT_GROUPS][HS_BINS];
   __shared__ long long cnt[HS_BINS];
-  const int b = threadIdx.x % HS_BINS, g = threadIdx.x / HS_BINS;
-  long long s = 0;
+  const int qd = threadIdx.x % (HS_BINS / 4), g = threadIdx.x / (HS_BINS / 4);
+  long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll 8
-  for (int k = g; k < nblocks; k += HS_CUT_GROUPS) s += part[k * HS_BINS + b];
-  grp[g][b] = s;
+  for (int k = g; k < nblocks; k += HS_CUT_GROUPS) {
+    const int4 v = *(const int4*)&part[k * HS_BINS + 4 * qd];
+    s0 += v.x;
+    s1 += v.y;
+    s2 += v.z;
+    s3 += v.w;
+  }
+  grp[g][4 * qd] = s0;
+  grp[g][4 * qd + 1] = s1;
+  grp[g][4 * qd + 2] = s2;
+  grp[g][4 * qd + 3] = s3;
   __syncthreads();
-  if (g == 0) {
+  if (threadIdx.x < HS_BINS) {
     long long t = 0;
-    for (int q = 0; q < HS_CUT_GROUPS; q++) t += grp[q][b];
-    cnt[b] = t;
+    for (int q = 0; q < HS_CUT_GROUPS; q++) t += grp[q][threadIdx.x];
+    cnt[threadIdx.x] = t;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
   auto sq = [](int e) { return ldexp(1.0, 2 * (e - 63)); };  // bound of t^2 in bin e
-  // largest cut >= e0 with sqrt(sum over bins [e0, cut) of cnt 4^(e-63)) <= bud
+  // largest cut >= e0 with sqrt(sum over bins [e0, cut) of cnt 4^(e-63)) <= bud (compared
+  // squared: no square root per bin)
   auto cut_from = [&](int e0, double bud, double* cum_out) {
     double cum = 0.0;
     int cut = HS_BINS;
+    const double bsq = bud * bud;
     for (int e = e0; e < HS_BINS; e++) {
       if (cnt[e] == 0) continue;
       const double add = (double)cnt[e] * sq(e);
-      if (sqrt(cum + add) > bud) {
+      if (cum + add > bsq) {
         cut = e;
         break;
       }
@@ -1025,7 +1039,7 @@ tsvm_status hybrid_syrk_split(Ctx* c, HybridSyrk* hs, const double* w, double bu
   const int64_t rpb = round_up(ceil_div(hs->n_pad, HS_BLOCKS), 256);
   const int nb = (int)ceil_div(hs->n_pad, rpb);
   hipLaunchKernelGGL(hs_hist_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb, part);
-  hipLaunchKernelGGL(hs_cutoff_kernel, dim3(1), dim3(HS_CUT_GROUPS * HS_BINS), 0, s,
+  hipLaunchKernelGGL(hs_cutoff_kernel, dim3(1), dim3(HS_CUT_THREADS), 0, s,
                      (const int*)part, nb, budget, hs->fp8_eps, hs->x2, hs->r8, hs->r2, hs->r64,
                      cut, slots);
   hipLaunchKernelGGL(hs_count_kernel, dim3(nb), dim3(256), 0, s, w, hs->nz, n, rpb,
