@@ -72,7 +72,7 @@ tsvm_status tsvm_create(int device, tsvm_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) x->c.cu_count = prop.multiProcessorCount;
-  tsvm::warm_sym_eig(&x->c);
+  tsvm::warm_dense_la(&x->c);
   if (tsvm::noalloc_scratch_setup(&x->c) != TSVM_OK) {
     hipStreamDestroy(x->c.stream);
     delete x;

@@ -274,8 +274,9 @@ tsvm_status grad_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t 
       C[bb * k + a] = v;
     }
   {
-    std::vector<double> L = C;
-    if (host_chol(L.data(), k) != 0)
+    const int chol = dev_chol_inv_t(c, C.data(), k, 0.0, dkk);  // positive definiteness
+    if (chol < 0) return fail(c, TSVM_ERR_HIP, "grad: rocSOLVER Cholesky failed");
+    if (chol > 0)
       return fail(c, TSVM_ERR_NUMERIC,
                   "grad: W^T K_Y W is not positive definite (the factor clips eigenvalues)");
   }

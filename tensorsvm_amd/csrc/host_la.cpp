@@ -1,5 +1,6 @@
-// Small dense FP64 linear algebra on the host for the factor construction (k x k,
-// k <= a few thousand): Cholesky, triangular inverse, and the symmetric eigen-solver.
+// Small dense FP64 linear algebra of the factor construction (k x k, k <= a few
+// thousand): Cholesky with the triangular inverse, products, and the symmetric
+// eigen-solver, on the GPU through rocSOLVER / rocBLAS on the context's stream.
 //
 // The reference computes the core eigen-decomposition on the host with LAPACKE_dsyevd
 // (tensorsvm-train.cu:2191).  Here it is rocSOLVER's dsyevd on the GPU (k x k, once per
@@ -11,7 +12,6 @@
 #include <string.h>
 
 #include <algorithm>
-#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -19,25 +19,6 @@
 namespace tsvm {
 
 namespace {
-
-// parallel-for over [0, n) on up to nt threads (coarse chunks)
-template <class F>
-void pfor(int64_t n, F f) {
-  int nt = (int)std::thread::hardware_concurrency();
-  if (nt > 16) nt = 16;
-  if (n < 256 || nt <= 1) {
-    f(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  int64_t chunk = (n + nt - 1) / nt;
-  for (int t = 0; t < nt; t++) {
-    int64_t a = t * chunk, b = a + chunk < n ? a + chunk : n;
-    if (a >= b) break;
-    th.emplace_back([=]() { f(a, b); });
-  }
-  for (auto& x : th) x.join();
-}
 
 void jacobi_eig(const double* A0, int64_t k, double* w, double* V) {
   std::vector<double> A(A0, A0 + k * k);
@@ -93,13 +74,19 @@ const char* host_lapack_name() { return "rocsolver_dsyevd"; }
 
 // Symmetric eigen-decomposition of the k x k core matrix on the GPU (rocSOLVER dsyevd on
 // the context's stream); false if the library call fails.
-static bool gpu_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* V) {
+// the context's rocBLAS handle (created on first use, bound to the context's stream)
+static rocblas_handle blas_handle(Ctx* c) {
   if (!c->rocblas) {
     rocblas_handle h;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return false;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
     c->rocblas = (void*)h;
     rocblas_set_stream(h, c->stream);
   }
+  return (rocblas_handle)c->rocblas;
+}
+
+static bool gpu_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* V) {
+  if (!blas_handle(c)) return false;
   void *dA, *dD, *dE, *dI;
   if (c->ws.get("eig_A", (size_t)(k * k * 8), &dA) != hipSuccess ||
       c->ws.get("eig_D", (size_t)(k * 8), &dD) != hipSuccess ||
@@ -130,10 +117,57 @@ tsvm_status host_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* 
   return TSVM_OK;
 }
 
-// One eigen-decomposition of a 512 x 512 tridiagonal matrix, so that the rocBLAS handle
-// and rocSOLVER's kernels (about 0.17 s the first time in a process) are set up when the
-// context is created rather than inside the first factor construction.
-void warm_sym_eig(Ctx* c) {
+// A row-major k x k matrix read column-major is its transpose, so a symmetric matrix with
+// its row-major strict lower part zeroed is, for rocSOLVER, the lower triangle of the same
+// matrix with zeros above.  dpotrf (lower) then dtrtri (lower) leave L^{-1} in the column-
+// major lower triangle, which read row-major is L^{-T}, upper triangular, the zeros below
+// untouched (both routines reference only the lower triangle).
+int dev_chol_inv_t(Ctx* c, const double* A, int64_t k, double shift, double* dB) {
+  rocblas_handle h = blas_handle(c);
+  void* dI;
+  if (!h || c->ws.get("chol_inv_info", 16, &dI) != hipSuccess) return -1;
+  std::vector<double> M(k * k);
+  for (int64_t i = 0; i < k; i++)
+    for (int64_t j = 0; j < k; j++) M[i * k + j] = j < i ? 0.0 : A[i * k + j] + (i == j ? shift : 0.0);
+  rocblas_int info[2] = {-1, -1};
+  rocblas_int* di = (rocblas_int*)dI;
+  // every path waits for the stream before returning: M and info are read by queued copies
+  bool ok = hipMemcpyAsync(dB, M.data(), k * k * 8, hipMemcpyHostToDevice, c->stream) == hipSuccess &&
+            rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)k, dB, (rocblas_int)k, di) ==
+                rocblas_status_success &&
+            hipMemcpyAsync(&info[0], di, sizeof(rocblas_int), hipMemcpyDeviceToHost, c->stream) ==
+                hipSuccess;
+  if (hipStreamSynchronize(c->stream) != hipSuccess || !ok) return -1;
+  if (info[0] != 0) return (int)info[0];  // leading minor of order info[0] not positive definite
+  ok = rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, (rocblas_int)k, dB,
+                        (rocblas_int)k, di + 1) == rocblas_status_success &&
+       hipMemcpyAsync(&info[1], di + 1, sizeof(rocblas_int), hipMemcpyDeviceToHost, c->stream) ==
+           hipSuccess;
+  if (hipStreamSynchronize(c->stream) != hipSuccess || !ok || info[1] != 0) return -1;
+  return 0;
+}
+
+// Row-major C = op(A) op(B) is column-major C^T = op(B)^T op(A)^T: the same operands in the
+// other order, each with its own transpose flag.
+tsvm_status dev_gemm_rm(Ctx* c, bool ta, bool tb, const double* A, const double* B, int64_t k,
+                        double* C) {
+  rocblas_handle h = blas_handle(c);
+  if (!h) return fail(c, TSVM_ERR_HIP, "rocBLAS handle could not be created");
+  const double one = 1.0, zero = 0.0;
+  const rocblas_int kk = (rocblas_int)k;
+  if (rocblas_dgemm(h, tb ? rocblas_operation_transpose : rocblas_operation_none,
+                    ta ? rocblas_operation_transpose : rocblas_operation_none, kk, kk, kk, &one, B,
+                    kk, A, kk, &zero, C, kk) != rocblas_status_success)
+    return fail(c, TSVM_ERR_HIP, "rocblas_dgemm (k = %ld) failed", (long)k);
+  return TSVM_OK;
+}
+
+// The k x k library calls of the factor construction once at k = 512 (an eigen-
+// decomposition of a tridiagonal matrix, a Cholesky with its triangular inverse, and the
+// two product forms), so that the rocBLAS handle and the libraries' kernels (about 0.4 s
+// the first time in a process) are set up when the context is created rather than inside
+// the first factor construction.  The buffers are the factor construction's own.
+void warm_dense_la(Ctx* c) {
   const int64_t k = 512;
   std::vector<double> A(k * k, 0.0), w(k), V(k * k);
   for (int64_t i = 0; i < k; i++) {
@@ -141,86 +175,17 @@ void warm_sym_eig(Ctx* c) {
     if (i + 1 < k) A[i * k + i + 1] = A[(i + 1) * k + i] = 0.5;
   }
   (void)gpu_sym_eig(c, A.data(), k, w.data(), V.data());
-}
-
-int host_chol(double* M, int64_t k) {
-  const int64_t B = 64;
-  for (int64_t j0 = 0; j0 < k; j0 += B) {
-    const int64_t jb = j0 + B < k ? B : k - j0;
-    for (int64_t j = j0; j < j0 + jb; j++) {  // diagonal block, unblocked
-      double s = M[j * k + j];
-      for (int64_t t = j0; t < j; t++) s -= M[j * k + t] * M[j * k + t];
-      if (!(s > 0.0)) return (int)(j + 1);
-      double l = sqrt(s);
-      M[j * k + j] = l;
-      for (int64_t i = j + 1; i < j0 + jb; i++) {
-        double v = M[i * k + j];
-        for (int64_t t = j0; t < j; t++) v -= M[i * k + t] * M[j * k + t];
-        M[i * k + j] = v / l;
-      }
-    }
-    const int64_t r0 = j0 + jb;
-    if (r0 >= k) break;
-    pfor(k - r0, [&](int64_t a, int64_t b) {  // panel: L_i = A_i L_jj^{-T}
-      for (int64_t i = r0 + a; i < r0 + b; i++) {
-        double* Mi = M + i * k;
-        for (int64_t j = j0; j < j0 + jb; j++) {
-          double v = Mi[j];
-          const double* Mj = M + j * k;
-          for (int64_t t = j0; t < j; t++) v -= Mi[t] * Mj[t];
-          Mi[j] = v / Mj[j];
-        }
-      }
-    });
-    pfor(k - r0, [&](int64_t a, int64_t b) {  // trailing update (lower part)
-      for (int64_t i = r0 + a; i < r0 + b; i++) {
-        double* Mi = M + i * k;
-        for (int64_t l = r0; l <= i; l++) {
-          const double* Ml = M + l * k;
-          double v = 0.0;
-          for (int64_t t = j0; t < j0 + jb; t++) v += Mi[t] * Ml[t];
-          Mi[l] -= v;
-        }
-      }
-    });
-  }
-  for (int64_t i = 0; i < k; i++)
-    for (int64_t j = i + 1; j < k; j++) M[i * k + j] = 0.0;
-  return 0;
-}
-
-// Linv = L^{-1} (both lower, row-major): column c solves L x = e_c
-void host_tri_inv_lower(const double* L, int64_t k, double* Linv) {
-  for (int64_t i = 0; i < k * k; i++) Linv[i] = 0.0;
-  pfor(k, [&](int64_t a, int64_t b) {
-    std::vector<double> x(k);
-    for (int64_t c = a; c < b; c++) {
-      for (int64_t i = 0; i < c; i++) x[i] = 0.0;
-      for (int64_t i = c; i < k; i++) {
-        double v = (i == c) ? 1.0 : 0.0;
-        const double* Li = L + i * k;
-        for (int64_t t = c; t < i; t++) v -= Li[t] * x[t];
-        x[i] = v / Li[i];
-      }
-      for (int64_t i = c; i < k; i++) Linv[i * k + c] = x[i];
-    }
-  });
-}
-
-// C = A * B (k x k, row-major), parallel over rows
-void host_gemm(const double* A, const double* B, int64_t k, double* C) {
-  pfor(k, [&](int64_t a, int64_t b) {
-    for (int64_t i = a; i < b; i++) {
-      double* Ci = C + i * k;
-      for (int64_t j = 0; j < k; j++) Ci[j] = 0.0;
-      for (int64_t t = 0; t < k; t++) {
-        double ait = A[i * k + t];
-        if (ait == 0.0) continue;
-        const double* Bt = B + t * k;
-        for (int64_t j = 0; j < k; j++) Ci[j] += ait * Bt[j];
-      }
-    }
-  });
+  void *pLt, *pA, *pT;
+  if (c->ws.get("lra_Lt", (size_t)(k * k * 8), &pLt) != hipSuccess ||
+      c->ws.get("lra_A", (size_t)(k * k * 8), &pA) != hipSuccess ||
+      c->ws.get("lra_T", (size_t)(k * k * 8), &pT) != hipSuccess)
+    return;
+  if (dev_chol_inv_t(c, A.data(), k, 0.0, (double*)pLt) != 0) return;
+  if (hipMemcpyAsync(pA, A.data(), k * k * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) return;
+  const tsvm_status st1 = dev_gemm_rm(c, true, false, (double*)pLt, (double*)pA, k, (double*)pT);
+  const tsvm_status st2 = dev_gemm_rm(c, false, false, (double*)pT, (double*)pLt, k, (double*)pA);
+  (void)hipStreamSynchronize(c->stream);
+  if (st1 != TSVM_OK || st2 != TSVM_OK) c->last_error.clear();
 }
 
 }  // namespace tsvm

@@ -182,14 +182,19 @@ tsvm_status chol_solve_f64(Ctx* c, const double* L, int64_t k, double* B, int64_
 tsvm_status masked_colsum(Ctx* c, const float* U, int64_t n, int64_t k, int64_t ldu,
                           const double* alpha, double thr, double* v_host);
 
-// ---- host small dense FP64 linear algebra (host_la.cpp) ---------------------------
-// Symmetric eigen-decomposition (ascending), V column i = eigenvector i, row-major.
+// ---- small dense FP64 linear algebra, k x k (host_la.cpp) ---------------------------
+// Symmetric eigen-decomposition (ascending), V column i = eigenvector i, row-major; A, w
+// and V on the host.
 tsvm_status host_sym_eig(Ctx* c, const double* A, int64_t k, double* w, double* V);
-void warm_sym_eig(Ctx* c);  // first-use set-up of the eigen-solver (context creation)
+void warm_dense_la(Ctx* c);  // first-use set-up of the k x k library calls (context creation)
 const char* host_lapack_name();
-int host_chol(double* M, int64_t k);  // lower, in place; 0 or failing column + 1
-void host_tri_inv_lower(const double* L, int64_t k, double* Linv);
-void host_gemm(const double* A, const double* B, int64_t k, double* C);  // k x k
+// dB (device, row-major) = L^{-T}, upper triangular, for A + shift I = L L^T (A symmetric,
+// row-major, on the host).  0; j > 0 if the leading minor of order j is not positive
+// definite; -1 if a library call failed.  Synchronises the stream.
+int dev_chol_inv_t(Ctx* c, const double* A, int64_t k, double shift, double* dB);
+// C = op(A) op(B), row-major k x k device matrices (op = transpose where the flag is set)
+tsvm_status dev_gemm_rm(Ctx* c, bool ta, bool tb, const double* A, const double* B, int64_t k,
+                        double* C);
 
 // ---- factor construction (lra.cpp) ------------------------------------------------
 // U (n x k, FP32, device) with U U^T ~ K_Y; Om (n x k, device) is consumed (reused as

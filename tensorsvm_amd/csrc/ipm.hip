@@ -30,6 +30,8 @@
 #include <string.h>
 
 #include <chrono>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -159,7 +161,7 @@ static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int
                              const double* T, const double* V, const double* winv,
                              const double* a, const double* D, const double* r5, const double* dx,
                              const double* sc, int corr, double* out, double* red, double* Tout,
-                             const ResArgs* ra = nullptr);
+                             const ResArgs* ra = nullptr, int* nred = nullptr);
 
 // ------------------------------------------------------------------------------------
 // Z^T V: block b owns a row range; thread t handles float4 column group t % (kp/4) of
@@ -539,12 +541,11 @@ __device__ inline double swap_sum16(double a, double b) {
   return f64_from(lo[0], hi[0]) + f64_from(lo[1], hi[1]);
 }
 
-// The passes for kp >= 256, where a wave owns whole rows.  Each wave takes a contiguous
-// chunk of rows (a multiple of ZP_U), ZP_U consecutive rows per step, so that the per-row
-// operands and results of a step share cache lines.  Each lane holds F float4 groups of
-// the step's rows.  The P x ZP_U partial products are summed over the wave by a
-// reduce-scatter (lane bit 5 keeps half of the set, bit 4 half of the rest, ... by
-// permlane swaps and DPP moves, no LDS), after which lane l holds product
+// The passes for kp >= 256, where a wave owns whole rows.  Each wave takes ZP_U consecutive
+// rows per step, so that the per-row operands and results of a step share cache lines.
+// Each lane holds F float4 groups of the step's rows.  The P x ZP_U partial products are
+// summed over the wave by a reduce-scatter (lane bit 5 keeps half of the set, bit 4 half
+// of the rest, ... by permlane swaps and DPP moves, no LDS), after which lane l holds product
 // m = l >> (6 - log2(P ZP_U)) (q = m / ZP_U, row u = m % ZP_U) and has loaded that row's
 // operands itself, so the row-wise update runs for all rows at once.  The updated values
 // return to every lane by readlane for the Z^T accumulation.  Besides ZP_SMW and
@@ -610,11 +611,13 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
   }
   const double dy = MODE == ZP_BORDER ? sc[corr ? SL_DYC : SL_DY] : 0.0;
   const double yeq = MODE == ZP_RES ? sc[SL_Y] : 0.0;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + wid;
-  const int64_t nw = (int64_t)gridDim.x * 4;
-  const int64_t chunk = ceil_div(ceil_div(n, nw), ZP_U) * ZP_U;
-  const int64_t r0 = wave * chunk;
-  const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
+  // The grid sweeps Z as one front: a block step is 4 ZP_U consecutive rows (the block's
+  // waves take ZP_U each), block b takes block steps b, b + grid, ...  (measured at
+  // n = 1.6M, kp = 512 against contiguous row ranges per wave: 1-3% faster)
+  constexpr int64_t SB = 4 * ZP_U;
+  const int64_t r0 = (int64_t)blockIdx.x * SB + wid * ZP_U;
+  const int64_t r1 = n;
+  const int64_t stride = (int64_t)gridDim.x * SB;
   // Software pipeline: the rows and per-row operands of the next step are loaded before
   // the current step's products, reductions and updates run (two register sets, the loop
   // unrolled by two so that they alternate without copies)
@@ -731,31 +734,38 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
     }
     if constexpr (Q > 0) {
 #pragma unroll
-      for (int q = 0; q < Q; q++)
+      for (int u = 0; u < ZP_U; u++) {
+        double uq[Q];
 #pragma unroll
-        for (int u = 0; u < ZP_U; u++) {
+        for (int q = 0; q < Q; q++)
           // ZP_RES: both values of row u sit in its lane group
-          const double uq =
-              base + u < r1
-                  ? (MODE == ZP_RES ? readlane_f64(q == 0 ? w : w1, u * GL)
-                                    : readlane_f64(w, (q * ZP_U + u) * GL))
-                  : 0.0;
+          uq[q] = base + u < r1 ? (MODE == ZP_RES ? readlane_f64(q == 0 ? w : w1, u * GL)
+                                                  : readlane_f64(w, (q * ZP_U + u) * GL))
+                                : 0.0;
 #pragma unroll
-          for (int f = 0; f < F; f++)
+        for (int f = 0; f < F; f++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[q][f][j] += uq * (double)z[u][f][j];
-        }
+          for (int j = 0; j < 4; j++) {
+            // converted again rather than kept from the dot products: the FP64 copies of
+            // the step's rows would otherwise stay live across the reduction (64 VGPRs)
+            float zf = z[u][f][j];
+            asm("" : "+v"(zf));
+            const double zd = (double)zf;
+#pragma unroll
+            for (int q = 0; q < Q; q++) acc[q][f][j] += uq[q] * zd;
+          }
+      }
     }
   };
   f32x4_t za[ZP_U][F], zb[ZP_U][F];
   double oa[NO], ob[NO];
   if (r0 < r1) load_step(r0, za, oa);
-  for (int64_t base = r0; base < r1; base += 2 * ZP_U) {
-    if (base + ZP_U < r1) load_step(base + ZP_U, zb, ob);
+  for (int64_t base = r0; base < r1; base += 2 * stride) {
+    if (base + stride < r1) load_step(base + stride, zb, ob);
     step(base, za, oa);
-    if (base + ZP_U >= r1) break;
-    if (base + 2 * ZP_U < r1) load_step(base + 2 * ZP_U, za, oa);
-    step(base + ZP_U, zb, ob);
+    if (base + stride >= r1) break;
+    if (base + 2 * stride < r1) load_step(base + 2 * stride, za, oa);
+    step(base + stride, zb, ob);
   }
   if constexpr (Q > 0) {
     const int half = wid & 1;
@@ -782,13 +792,34 @@ __global__ __launch_bounds__(256) void zpass_wide_kernel(
   if constexpr (RED) block_reduce_store<R>(rv, op, red);
 }
 
-static int zpass_blocks(Ctx* c) { return c->cu_count * 4 < EW_BLOCKS ? c->cu_count * 4 : EW_BLOCKS; }
+// Blocks of a pass over Z: as many as the CUs hold at once (at most four per CU, from the
+// kernel's register and LDS demand), so that every wave's chunk of rows is in flight from
+// the start and no second round of blocks runs behind the first.
+static int zpass_blocks(Ctx* c, const void* fn, size_t sm) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, size_t>, int> occ_of;
+  int occ = 0;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = occ_of.find({fn, sm});
+    if (it != occ_of.end()) occ = it->second;
+  }
+  if (occ == 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, sm) != hipSuccess || occ < 1)
+      occ = 1;
+    if (occ > 4) occ = 4;
+    std::lock_guard<std::mutex> lk(mu);
+    occ_of[{fn, sm}] = occ;
+  }
+  const int nb = c->cu_count * occ;
+  return nb < EW_BLOCKS ? nb : EW_BLOCKS;
+}
 
 static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int64_t n, int kp,
                              const double* T, const double* V, const double* winv,
                              const double* a, const double* D, const double* r5, const double* dx,
                              const double* sc, int corr, double* out, double* red, double* Tout,
-                             const ResArgs* ra) {
+                             const ResArgs* ra, int* nred) {
   const int G = kp / 4;
   const bool wide = G >= 64;
   bool ok_shape = false;
@@ -802,15 +833,8 @@ static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int
   int L = 64;
   while (L > G) L >>= 1;
   const int F = (int)ceil_div(G, L);  // wide: <= 4, else <= 2
-  const int nblk = zpass_blocks(c);
-  void* part = nullptr;
-  if (q > 0) TSVM_HIP(c, c->ws.get("zpass_part", (size_t)nblk * q * kp * 8, &part));
-  const size_t sm = (size_t)2 * q * kp * 8;
-  const dim3 g(nblk), b(256);
-  const ResArgs rargs = ra ? *ra : ResArgs{};
-#define ZK(KN, MO, PP, QQ, FF)                                                             \
-  hipLaunchKernelGGL((KN<MO, PP, QQ, FF>), g, b, sm, c->stream, Z, n, kp, T, V, winv, a, D, \
-                     r5, dx, sc, corr, out, red, (double*)part, rargs)
+  const void* fn = nullptr;
+#define ZK(KN, MO, PP, QQ, FF) fn = (const void*)&KN<MO, PP, QQ, FF>
 #define ZW(MO, PP, QQ)                                         \
   if (F == 1) ZK(zpass_wide_kernel, MO, PP, QQ, 1);            \
   else if (F == 2) ZK(zpass_wide_kernel, MO, PP, QQ, 2);       \
@@ -845,7 +869,22 @@ static tsvm_status ipm_zpass(Ctx* c, int mode, int p, int q, const float* Z, int
 #undef ZF
 #undef ZW
 #undef ZK
-  TSVM_HIP(c, hipGetLastError());
+  const size_t sm = (size_t)2 * q * kp * 8;
+  const int nblk = zpass_blocks(c, fn, sm);
+  if (nred) *nred = nblk;
+  double* part = nullptr;
+  if (q > 0) {
+    // sized for the largest grid and two vectors, so that it is allocated once
+    const size_t mx = (size_t)(c->cu_count * 4 < EW_BLOCKS ? c->cu_count * 4 : EW_BLOCKS);
+    void* pv;
+    TSVM_HIP(c, c->ws.get("zpass_part", (nblk > (int)mx ? (size_t)nblk : mx) * 2 * kp * 8, &pv));
+    part = (double*)pv;
+  }
+  ResArgs rargs = ra ? *ra : ResArgs{};
+  void* args[] = {(void*)&Z,  (void*)&n,   (void*)&kp,  (void*)&T,  (void*)&V,     (void*)&winv,
+                  (void*)&a,  (void*)&D,   (void*)&r5,  (void*)&dx, (void*)&sc,    (void*)&corr,
+                  (void*)&out, (void*)&red, (void*)&part, (void*)&rargs};
+  TSVM_HIP(c, hipLaunchKernel(fn, dim3(nblk), dim3(256), args, sm, c->stream));
   if (q > 0) TSVM_CHECK(reduce_parts(c, (const double*)part, nblk, q * kp, Tout));
   return TSVM_OK;
 }
@@ -1219,7 +1258,6 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     hipLaunchKernelGGL(reduce_slots, dim3(1), dim3(256), 0, st, (const double*)red, nblk, rs, sc);
     return TSVM_OK;
   };
-  const int zblk = zpass_blocks(c);
   // the fused residual pass runs on the wide-row kernel; above kp = 768 (four float4 groups
   // per lane) its register demand spills, and the separate passes are kept
   const bool fuse_res = kp >= 256 && kp <= 768;
@@ -1241,9 +1279,10 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
                  bool have_tv = false) -> tsvm_status {
     if (!have_tv) TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, V, n, pp, Tsol));
     TSVM_CHECK(chol_solve_f64(c, M, kp, Tsol, pp, dinfo));
+    int nr = 0;
     TSVM_CHECK(ipm_zpass(c, ZP_SMW, pp, Tz ? pp : 0, Z, n, kp, Tsol, V, winv, a, nullptr, nullptr,
-                         nullptr, sc, 0, sol, red, Tz));
-    return reduce(std::vector<int>(slots.size(), RED_SUM), slots, zblk);
+                         nullptr, sc, 0, sol, red, Tz, nullptr, &nr));
+    return reduce(std::vector<int>(slots.size(), RED_SUM), slots, nr);
   };
 
   // Refinement of the bordered Newton system for the step in sol3 (r5 in rhs row 0):
@@ -1266,14 +1305,15 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     for (int sw = 0; sw < nsw; sw++) {
       const bool zt = sw + 1 < nsw || carry_tx;  // Z^T s5 wanted: T3v stays Z^T sol3
       // residual, V = r5' / D and Z^T V in one pass
+      int nr = 0;
       TSVM_CHECK(ipm_zpass(c, ZP_BORDER, 1, 1, Z, n, kp, T3v, nullptr, winv, a, D, rhs, sol3, sc,
-                           corr, V, red, Tsol));
-      TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_R2B, SL_BRES, SL_BRHS}, zblk));
+                           corr, V, red, Tsol, nullptr, &nr));
+      TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM}, {SL_R2B, SL_BRES, SL_BRHS}, nr));
       // s5 = S^-1 r5'; the correction is a combination of s5 and b = S^-1 a
       TSVM_CHECK(chol_solve_f64(c, M, kp, Tsol, 1, dinfo));
       TSVM_CHECK(ipm_zpass(c, ZP_SMW, 1, zt ? 1 : 0, Z, n, kp, Tsol, V, winv, a, nullptr,
-                           nullptr, nullptr, sc, 0, solc, red, zt ? Tc : nullptr));
-      TSVM_CHECK(reduce({RED_SUM}, {SL_AR7B}, zblk));
+                           nullptr, nullptr, sc, 0, solc, red, zt ? Tc : nullptr, nullptr, &nr));
+      TSVM_CHECK(reduce({RED_SUM}, {SL_AR7B}, nr));
       hipLaunchKernelGGL(k_border_lin, eg, eb, 0, st, n, corr, (const double*)(solp + n),
                          (const double*)solc, sc, sol3, kp, (const double*)(zt ? Tc : nullptr),
                          (const double*)Tb, T3v);
@@ -1320,10 +1360,11 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     // ---- residual pass (1606-1643); with wide rows it also forms the predictor's scaled
     // right-hand sides [r5, a]/D and their Z^T (into Tsol) from the same read of Z
     if (!carry_tx || iter == 0) TSVM_CHECK(ipm_gemv_t(c, Z, n, kp, x, n, 1, Tx));
+    int nres = EW_BLOCKS;
     if (fuse_res) {
       const ResArgs ra{x, s, xi, dx, ds, dxi, D, winv, rhs, V, C};
       TSVM_CHECK(ipm_zpass(c, ZP_RES, 1, 2, Z, n, kp, Tx, nullptr, nullptr, a, nullptr, nullptr,
-                           nullptr, sc, 0, nullptr, red, Tsol, &ra));
+                           nullptr, sc, 0, nullptr, red, Tsol, &ra, &nres));
     } else {
       TSVM_CHECK(ipm_gemv_n(c, Z, n, kp, Tx, 1, e, n));
       hipLaunchKernelGGL(k_resid, eg, eb, 0, st, n, C, (const double*)x, (const double*)s,
@@ -1332,7 +1373,7 @@ tsvm_status ipm_dev(Ctx* c, const float* Zin, const float* af, int64_t n, int64_
     }
     TSVM_CHECK(reduce({RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_SUM, RED_MAX, RED_MIN},
                       {SL_MU, SL_NDX2, SL_AX, SL_XQX, SL_SX, SL_SXI, SL_L1, SL_DMAX, SL_DMIN},
-                      fuse_res ? zblk : EW_BLOCKS));
+                      nres));
     // hybrid capacitance: split the rows for this iteration's weights (the FP64 row
     // count comes back with the scalars)
     if (bf16) TSVM_CHECK(hybrid_syrk_split(c, &hs, winv, IPM_SYRK_BUDGET, sc + SL_HSB));

@@ -8,9 +8,9 @@
 //                                         range(W), which the two-term form keeps ~2^-11
 //                                         smaller)
 //   2. W <- orth(W)                       shifted Cholesky-QR (Fukaya et al., SIAM J. Sci.
-//                                         Comput. 2020): G = W^T W on FP64 MFMA, host
-//                                         Cholesky of G + s I, W <- W L^{-T}; repeated until
-//                                         G = I to 1e-6 (at most 3 passes)
+//                                         Comput. 2020): G = W^T W on FP64 MFMA, Cholesky
+//                                         of G + s I and L^{-T} (rocSOLVER), W <- W L^{-T};
+//                                         repeated until G = I to 1e-6 (at most 3 passes)
 //   3. Gp = W^T W                         Gram of the FP32 basis as stored
 //   4. Q = K_Y W                          second kernel product (BF16 form: the fp16 product;
 //                                         the basis itself stays FP32 -- rounding it would
@@ -19,7 +19,8 @@
 //                                         only perturbs C inside span(W), where the
 //                                         trailing block sees ~u sigma_tail)
 //   5. Cb = W^T Q (FP64), symmetrised     (the reference's C, 2122-2156)
-//   6. host: Gp = L L^T, C = L^-1 Cb L^-T = V diag(w) V^T (dsyevd), X = L^-T V sqrt(w+)
+//   6. k x k on the GPU (rocSOLVER / rocBLAS): Gp = L L^T, C = L^-1 Cb L^-T = V diag(w) V^T
+//      (dsyevd), X = L^-T V sqrt(w+)
 //      -- with Wo = W L^-T orthonormal this is exactly the reference's C = Wo^T K_Y Wo,
 //      so U U^T = Wo C+ Wo^T is the projected kernel even where W is only nearly
 //      orthonormal
@@ -43,8 +44,9 @@ double now() {
 }  // namespace
 
 // Shifted Cholesky-QR of the n x k FP32 block W (in place; tmp is an n x k scratch):
-// G = W^T W on FP64 MFMA, host Cholesky of G + s I, W <- W L^{-T}, repeated until G = I
-// to 1e-6 (at most 3 passes).  On return G holds the Gram of the basis as stored.
+// G = W^T W on FP64 MFMA, Cholesky of G + s I and L^{-T} on the GPU, W <- W L^{-T},
+// repeated until G = I to 1e-6 (at most 3 passes).  On return G holds the Gram of the
+// basis as stored.
 tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
                         std::vector<double>& G, int* passes) {
   hipStream_t s = c->stream;
@@ -54,7 +56,6 @@ tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
   double* dG = (double*)pG;
   double* dB = (double*)pB;
   G.assign(k * k, 0.0);
-  std::vector<double> L(k * k), Li(k * k), T(k * k);
   float* cur = W;
   float* oth = tmp;
   bool have_gram = false;
@@ -76,17 +77,13 @@ tsvm_status cholqr_orth(Ctx* c, float* W, float* tmp, int64_t n, int64_t k,
     // shift (pass 0 only) keeps the Cholesky of an ill-conditioned Gram defined
     double shift = pass == 0 ? 11.0 * ((double)n * k + (double)k * (k + 1)) * 1.11e-16 * tr : 0.0;
     int fails = 0;
-    for (;;) {
-      memcpy(L.data(), G.data(), k * k * 8);
-      for (int64_t i = 0; i < k; i++) L[i * k + i] += shift;
-      if (host_chol(L.data(), k) == 0) break;
+    for (;;) {  // dB = L^{-T}
+      const int r = dev_chol_inv_t(c, G.data(), k, shift, dB);
+      if (r < 0) return fail(c, TSVM_ERR_HIP, "cholqr: rocSOLVER Cholesky / inverse failed");
+      if (r == 0) break;
       shift = shift > 0 ? shift * 10.0 : 1e-14 * tr;
       if (++fails > 30) return fail(c, TSVM_ERR_NUMERIC, "cholqr: Cholesky failed");
     }
-    host_tri_inv_lower(L.data(), k, Li.data());
-    for (int64_t i = 0; i < k; i++)  // B = L^{-T}
-      for (int64_t j = 0; j < k; j++) T[i * k + j] = Li[j * k + i];
-    TSVM_HIP(c, hipMemcpyAsync(dB, T.data(), k * k * 8, hipMemcpyHostToDevice, s));
     TSVM_CHECK(gemm_nn_tall(c, cur, k, k, dB, k, n, oth, k));
     std::swap(cur, oth);
     have_gram = false;
@@ -145,7 +142,7 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   float* W = (float*)pW;
   double* dG = (double*)pG;
   double* dB = (double*)pB;
-  std::vector<double> G(k * k), L(k * k), Li(k * k), T(k * k);
+  std::vector<double> G(k * k), T(k * k);
 
   // 1. W = K_Y Omega (timed with events on the stream it runs on)
   Event e0, e1;
@@ -188,17 +185,25 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
   const double tc0 = now();
   for (int64_t i = 0; i < k; i++)
     for (int64_t j = 0; j < k; j++) Cs[i * k + j] = 0.5 * (Cb[i * k + j] + Cb[j * k + i]);
-  // 6. C = L^-1 Cs L^-T, eig, X = L^-T V sqrt(w+)
-  memcpy(L.data(), G.data(), k * k * 8);
-  std::vector<double> LiT(k * k);
+  // 6. C = L^-1 Cs L^-T, eig, X = L^-T V sqrt(w+) (the k x k products on the GPU: dLt =
+  // L^-T, dA and dT scratch, X into dB)
+  void *pLt, *pA, *pT;
+  TSVM_HIP(c, c->ws.get("lra_Lt", (size_t)(k * k * 8), &pLt));
+  TSVM_HIP(c, c->ws.get("lra_A", (size_t)(k * k * 8), &pA));
+  TSVM_HIP(c, c->ws.get("lra_T", (size_t)(k * k * 8), &pT));
+  double* dLt = (double*)pLt;
+  double* dA = (double*)pA;
+  double* dT = (double*)pT;
   double tc1, tc2;
   int64_t realk = k;
-  if (host_chol(L.data(), k) == 0) {
-    host_tri_inv_lower(L.data(), k, Li.data());
-    host_gemm(Li.data(), Cs.data(), k, T.data());  // T = L^-1 Cs
-    for (int64_t i = 0; i < k; i++)
-      for (int64_t j = 0; j < k; j++) LiT[i * k + j] = Li[j * k + i];
-    host_gemm(T.data(), LiT.data(), k, Cb.data());  // Cb = L^-1 Cs L^-T
+  const int chol = dev_chol_inv_t(c, G.data(), k, 0.0, dLt);
+  if (chol < 0) return fail(c, TSVM_ERR_HIP, "lra: rocSOLVER Cholesky / inverse failed");
+  if (chol == 0) {
+    TSVM_HIP(c, hipMemcpyAsync(dA, Cs.data(), k * k * 8, hipMemcpyHostToDevice, s));
+    TSVM_CHECK(dev_gemm_rm(c, true, false, dLt, dA, k, dT));   // T = L^-1 Cs
+    TSVM_CHECK(dev_gemm_rm(c, false, false, dT, dLt, k, dA));  // L^-1 Cs L^-T
+    TSVM_HIP(c, hipMemcpyAsync(Cb.data(), dA, k * k * 8, hipMemcpyDeviceToHost, s));
+    TSVM_HIP(c, hipStreamSynchronize(s));
     for (int64_t i = 0; i < k; i++)
       for (int64_t j = 0; j < i; j++) {
         double v = 0.5 * (Cb[i * k + j] + Cb[j * k + i]);
@@ -214,7 +219,8 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
       else realk--;
       for (int64_t i = 0; i < k; i++) V[i * k + j] *= sc;
     }
-    host_gemm(LiT.data(), V.data(), k, T.data());  // X = L^-T V sqrt(w+)
+    TSVM_HIP(c, hipMemcpyAsync(dA, V.data(), k * k * 8, hipMemcpyHostToDevice, s));
+    TSVM_CHECK(dev_gemm_rm(c, false, false, dLt, dA, k, dB));  // X = L^-T V sqrt(w+)
   } else {
     // Rank-deficient sketch (gamma = 0 gives K = 1 1^T; duplicated rows): range(K_Y Omega)
     // has dimension r < k and the basis Gram is singular.  With G = Vg diag(g) Vg^T and the
@@ -267,11 +273,11 @@ tsvm_status lra_dev(Ctx* c, const float* X, const float* y, int64_t n, int64_t d
           for (int64_t t = 0; t < r; t++) acc += B[i * r + t] * Vr[t * r + (j - r0)];
         T[i * k + j] = acc;
       }
+    TSVM_HIP(c, hipMemcpyAsync(dB, T.data(), k * k * 8, hipMemcpyHostToDevice, s));
   }
   inf.realk = realk;
   inf.eig_max = w[k - 1];
   inf.eig_min = w[0];
-  TSVM_HIP(c, hipMemcpyAsync(dB, T.data(), k * k * 8, hipMemcpyHostToDevice, s));
   double t4 = now();
   inf.t_core = t4 - t3;
   if (c->progress) {
