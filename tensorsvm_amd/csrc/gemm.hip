@@ -39,8 +39,18 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     int64_t rows_per_split, double* __restrict__ part, const int* __restrict__ rows = nullptr) {
   constexpr int CG = TILE / 64;  // 64-column groups of a staged row
   constexpr int NI = TILE / 32;  // MFMA tiles per wave along each side
-  const int ta = blockIdx.x, tb = blockIdx.y;
-  if (lower_only && tb > ta) return;
+  // lower_only: blockIdx.x = t enumerates the lower tiles row by row, (ta, tb) with
+  // ta (ta + 1) / 2 + tb = t, tb <= ta; the grid holds no upper tiles (a grid with idle
+  // upper tiles leaves the same XCDs without work in every split: blocks go to the XCDs
+  // round-robin by their linear index)
+  int ta = blockIdx.x, tb = blockIdx.y;
+  if (lower_only) {
+    const int t = blockIdx.x;
+    ta = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((ta + 1) * (ta + 2) / 2 <= t) ta++;
+    while (ta * (ta + 1) / 2 > t) ta--;
+    tb = t - ta * (ta + 1) / 2;
+  }
   __shared__ double sA[2][TN_R][TILE];
   __shared__ double sB[2][TN_R][TILE];
   const int64_t split = blockIdx.z;
@@ -212,7 +222,8 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   nsplit = ceil_div(n > 0 ? n : 1, rps);
   void* part;
   TSVM_HIP(c, c->ws.get("gemm_part", (size_t)(nsplit * ka * kb * 8), &part));
-  dim3 grid((unsigned)ta, (unsigned)tb, (unsigned)nsplit);
+  const dim3 grid = lower_only ? dim3((unsigned)tiles, 1u, (unsigned)nsplit)
+                               : dim3((unsigned)ta, (unsigned)tb, (unsigned)nsplit);
   // whole tiles with 16-byte rows: vector loads without column masks
   const bool vec = ka % tile == 0 && kb % tile == 0 && lda % 4 == 0 && ldb % 4 == 0;
 #define GTN(V, W, I)                                                                              \
