@@ -315,8 +315,15 @@ __global__ __launch_bounds__(256) void gemm_nn_tall_rb_kernel(const float* __res
                                                               float* __restrict__ Y, int64_t ldy) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
-  const int64_t row0 = (int64_t)blockIdx.x * (64 * GT_RB) + wave * (16 * GT_RB);
-  const int64_t col0 = (int64_t)blockIdx.y * 64;
+  // 1-D grid, XCD-aware: blocks go to the 8 XCDs round-robin by index, and an XCD's
+  // consecutive blocks take the column tiles of one row block in turn, so the row block of
+  // A (256 rows) is read from HBM once and from the XCD's L2 for the other column tiles
+  const int64_t ncb = kb / 64, nrb = (n + 64 * GT_RB - 1) / (64 * GT_RB);
+  const int64_t bid = blockIdx.x, j = bid / 8;
+  const int64_t rblk = (j / ncb) * 8 + bid % 8;
+  if (rblk >= nrb) return;
+  const int64_t row0 = rblk * (64 * GT_RB) + wave * (16 * GT_RB);
+  const int64_t col0 = (j % ncb) * 64;
   const float* pa[GT_RB];
 #pragma unroll
   for (int rb = 0; rb < GT_RB; rb++) {
@@ -374,7 +381,9 @@ tsvm_status gemm_nn_tall(Ctx* c, const float* A, int64_t lda, int64_t ka, const 
                          int64_t kb, int64_t n, float* Y, int64_t ldy) {
   if (n <= 0 || kb <= 0) return TSVM_OK;
   if (ka % 16 == 0 && kb % 64 == 0 && lda % 4 == 0 && ((uintptr_t)A & 15) == 0) {
-    dim3 grid((unsigned)ceil_div(n, 64 * GT_RB), (unsigned)(kb / 64));
+    // row blocks rounded up to whole rounds of the 8 XCDs (the blocks past n return at once)
+    const int64_t nrb = ceil_div(n, 64 * GT_RB);
+    const dim3 grid((unsigned)(round_up(nrb, 8) * (kb / 64)));
     hipLaunchKernelGGL(gemm_nn_tall_rb_kernel, grid, dim3(256), 0, c->stream, A, lda, ka, B, kb,
                        n, Y, ldy);
   } else {
