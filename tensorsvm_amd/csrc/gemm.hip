@@ -36,16 +36,23 @@ template <bool VEC, bool HAS_W, bool IDX = false, int TILE = 64>
 __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
     const float* __restrict__ A, int64_t lda, int64_t ka, const float* __restrict__ B,
     int64_t ldb, int64_t kb, int64_t n, const double* __restrict__ w, int lower_only,
-    int64_t rows_per_split, double* __restrict__ part, const int* __restrict__ rows = nullptr) {
+    int64_t rows_per_split, int64_t nsplit, double* __restrict__ part,
+    const int* __restrict__ rows = nullptr) {
   constexpr int CG = TILE / 64;  // 64-column groups of a staged row
   constexpr int NI = TILE / 32;  // MFMA tiles per wave along each side
-  // lower_only: blockIdx.x = t enumerates the lower tiles row by row, (ta, tb) with
-  // ta (ta + 1) / 2 + tb = t, tb <= ta; the grid holds no upper tiles (a grid with idle
-  // upper tiles leaves the same XCDs without work in every split: blocks go to the XCDs
-  // round-robin by their linear index)
-  int ta = blockIdx.x, tb = blockIdx.y;
+  // 1-D grid, XCD-aware: blocks go to the 8 XCDs round-robin by index, and an XCD's
+  // consecutive blocks take the tiles of one row split in turn, so the split's rows are
+  // read from HBM once and from the XCD's L2 for its other tiles.  lower_only: tile t
+  // enumerates the lower tiles row by row, (ta, tb) with ta (ta + 1) / 2 + tb = t,
+  // tb <= ta (no upper tiles in the grid: idle ones would unbalance the XCDs).
+  const int nta = (int)((ka + TILE - 1) / TILE), ntb = (int)((kb + TILE - 1) / TILE);
+  const int64_t nt = lower_only ? (int64_t)nta * (nta + 1) / 2 : (int64_t)nta * ntb;
+  const int64_t bid = blockIdx.x, jx = bid / 8;
+  const int64_t split = (jx / nt) * 8 + bid % 8;
+  if (split >= nsplit) return;
+  const int t = (int)(jx % nt);
+  int ta = t % nta, tb = t / nta;
   if (lower_only) {
-    const int t = blockIdx.x;
     ta = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while ((ta + 1) * (ta + 2) / 2 <= t) ta++;
     while (ta * (ta + 1) / 2 > t) ta--;
@@ -53,7 +60,6 @@ __global__ __launch_bounds__(256) void gemm_tn_f64_kernel(
   }
   __shared__ double sA[2][TN_R][TILE];
   __shared__ double sB[2][TN_R][TILE];
-  const int64_t split = blockIdx.z;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
@@ -222,18 +228,20 @@ tsvm_status gemm_tn_f64(Ctx* c, const float* A, int64_t lda, int64_t ka, const f
   nsplit = ceil_div(n > 0 ? n : 1, rps);
   void* part;
   TSVM_HIP(c, c->ws.get("gemm_part", (size_t)(nsplit * ka * kb * 8), &part));
-  const dim3 grid = lower_only ? dim3((unsigned)tiles, 1u, (unsigned)nsplit)
-                               : dim3((unsigned)ta, (unsigned)tb, (unsigned)nsplit);
+  // whole rounds of the 8 XCDs (the splits past nsplit return at once)
+  const dim3 grid((unsigned)(round_up(nsplit, 8) * tiles));
   // whole tiles with 16-byte rows: vector loads without column masks
   const bool vec = ka % tile == 0 && kb % tile == 0 && lda % 4 == 0 && ldb % 4 == 0;
 #define GTN(V, W, I)                                                                              \
   do {                                                                                          \
     if (tile == 128)                                                                            \
       hipLaunchKernelGGL((gemm_tn_f64_kernel<V, W, I, 128>), grid, dim3(256), 0, c->stream, A,   \
-                         lda, ka, B, ldb, kb, n, w, lower_only ? 1 : 0, rps, (double*)part, rows); \
+                         lda, ka, B, ldb, kb, n, w, lower_only ? 1 : 0, rps, nsplit,             \
+                         (double*)part, rows);                                                  \
     else                                                                                        \
       hipLaunchKernelGGL((gemm_tn_f64_kernel<V, W, I, 64>), grid, dim3(256), 0, c->stream, A,    \
-                         lda, ka, B, ldb, kb, n, w, lower_only ? 1 : 0, rps, (double*)part, rows); \
+                         lda, ka, B, ldb, kb, n, w, lower_only ? 1 : 0, rps, nsplit,             \
+                         (double*)part, rows);                                                  \
   } while (0)
   if (rows) {
     // the FP64 rows of the hybrid capacitance: rows read in place through the index list
