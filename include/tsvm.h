@@ -64,7 +64,9 @@ typedef struct tsvm_ctx tsvm_ctx;
 /* Context: device ordinal, one HIP stream, reusable device workspace.
  * Replaces the per-call cublasCreate/cusolverDnCreate/cudaStreamCreate handles of
  * LRA (tensorsvm-train.cu:1946-1961) and mpc (1509-1513).  Creation also sets up the
- * eigen-solver library (rocBLAS handle, rocSOLVER kernels; ~0.2 s once per process). */
+ * k x k library calls of the factor construction (rocBLAS handle; rocSOLVER dsyevd,
+ * dpotrf, dtrtri and rocBLAS dgemm run once so that their kernels are loaded; ~0.4 s once
+ * per process). */
 tsvm_status tsvm_create(int device, tsvm_ctx **out);
 void tsvm_destroy(tsvm_ctx *ctx);
 const char *tsvm_last_error(const tsvm_ctx *ctx);
